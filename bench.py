@@ -1,0 +1,189 @@
+"""HPL benchmark of the MI355X LU solve (P=Q=1 HPL_pdgesv path).
+
+One step = one timed HPL_pdgesv (factor + back-solve) of the seeded HPL
+matrix, timed as the reference times it (src/HPL_pdtest.cpp:128-136): the
+matrix is generated before the timed region and the residual check runs after
+it.  Every step must pass the HPL scaled-residual test (< 16).
+
+Default workload: N=193280, NB=896 in FP64 on one GPU (the [A|b] matrix fills
+278.4 GiB of HBM).  Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+# host BLAS threads for the CPU baseline; set before numpy/scipy load it
+_CORES = int(os.environ.get("HPLX_CPU_CORES", "16"))
+os.environ.setdefault("OPENBLAS_NUM_THREADS", str(_CORES))
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("HPL-equiv TFLOP/s at residual<16, 1 GPU: FP64 LU vs BF16/FP8 "
+          "LU+FP64 IR")
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "trailing_update_traffic.json")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def hpl_flops(n):
+    return (2.0 / 3.0) * n ** 3 + 2.0 * n ** 2
+
+
+def ref_flops(n):
+    return (2.0 / 3.0) * n ** 3 + 1.5 * n ** 2
+
+
+def cpu_baseline(n_sample, nb_sample, cores):
+    """The oracle (a C restatement of the same P=Q=1 algorithm) on host
+    OpenBLAS, on a bounded sample of the workload."""
+    from oracle import oracle as o
+    desc = o.use_host_blas(True)
+    t0 = time.perf_counter()
+    r = o.pdgesv(n_sample, nb_sample)
+    t = time.perf_counter() - t0
+    res = o.residual(n_sample, r["x"])
+    o.use_host_blas(False)
+    return {"value": hpl_flops(n_sample) / t / 1e12, "unit": "TFLOP/s",
+            "cores": cores, "kind": "port",
+            "sample": "one P=Q=1 solve N=%d NB=%d NBMIN=32 NDIV=2 (oracle "
+                      "hpl_oracle.c, dgemm/dtrsm from %s, %d threads), "
+                      "%.2f s, resid %.3f" % (n_sample, nb_sample, desc, cores,
+                                              t, res["resid"])}
+
+
+def load_traffic(n, nb):
+    if not os.path.exists(PROFILE_TRAFFIC):
+        return None
+    try:
+        d = json.load(open(PROFILE_TRAFFIC))
+    except ValueError:
+        return None
+    if d.get("N") == n and d.get("NB") == nb:
+        return d.get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--n", type=int, default=193280)
+    ap.add_argument("--nb", type=int, default=896)
+    ap.add_argument("--frac", type=float, default=0.6)
+    ap.add_argument("--cpu-n", type=int, default=16384)
+    ap.add_argument("--cpu-nb", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check-pivots", type=int, default=0,
+                    help="compare the first K pivots with LAPACK dgetrf on the "
+                         "N x K column prefix (prefix oracle)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU (got --gpus %d); "
+              "multi-GPU runs are not part of it" % args.gpus, flush=True)
+        sys.exit(2)
+
+    import numpy as np
+
+    import rochpl_amd as rp
+
+    n, nb = args.n, args.nb
+    ctx = rp.Context(0)
+    palg = rp.palg_default()
+    palg.frac = args.frac
+
+    # kernel pre-warm on a small matrix (cf. HPL_pdwarmup); not a step
+    wn = min(n, 4096)
+    wmat = ctx.pmat_alloc(wn, min(nb, wn))
+    ctx.pdrandmat(wn, wn + 1, wmat.nb, wmat.A, wmat.ld, rp.HPL_ISEED, 0.0)
+    ctx.pdgesv(palg, wmat)
+    ctx.pmat_free(wmat)
+
+    log("allocating N=%d NB=%d (%.1f GiB)" % (n, nb, rp.hpl_ld(n) * (n + 1)
+                                               * 8 / 2 ** 30))
+    mat = ctx.pmat_alloc(n, nb)
+    times, checks, stats = [], [], []
+    for step in range(args.warmup + args.steps):
+        ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        info = ctx.pdgesv(palg, mat)
+        t = time.perf_counter() - t0
+        if info != 0:
+            raise SystemExit("pdgesv: exact zero pivot at column %d" % info)
+        st = ctx.update_stats()
+        chk = ctx.pdtest_check(mat)
+        timed = step >= args.warmup
+        log("%s step %d: %.3f s  %.2f TFLOP/s  resid %.4f  trailing GEMM "
+            "%.2f TFLOP/s over %d launches" % (
+                "timed" if timed else "warmup", step, t,
+                hpl_flops(n) / t / 1e12, chk["resid"],
+                st[2] / (st[0] * 1e-3) / 1e12 if st[0] > 0 else 0.0, st[1]))
+        if not chk["resid"] < 16.0:
+            raise SystemExit("residual check FAILED: %r" % chk)
+        if timed:
+            times.append(t)
+            checks.append(chk)
+            stats.append(st)
+
+    pivots = None
+    if args.check_pivots > 0:
+        import scipy.linalg.lapack as la
+
+        from oracle import oracle as o
+        k = min(args.check_pivots, n)
+        log("prefix pivot check: first %d pivots vs LAPACK dgetrf" % k)
+        gpiv = ctx.download_vec(mat.ipiv, k, np.int32)
+        pref = np.asfortranarray(o.matgen_tile(n, 0, 0, n, k))
+        _, lpiv, _ = la.dgetrf(pref, overwrite_a=1)
+        pivots = {"prefix": k, "equal": bool(np.array_equal(gpiv, lpiv[:k]))}
+    ctx.pmat_free(mat)
+    ctx.close()
+
+    t_mean = sum(times) / len(times)
+    value = hpl_flops(n) / t_mean / 1e12
+    gemm_ms = sum(s[0] for s in stats)
+    gemm_launch = sum(s[1] for s in stats)
+    gemm_flops = sum(s[2] for s in stats)
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    traffic = load_traffic(n, nb)
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_mean * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: HPL 64-bit LCG matgen, seed 100, uniform "
+                "[-0.5,0.5], b = column N",
+        "config": {"workload": "hpl_pdgesv_fp64_p1q1", "N": n, "NB": nb,
+                   "P": 1, "Q": 1, "pfact": "right", "rfact": "right",
+                   "nbmin": 32, "ndiv": 2, "depth": 1, "swap": "long",
+                   "frac": args.frac},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": traffic,
+                     "kernel": "trailing-update FP64 MFMA GEMM "
+                               "(A22 -= L21*U12), %d launches, %.1f ms"
+                               % (gemm_launch, gemm_ms)},
+        "residual": round(max(c["resid"] for c in checks), 6),
+        "residual_r0_form": round(max(c["resid_r0_form"] for c in checks), 6),
+        "tflops_ref_formula": round(ref_flops(n) / t_mean / 1e12, 3),
+        "pivot_prefix_check": pivots,
+    }
+    if not args.no_cpu:
+        log("CPU baseline sample N=%d" % args.cpu_n)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_nb, _CORES)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,203 @@
+/*
+ * hplx.h -- C ABI of rochpl_amd, an MI355X-native LU-with-partial-pivoting
+ * solver that drops in for rocHPL's P=Q=1 HPL_pdgesv path.
+ *
+ * Conventions
+ *   - Every entry point returns int: HPLX_SUCCESS (0), a negative HPLX_ERR_*
+ *     code, or (hplx_pdgesv / hplx_pdfact) a positive LAPACK-style column
+ *     index of the first exactly-zero pivot.  The library never calls exit();
+ *     the text of the last error is kept per context (hplx_last_error).
+ *   - Matrices are column-major doubles in device memory (HBM); sizes and
+ *     leading dimensions are plain ints as in the reference, offsets are
+ *     computed in 64 bits.
+ *   - All device work is queued on the context's stream (hplx_set_stream);
+ *     entry points that return host values synchronise that stream.
+ *   - Indices are 0-based.  A pivot vector ipiv[i] = r means "row i was
+ *     interchanged with row r" (LAPACK semantics, 0-based), exactly what the
+ *     reference's HPL_pdmxswp writes into ipiv (src/pfact/HPL_pdmxswp.cpp:121).
+ */
+#ifndef HPLX_H
+#define HPLX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HPLX_SUCCESS 0
+#define HPLX_ERR_INVALID -1     /* bad argument                            */
+#define HPLX_ERR_UNSUPPORTED -2 /* valid HPL setting this build lacks      */
+#define HPLX_ERR_HIP -3         /* HIP runtime error                       */
+#define HPLX_ERR_ALLOC -4       /* device or pinned allocation failed      */
+
+/* ---- enums with the reference's values --------------------------------- */
+/* HPL_T_FACT, include/hpl_pgesv.hpp:39-43 */
+#define HPLX_LEFT_LOOKING 301
+#define HPLX_CROUT 302
+#define HPLX_RIGHT_LOOKING 303
+/* HPL_T_TOP, include/hpl_comm.hpp:31-38 (accepted, no-op on one GPU) */
+#define HPLX_1RING 401
+#define HPLX_1RING_M 402
+#define HPLX_2RING 403
+#define HPLX_2RING_M 404
+#define HPLX_BLONG 405
+#define HPLX_BLONG_M 406
+/* HPL_T_SWAP, include/hpl_pgesv.hpp:45-51 */
+#define HPLX_SWAP00 451
+#define HPLX_SWAP01 452
+#define HPLX_SW_MIX 453
+#define HPLX_NO_SWP 499
+
+/* ---- algorithm parameters: mirrors HPL_T_palg (include/hpl_pgesv.hpp:63-81)
+ * The function-pointer plug points pffun/rffun/upfun are selected by the
+ * pfact/rfact/L1notran/Unotran values; only the reference's P=Q=1
+ * configuration (pfact=rfact=HPLX_RIGHT_LOOKING, L1 and U transposed,
+ * depth 1, long swap, equil 0) is implemented, anything else returns
+ * HPLX_ERR_UNSUPPORTED from hplx_pdgesv. */
+typedef struct hplx_palg {
+  int    btopo;    /* row broadcast topology (no-op at Q=1)        */
+  int    depth;    /* look-ahead depth (must be 1, src/HPL_pdinfo.cpp:930-935) */
+  int    nbdiv;    /* recursive division factor (>= 2)            */
+  int    nbmin;    /* recursion stopping criterium (>= 1)         */
+  int    pfact;    /* panel fact variant                          */
+  int    rfact;    /* recursive fact variant                      */
+  int    fswap;    /* swapping algorithm                          */
+  int    fsthr;    /* swapping threshold                          */
+  int    equil;    /* equilibration (must be 0)                   */
+  int    L1notran; /* 0: L1 transposed (required)                 */
+  int    Unotran;  /* 0: U transposed (required)                  */
+  int    align;    /* memory alignment in doubles                 */
+  double frac;     /* update split fraction (kept; P=1 ignores it) */
+  int    its;      /* iterations of the timed solve               */
+} hplx_palg;
+
+/* Fill *palg with the reference defaults (src/HPL_pdinfo.cpp:487-562):
+ * right/right, nbmin 32, nbdiv 2, depth 1, long swap, thresh 64, frac 0.3. */
+void hplx_palg_default(hplx_palg* palg);
+
+/* ---- distributed matrix: mirrors HPL_T_pmat (include/hpl_pgesv.hpp:83-109)
+ * A is the augmented [A|b], n x (n+1), leading dimension ld, b = column n.
+ * ipiv and info are new: the reference returns neither. */
+typedef struct hplx_pmat {
+  double* A;    /* device: ld x (n+1)                                    */
+  double* X;    /* device: n, solution after hplx_pdgesv                  */
+  int*    ipiv; /* device: n, global 0-based pivot rows                   */
+  int     n;    /* global problem size                                   */
+  int     nb;   /* blocking factor                                       */
+  int     ld;   /* leading dimension = ((n+95)/128)*128+32                */
+  int     mp;   /* local rows (= n at P=Q=1)                              */
+  int     nq;   /* local columns (= n+1)                                  */
+  int     info; /* 0, or 1-based column of the first exactly-zero pivot  */
+  void*   ws;   /* library-owned device workspace (opaque)                */
+} hplx_pmat;
+
+typedef struct hplx_ctx hplx_ctx;
+
+/* ---- context (replaces HPL_InitGPU / HPL_FreeGPU, src/HPL_InitGPU.cpp:29-130:
+ * device selection, streams, events; no rocBLAS handle) ------------------ */
+int         hplx_ctx_create(int device, hplx_ctx** ctx);
+int         hplx_ctx_destroy(hplx_ctx* ctx);
+const char* hplx_last_error(const hplx_ctx* ctx);
+/* Queue all work on this stream (a hipStream_t; NULL = the context's own). */
+int         hplx_set_stream(hplx_ctx* ctx, void* stream);
+int         hplx_synchronize(hplx_ctx* ctx);
+/* Number of compute units of the context's device. */
+int         hplx_num_cus(hplx_ctx* ctx);
+
+/* ---- plain device memory helpers (for callers without a runtime of their
+ * own, e.g. ctypes tests) ------------------------------------------------- */
+int hplx_malloc(hplx_ctx* ctx, size_t bytes, void** dptr);
+int hplx_free(hplx_ctx* ctx, void* dptr);
+int hplx_memcpy_h2d(hplx_ctx* ctx, void* dst, const void* src, size_t bytes);
+int hplx_memcpy_d2h(hplx_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- allocation: HPL_pdmatgen / HPL_pdmatfree
+ * (src/matgen/HPL_pdmatgen.cpp:85-436).  Allocates A|b with the reference's
+ * ld padding, X, ipiv and the solver workspace in one place; the solver
+ * itself never allocates. */
+int hplx_pmat_alloc(hplx_ctx* ctx, int n, int nb, hplx_pmat* mat);
+int hplx_pmat_free(hplx_ctx* ctx, hplx_pmat* mat);
+
+/* ---- HPL_pdrandmat (include/hpl_pmatgen.hpp:53-59,
+ * src/matgen/HPL_pdrandmat_device.cpp:94-201): fill the m x n matrix A (lda)
+ * from the 64-bit LCG with seed `seed`; element (i,j) uses state i + j*m.
+ * diag_shift (new) is added to every element with i == j. */
+int hplx_pdrandmat(hplx_ctx* ctx, int m, int n, int nb, double* A, int lda,
+                   int seed, double diag_shift);
+
+/* ---- HPL_pdgesv (include/hpl_pgesv.hpp:162, src/pgesv/HPL_pdgesv.cpp:19-368)
+ * Factors [A|b] in place (upper triangle = U, strictly lower = the panel
+ * multipliers L, left unpivoted by later panels as in the reference), writes
+ * x to mat->X and the pivots to mat->ipiv.  Returns 0, an error, or a
+ * positive zero-pivot column (then X is not computed). */
+int hplx_pdgesv(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat);
+
+/* ---- HPL_pdtest residual check (src/HPL_pdtest.cpp:303-432) with [A|b]
+ * regenerated on the fly from `seed` (A is not re-read; its storage may hold
+ * the factors).  Uses mat->X and mat's workspace.  out[9]:
+ *   [0] ||Ax-b||_oo/(eps*(||A||_oo*||x||_oo+||b||_oo)*N), true inf-norm
+ *   [1] ||Ax-b||_oo
+ *   [2] the same ratio with |r_0| in place of ||r||_oo (what the reference
+ *       prints, src/pauxil/HPL_pdlange_device.cpp:284)
+ *   [3] ||A||_oo [4] ||A||_1 [5] ||x||_oo [6] ||x||_1 [7] ||b||_oo [8] |r_0| */
+int hplx_pdtest_check(hplx_ctx* ctx, const hplx_pmat* mat, int seed,
+                      double diag_shift, double* out);
+
+/* ---- hot-path components, callable one at a time (parity tests) -------- */
+
+/* HPL_pdfact (src/pfact/HPL_pdfact.cpp:20-110) with rffun = HPL_pdrpanrlT
+ * (src/pfact/HPL_pdrpanrlT.cpp:19-161) and pffun = HPL_pdpanrlT
+ * (src/pfact/HPL_pdpanrlT_device.cpp:23-468): factor the m x jb panel P (lda)
+ * in place; row interchanges span the jb panel columns.  ipiv (device, jb)
+ * receives panel-relative pivot rows plus row_offset. */
+int hplx_pdfact(hplx_ctx* ctx, int nbmin, int nbdiv, double* P, int lda,
+                int m, int jb, int* ipiv, int row_offset);
+
+/* HPL_pipid + HPL_pdpanel_swapids P=1 branch (src/pgesv/HPL_pipid.cpp:19-164,
+ * src/panel/HPL_pdpanel_swapids.cpp:22-41), on the device: from jb
+ * panel-relative interchanges ipiv (device) build permU (device, 2*jb ints):
+ * permU[i] = source row of top row i, permU[jb+i] = destination of the
+ * displaced top row i or -1. */
+int hplx_pipid(hplx_ctx* ctx, int jb, const int* ipiv, int* permU);
+
+/* HPL_dlaswp00N (src/pauxil/HPL_dlaswp00N_device.cpp:22-112): apply permU to
+ * the n columns of A (lda): rows [0,jb) gathered, displaced rows scattered. */
+int hplx_dlaswp00N(hplx_ctx* ctx, int jb, int n, double* A, int lda,
+                   const int* permU);
+
+/* HPL_pdupdateTT P=1 branch (src/pgesv/HPL_pdupdateTT.cpp:91-165) fused
+ * with HPL_pdlaswp_end (src/pgesv/HPL_pdlaswp.cpp:386-389): for the panel
+ * whose top-left element is A[j + j*lda] (width jb, m = rows from j down)
+ * and its n trailing columns: swap rows with permU, U12 = L11^{-1} A12,
+ * A22 -= L21 * U12.  W is caller scratch of jb*n doubles (device). */
+int hplx_pdupdate(hplx_ctx* ctx, double* A, int lda, int m, int jb, int n,
+                  const int* permU, double* W);
+
+/* C = beta*C + alpha*A*B (A m x k, B k x n, column-major, no transposes) on
+ * the FP64 MFMA kernel that carries the trailing update
+ * (rocblas_dgemm at src/pgesv/HPL_pdupdateTT.cpp:131-144).  alpha must be
+ * +1 or -1 and beta 0 or 1. */
+int hplx_dgemm(hplx_ctx* ctx, int m, int n, int k, double alpha,
+               const double* A, int lda, const double* B, int ldb,
+               double beta, double* C, int ldc);
+
+/* HPL_pdtrsv (src/pgesv/HPL_pdtrsv_device.cpp:29-346): solve U x = b where
+ * U is the upper triangle of the n x n matrix A (lda) and b its column n,
+ * blocked by nb (<= 1024); x (device, n). */
+int hplx_pdtrsv(hplx_ctx* ctx, int n, int nb, const double* A, int lda,
+                double* x);
+
+/* Device time in ms of the FP64 trailing-update GEMM kernels queued since
+ * the last hplx_pdgesv started (HIP events around each launch), with the
+ * count of launches and their algorithmic flops. */
+int hplx_update_stats(hplx_ctx* ctx, double* gemm_ms, int* launches,
+                      double* gemm_flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPLX_H */

@@ -1,0 +1,307 @@
+"""rochpl_amd -- MI355X-native drop-in for rocHPL's P=Q=1 HPL_pdgesv path.
+
+The product is the C-ABI library ``rochpl_amd/libhplx.so`` (hand-written HIP
+for gfx950, declared in ``include/hplx.h``).  This module is a thin ctypes
+mirror of that ABI with the reference's names (HPL_pdgesv -> pdgesv, ...), so
+Python callers and tests read like the reference's own driver
+(src/HPL_pdtest.cpp).  There is no CPU fallback: every call goes to the HIP
+library and fails loudly if it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhplx.so")
+CSRC = os.path.join(_HERE, "csrc")
+HPL_ISEED = 100  # include/hpl_ptest.hpp:57
+
+HPLX_SUCCESS = 0
+HPLX_ERR_INVALID = -1
+HPLX_ERR_UNSUPPORTED = -2
+HPLX_ERR_HIP = -3
+HPLX_ERR_ALLOC = -4
+
+HPLX_LEFT_LOOKING, HPLX_CROUT, HPLX_RIGHT_LOOKING = 301, 302, 303
+HPLX_1RING, HPLX_1RING_M, HPLX_2RING, HPLX_2RING_M, HPLX_BLONG, \
+    HPLX_BLONG_M = 401, 402, 403, 404, 405, 406
+HPLX_SWAP00, HPLX_SWAP01, HPLX_SW_MIX, HPLX_NO_SWP = 451, 452, 453, 499
+
+# Every symbol include/hplx.h declares (checked by tests/test_abi.py).
+EXPORTS = [
+    "hplx_palg_default", "hplx_ctx_create", "hplx_ctx_destroy",
+    "hplx_last_error", "hplx_set_stream", "hplx_synchronize", "hplx_num_cus",
+    "hplx_malloc", "hplx_free", "hplx_memcpy_h2d", "hplx_memcpy_d2h",
+    "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
+    "hplx_pdtest_check", "hplx_pdfact", "hplx_pipid", "hplx_dlaswp00N",
+    "hplx_pdupdate", "hplx_dgemm", "hplx_pdtrsv", "hplx_update_stats",
+]
+
+
+class HplxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("hplx error %d: %s" % (code, msg))
+        self.code = code
+
+
+class PAlg(ctypes.Structure):
+    """hplx_palg, the mirror of HPL_T_palg (include/hpl_pgesv.hpp:63-81)."""
+    _fields_ = [("btopo", ctypes.c_int), ("depth", ctypes.c_int),
+                ("nbdiv", ctypes.c_int), ("nbmin", ctypes.c_int),
+                ("pfact", ctypes.c_int), ("rfact", ctypes.c_int),
+                ("fswap", ctypes.c_int), ("fsthr", ctypes.c_int),
+                ("equil", ctypes.c_int), ("L1notran", ctypes.c_int),
+                ("Unotran", ctypes.c_int), ("align", ctypes.c_int),
+                ("frac", ctypes.c_double), ("its", ctypes.c_int)]
+
+
+class PMat(ctypes.Structure):
+    """hplx_pmat, the mirror of HPL_T_pmat (include/hpl_pgesv.hpp:83-109)."""
+    _fields_ = [("A", ctypes.c_void_p), ("X", ctypes.c_void_p),
+                ("ipiv", ctypes.c_void_p), ("n", ctypes.c_int),
+                ("nb", ctypes.c_int), ("ld", ctypes.c_int),
+                ("mp", ctypes.c_int), ("nq", ctypes.c_int),
+                ("info", ctypes.c_int), ("ws", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile libhplx.so with hipcc for gfx950 (rochpl_amd/csrc/Makefile)."""
+    args = ["make", "-s", "-j8"]
+    if force:
+        args.append("-B")
+    subprocess.run(args, cwd=CSRC, check=True)
+    return LIB_PATH
+
+
+def lib():
+    """Load libhplx.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("rochpl_amd: %s is missing -- run "
+                           "__graft_entry__.build() (no CPU fallback)"
+                           % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    i32, f64, vp, sz = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, \
+        ctypes.c_size_t
+    P = ctypes.POINTER
+    sig = {
+        "hplx_palg_default": (None, [P(PAlg)]),
+        "hplx_ctx_create": (i32, [i32, P(vp)]),
+        "hplx_ctx_destroy": (i32, [vp]),
+        "hplx_last_error": (ctypes.c_char_p, [vp]),
+        "hplx_set_stream": (i32, [vp, vp]),
+        "hplx_synchronize": (i32, [vp]),
+        "hplx_num_cus": (i32, [vp]),
+        "hplx_malloc": (i32, [vp, sz, P(vp)]),
+        "hplx_free": (i32, [vp, vp]),
+        "hplx_memcpy_h2d": (i32, [vp, vp, vp, sz]),
+        "hplx_memcpy_d2h": (i32, [vp, vp, vp, sz]),
+        "hplx_pmat_alloc": (i32, [vp, i32, i32, P(PMat)]),
+        "hplx_pmat_free": (i32, [vp, P(PMat)]),
+        "hplx_pdrandmat": (i32, [vp, i32, i32, i32, vp, i32, i32, f64]),
+        "hplx_pdgesv": (i32, [vp, P(PAlg), P(PMat)]),
+        "hplx_pdtest_check": (i32, [vp, P(PMat), i32, f64, P(f64)]),
+        "hplx_pdfact": (i32, [vp, i32, i32, vp, i32, i32, i32, vp, i32]),
+        "hplx_pipid": (i32, [vp, i32, vp, vp]),
+        "hplx_dlaswp00N": (i32, [vp, i32, i32, vp, i32, vp]),
+        "hplx_pdupdate": (i32, [vp, vp, i32, i32, i32, i32, vp, vp]),
+        "hplx_dgemm": (i32, [vp, i32, i32, i32, f64, vp, i32, vp, i32, f64,
+                             vp, i32]),
+        "hplx_pdtrsv": (i32, [vp, i32, i32, vp, i32, vp]),
+        "hplx_update_stats": (i32, [vp, P(f64), P(i32), P(f64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def palg_default() -> PAlg:
+    p = PAlg()
+    lib().hplx_palg_default(ctypes.byref(p))
+    return p
+
+
+def hpl_ld(n: int) -> int:
+    """src/matgen/HPL_pdmatgen.cpp:109-112."""
+    return ((max(n, 0) + 95) // 128) * 128 + 32
+
+
+class DeviceArray:
+    """A device allocation made through the library (hplx_malloc)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        ctx._chk(lib().hplx_malloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def offset(self, nbytes: int) -> int:
+        return self.ptr + int(nbytes)
+
+    def upload(self, arr: np.ndarray, byte_offset: int = 0) -> None:
+        arr = np.asarray(arr)
+        if not (arr.flags.c_contiguous or arr.flags.f_contiguous):
+            arr = np.ascontiguousarray(arr)
+        assert byte_offset + arr.nbytes <= self.nbytes
+        self.ctx._chk(lib().hplx_memcpy_h2d(
+            self.ctx.h, ctypes.c_void_p(self.ptr + byte_offset),
+            arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes))
+
+    def download(self, shape, dtype, order="F", byte_offset: int = 0):
+        out = np.empty(shape, dtype=dtype, order=order)
+        assert byte_offset + out.nbytes <= self.nbytes
+        self.ctx._chk(lib().hplx_memcpy_d2h(
+            self.ctx.h, out.ctypes.data_as(ctypes.c_void_p),
+            ctypes.c_void_p(self.ptr + byte_offset), out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().hplx_free(self.ctx.h, ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """hplx_ctx: device, stream and the solver's small workspace
+    (replaces HPL_InitGPU, src/HPL_InitGPU.cpp:29-130)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib().hplx_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise HplxError(rc, "hplx_ctx_create(device=%d) failed" % device)
+        self.h = h
+
+    def _chk(self, rc: int) -> int:
+        if rc < 0:
+            raise HplxError(rc, lib().hplx_last_error(self.h).decode())
+        return rc
+
+    def close(self) -> None:
+        if self.h:
+            lib().hplx_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def num_cus(self) -> int:
+        return lib().hplx_num_cus(self.h)
+
+    def synchronize(self) -> None:
+        self._chk(lib().hplx_synchronize(self.h))
+
+    def set_stream(self, stream_handle: int | None) -> None:
+        self._chk(lib().hplx_set_stream(self.h, stream_handle))
+
+    def alloc(self, nbytes: int) -> DeviceArray:
+        return DeviceArray(self, nbytes)
+
+    # ---- HPL_pdmatgen / HPL_pdmatfree -------------------------------------
+    def pmat_alloc(self, n: int, nb: int) -> PMat:
+        m = PMat()
+        self._chk(lib().hplx_pmat_alloc(self.h, n, nb, ctypes.byref(m)))
+        return m
+
+    def pmat_free(self, mat: PMat) -> None:
+        self._chk(lib().hplx_pmat_free(self.h, ctypes.byref(mat)))
+
+    # ---- HPL_pdrandmat ------------------------------------------------------
+    def pdrandmat(self, m: int, n: int, nb: int, A: int, lda: int,
+                  seed: int = HPL_ISEED, diag_shift: float = 0.0) -> None:
+        self._chk(lib().hplx_pdrandmat(self.h, m, n, nb, ctypes.c_void_p(A),
+                                       lda, seed, float(diag_shift)))
+
+    # ---- HPL_pdgesv ---------------------------------------------------------
+    def pdgesv(self, palg: PAlg, mat: PMat) -> int:
+        """Returns 0 or the 1-based column of the first exact zero pivot."""
+        return self._chk(lib().hplx_pdgesv(self.h, ctypes.byref(palg),
+                                           ctypes.byref(mat)))
+
+    # ---- HPL_pdtest verification --------------------------------------------
+    def pdtest_check(self, mat: PMat, seed: int = HPL_ISEED,
+                     diag_shift: float = 0.0) -> dict:
+        out = (ctypes.c_double * 9)()
+        self._chk(lib().hplx_pdtest_check(self.h, ctypes.byref(mat), seed,
+                                          float(diag_shift), out))
+        keys = ["resid", "rnorm", "resid_r0_form", "AnormI", "Anorm1",
+                "XnormI", "Xnorm1", "BnormI", "r0"]
+        return dict(zip(keys, list(out)))
+
+    # ---- components -----------------------------------------------------------
+    def pdfact(self, P: int, lda: int, m: int, jb: int, ipiv: int,
+               nbmin: int = 32, nbdiv: int = 2, row_offset: int = 0) -> int:
+        return self._chk(lib().hplx_pdfact(self.h, nbmin, nbdiv,
+                                           ctypes.c_void_p(P), lda, m, jb,
+                                           ctypes.c_void_p(ipiv), row_offset))
+
+    def pipid(self, jb: int, ipiv: int, permU: int) -> None:
+        self._chk(lib().hplx_pipid(self.h, jb, ctypes.c_void_p(ipiv),
+                                   ctypes.c_void_p(permU)))
+
+    def dlaswp00N(self, jb: int, n: int, A: int, lda: int, permU: int) -> None:
+        self._chk(lib().hplx_dlaswp00N(self.h, jb, n, ctypes.c_void_p(A), lda,
+                                       ctypes.c_void_p(permU)))
+
+    def pdupdate(self, A: int, lda: int, m: int, jb: int, n: int, permU: int,
+                 W: int) -> None:
+        self._chk(lib().hplx_pdupdate(self.h, ctypes.c_void_p(A), lda, m, jb,
+                                      n, ctypes.c_void_p(permU),
+                                      ctypes.c_void_p(W)))
+
+    def dgemm(self, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc) -> None:
+        self._chk(lib().hplx_dgemm(self.h, m, n, k, float(alpha),
+                                   ctypes.c_void_p(A), lda, ctypes.c_void_p(B),
+                                   ldb, float(beta), ctypes.c_void_p(C), ldc))
+
+    def pdtrsv(self, n: int, nb: int, A: int, lda: int, x: int) -> None:
+        self._chk(lib().hplx_pdtrsv(self.h, n, nb, ctypes.c_void_p(A), lda,
+                                    ctypes.c_void_p(x)))
+
+    def update_stats(self):
+        ms, cnt, fl = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        self._chk(lib().hplx_update_stats(self.h, ctypes.byref(ms),
+                                          ctypes.byref(cnt), ctypes.byref(fl)))
+        return ms.value, cnt.value, fl.value
+
+    # ---- whole-matrix helpers -------------------------------------------------
+    def download_matrix(self, mat: PMat, rows: int | None = None,
+                        cols: int | None = None) -> np.ndarray:
+        rows = mat.n if rows is None else rows
+        cols = mat.n + 1 if cols is None else cols
+        out = np.empty((mat.ld, cols), dtype=np.float64, order="F")
+        self._chk(lib().hplx_memcpy_d2h(self.h,
+                                        out.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.c_void_p(mat.A), out.nbytes))
+        return out[:rows, :]
+
+    def download_vec(self, ptr: int, n: int, dtype=np.float64) -> np.ndarray:
+        out = np.empty(n, dtype=dtype)
+        if n:
+            self._chk(lib().hplx_memcpy_d2h(
+                self.h, out.ctypes.data_as(ctypes.c_void_p),
+                ctypes.c_void_p(ptr), out.nbytes))
+        return out

@@ -1,0 +1,497 @@
+// C ABI of rochpl_amd (include/hplx.h): context, allocation, the P=Q=1
+// HPL_pdgesv panel loop and the residual check.
+//
+// The panel loop restates src/pgesv/HPL_pdgesv.cpp:168-245 at P=Q=1 with the
+// host out of the loop: every step is queued on one stream and no host
+// synchronisation happens until the back-solve.  Per panel (width jb, at
+// global row/column j):
+//   1. panel factorisation in place (HPL_pdfact)             -> ipiv[j:j+jb)
+//   2. pivot composition on the device (HPL_pdpanel_swapids) -> permU
+//   3. explicit inverse of the unit lower L11                -> Linv
+//   4. row swaps of the trailing columns; the new top rows go to W
+//      (HPL_pdlaswp_end / HPL_dlaswp00N)
+//   5. U12 = Linv * W written in place over A12      (dtrsm of HPL_pdupdateTT)
+//   6. A22 -= L21 * U12                              (dgemm of HPL_pdupdateTT)
+// The reference's lookahead and split update only reorder these steps for
+// overlap; each matrix element sees the same sequence of operations.  b is
+// column n of the augmented matrix and is carried through 4-6, so after the
+// loop the back-solve U x = b' finishes the solve (HPL_pdtrsv).
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "hplx_internal.hpp"
+
+struct hplx_ctx {
+  hplx::Ctx c;
+};
+
+namespace {
+
+using hplx::Ctx;
+
+int fail(Ctx& c, int code, const char* fmt, ...) {
+  char    buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  c.last_error = buf;
+  return code;
+}
+
+#define HIPCHK(C, expr)                                                    \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if(e_ != hipSuccess)                                                   \
+      return fail((C), HPLX_ERR_HIP, "%s failed: %s (%s:%d)", #expr,       \
+                  hipGetErrorString(e_), __FILE__, __LINE__);              \
+  } while(0)
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int hpl_ld(int n) { return ((n > 0 ? n : 0) + 95) / 128 * 128 + 32; }
+
+int use(hplx_ctx* ctx) {
+  if(!ctx) return HPLX_ERR_INVALID;
+  hipError_t e = hipSetDevice(ctx->c.device);
+  if(e != hipSuccess)
+    return fail(ctx->c, HPLX_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  return HPLX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hplx_palg_default(hplx_palg* p) {
+  if(!p) return;
+  p->btopo    = HPLX_1RING;
+  p->depth    = 1;
+  p->nbdiv    = 2;
+  p->nbmin    = 32;
+  p->pfact    = HPLX_RIGHT_LOOKING;
+  p->rfact    = HPLX_RIGHT_LOOKING;
+  p->fswap    = HPLX_SWAP01;
+  p->fsthr    = 64;
+  p->equil    = 0;
+  p->L1notran = 0;
+  p->Unotran  = 0;
+  p->align    = 8;
+  p->frac     = 0.3;
+  p->its      = 1;
+}
+
+int hplx_ctx_create(int device, hplx_ctx** out) {
+  if(!out) return HPLX_ERR_INVALID;
+  *out          = nullptr;
+  hplx_ctx* ctx = new(std::nothrow) hplx_ctx();
+  if(!ctx) return HPLX_ERR_ALLOC;
+  Ctx& c   = ctx->c;
+  c.device = device;
+  int ndev = 0;
+  if(hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    delete ctx;
+    return HPLX_ERR_HIP;
+  }
+  if(hipSetDevice(device) != hipSuccess ||
+     hipDeviceGetAttribute(&c.num_cus, hipDeviceAttributeMultiprocessorCount,
+                           device) != hipSuccess ||
+     hipStreamCreateWithFlags(&c.own_stream, hipStreamNonBlocking) !=
+         hipSuccess) {
+    delete ctx;
+    return HPLX_ERR_HIP;
+  }
+  c.stream = c.own_stream;
+  // small workspace
+  const size_t nLinv  = static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB * 8;
+  const size_t nperm  = 2 * hplx::kMaxNB * sizeof(int);
+  const size_t nslots = 2 * static_cast<size_t>(hplx::kMaxGrid + 1) *
+                        hplx::kSlotStride * sizeof(double);
+  size_t off_perm  = align_up(nLinv, 256);
+  size_t off_slots = align_up(off_perm + nperm, 256);
+  size_t off_cnt   = align_up(off_slots + nslots, 256);
+  size_t off_flags = off_cnt + 256;
+  size_t off_dout  = off_flags + 256;
+  size_t total     = off_dout + 256;
+  if(hipMalloc(&c.small_base, total) != hipSuccess) {
+    hipStreamDestroy(c.own_stream);
+    delete ctx;
+    return HPLX_ERR_ALLOC;
+  }
+  char* b   = static_cast<char*>(c.small_base);
+  c.Linv    = reinterpret_cast<double*>(b);
+  c.permU   = reinterpret_cast<int*>(b + off_perm);
+  c.slots   = reinterpret_cast<double*>(b + off_slots);
+  c.counter = reinterpret_cast<uint32_t*>(b + off_cnt);
+  c.flags   = reinterpret_cast<int*>(b + off_flags);
+  c.dout    = reinterpret_cast<double*>(b + off_dout);
+  if(hipMemset(c.small_base, 0, total) != hipSuccess ||
+     hipDeviceSynchronize() != hipSuccess) {
+    hipFree(c.small_base);
+    hipStreamDestroy(c.own_stream);
+    delete ctx;
+    return HPLX_ERR_HIP;
+  }
+  *out = ctx;
+  return HPLX_SUCCESS;
+}
+
+int hplx_ctx_destroy(hplx_ctx* ctx) {
+  if(!ctx) return HPLX_SUCCESS;
+  Ctx& c = ctx->c;
+  hipSetDevice(c.device);
+  hipStreamSynchronize(c.stream);
+  for(hipEvent_t e : c.ev) hipEventDestroy(e);
+  if(c.small_base) hipFree(c.small_base);
+  if(c.own_stream) hipStreamDestroy(c.own_stream);
+  delete ctx;
+  return HPLX_SUCCESS;
+}
+
+const char* hplx_last_error(const hplx_ctx* ctx) {
+  return ctx ? ctx->c.last_error.c_str() : "null context";
+}
+
+int hplx_set_stream(hplx_ctx* ctx, void* stream) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  c.stream = stream ? static_cast<hipStream_t>(stream) : c.own_stream;
+  return HPLX_SUCCESS;
+}
+
+int hplx_synchronize(hplx_ctx* ctx) {
+  int r = use(ctx);
+  if(r) return r;
+  HIPCHK(ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_num_cus(hplx_ctx* ctx) { return ctx ? ctx->c.num_cus : 0; }
+
+int hplx_malloc(hplx_ctx* ctx, size_t bytes, void** dptr) {
+  int r = use(ctx);
+  if(r) return r;
+  if(!dptr) return HPLX_ERR_INVALID;
+  if(hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess)
+    return fail(ctx->c, HPLX_ERR_ALLOC, "hipMalloc(%zu) failed", bytes);
+  return HPLX_SUCCESS;
+}
+
+int hplx_free(hplx_ctx* ctx, void* dptr) {
+  int r = use(ctx);
+  if(r) return r;
+  HIPCHK(ctx->c, hipStreamSynchronize(ctx->c.stream));
+  if(dptr) HIPCHK(ctx->c, hipFree(dptr));
+  return HPLX_SUCCESS;
+}
+
+int hplx_memcpy_h2d(hplx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int r = use(ctx);
+  if(r) return r;
+  HIPCHK(ctx->c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                                ctx->c.stream));
+  HIPCHK(ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_memcpy_d2h(hplx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int r = use(ctx);
+  if(r) return r;
+  HIPCHK(ctx->c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                                ctx->c.stream));
+  HIPCHK(ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_pmat_alloc(hplx_ctx* ctx, int n, int nb, hplx_pmat* mat) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(!mat || n < 1 || nb < 1)
+    return fail(c, HPLX_ERR_INVALID, "pmat_alloc: n=%d nb=%d", n, nb);
+  if(nb > hplx::kMaxNB)
+    return fail(c, HPLX_ERR_UNSUPPORTED, "NB=%d exceeds %d", nb, hplx::kMaxNB);
+  std::memset(mat, 0, sizeof(*mat));
+  mat->n  = n;
+  mat->nb = nb;
+  mat->ld = hpl_ld(n);
+  mat->mp = n;
+  mat->nq = n + 1;
+  const size_t nA = static_cast<size_t>(mat->ld) * (n + 1);
+  hplx::MatWs* ws = new(std::nothrow) hplx::MatWs();
+  if(!ws) return HPLX_ERR_ALLOC;
+  const size_t wl = static_cast<size_t>(nb) * (n + 1);
+  const size_t cl = hplx::check_part_len(n);
+  ws->W_len       = wl > cl ? wl : cl;
+  if(hipMalloc(&mat->A, nA * sizeof(double)) != hipSuccess) {
+    delete ws;
+    return fail(c, HPLX_ERR_ALLOC, "hipMalloc of A|b (%zu bytes) failed",
+                nA * sizeof(double));
+  }
+  if(hipMalloc(&mat->X, static_cast<size_t>(n) * sizeof(double)) !=
+         hipSuccess ||
+     hipMalloc(&mat->ipiv, static_cast<size_t>(n) * sizeof(int)) !=
+         hipSuccess ||
+     hipMalloc(&ws->W, ws->W_len * sizeof(double)) != hipSuccess) {
+    if(mat->X) hipFree(mat->X);
+    if(mat->ipiv) hipFree(mat->ipiv);
+    hipFree(mat->A);
+    delete ws;
+    std::memset(mat, 0, sizeof(*mat));
+    return fail(c, HPLX_ERR_ALLOC, "hipMalloc of workspace failed");
+  }
+  mat->ws = ws;
+  return HPLX_SUCCESS;
+}
+
+int hplx_pmat_free(hplx_ctx* ctx, hplx_pmat* mat) {
+  int r = use(ctx);
+  if(r) return r;
+  if(!mat) return HPLX_ERR_INVALID;
+  hipStreamSynchronize(ctx->c.stream);
+  hplx::MatWs* ws = static_cast<hplx::MatWs*>(mat->ws);
+  if(ws) {
+    if(ws->W) hipFree(ws->W);
+    delete ws;
+  }
+  if(mat->ipiv) hipFree(mat->ipiv);
+  if(mat->X) hipFree(mat->X);
+  if(mat->A) hipFree(mat->A);
+  std::memset(mat, 0, sizeof(*mat));
+  return HPLX_SUCCESS;
+}
+
+int hplx_pdrandmat(hplx_ctx* ctx, int m, int n, int nb, double* A, int lda,
+                   int seed, double diag_shift) {
+  int r = use(ctx);
+  if(r) return r;
+  (void)nb;
+  if(m < 0 || n < 0 || (m > 0 && (!A || lda < m)) || seed < 0)
+    return fail(ctx->c, HPLX_ERR_INVALID, "pdrandmat: m=%d n=%d lda=%d", m, n,
+                lda);
+  HIPCHK(ctx->c, hplx::launch_matgen(ctx->c.stream, m, n, A, lda, seed,
+                                     diag_shift));
+  return HPLX_SUCCESS;
+}
+
+static int check_palg(Ctx& c, const hplx_palg* p) {
+  if(!p) return fail(c, HPLX_ERR_INVALID, "null palg");
+  if(p->pfact != HPLX_RIGHT_LOOKING || p->rfact != HPLX_RIGHT_LOOKING)
+    return fail(c, HPLX_ERR_UNSUPPORTED,
+                "only right-looking panel/recursive factorisation is built "
+                "(pfact=%d rfact=%d)",
+                p->pfact, p->rfact);
+  if(p->L1notran != 0 || p->Unotran != 0)
+    return fail(c, HPLX_ERR_UNSUPPORTED, "L1 and U must be in transposed form");
+  if(p->equil != 0)
+    return fail(c, HPLX_ERR_UNSUPPORTED, "equilibration is not supported");
+  if(p->nbmin < 1 || p->nbdiv < 2)
+    return fail(c, HPLX_ERR_INVALID, "nbmin=%d nbdiv=%d", p->nbmin, p->nbdiv);
+  return HPLX_SUCCESS;
+}
+
+int hplx_pdgesv(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if((r = check_palg(c, palg))) return r;
+  if(!mat || !mat->A || !mat->ws) return fail(c, HPLX_ERR_INVALID, "bad pmat");
+  const int N   = mat->n;
+  const int nb  = mat->nb;
+  const int lda = mat->ld;
+  if(N > c.num_cus * hplx::kLeafThreads * hplx::kLeafMaxRPT)
+    return fail(c, HPLX_ERR_UNSUPPORTED, "N=%d exceeds the panel kernel's %d rows",
+                N, c.num_cus * hplx::kLeafThreads * hplx::kLeafMaxRPT);
+  hplx::MatWs* ws = static_cast<hplx::MatWs*>(mat->ws);
+  double*      A  = mat->A;
+  hipStream_t  s  = c.stream;
+
+  const int npanels = (N + nb - 1) / nb;
+  while(static_cast<int>(c.ev.size()) < 2 * npanels) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c.ev.push_back(e);
+  }
+  c.ev_pairs   = 0;
+  c.gemm_flops = 0.0;
+
+  HIPCHK(c, hplx::reset_panel_state(c));
+  for(int j = 0; j < N; j += nb) {
+    const int jb = (N - j) < nb ? (N - j) : nb;
+    const int m  = N - j;
+    double*   P  = A + j + static_cast<size_t>(j) * lda;
+    HIPCHK(c, hplx::panel_factor(c, palg->nbmin, palg->nbdiv, P, lda, m, jb,
+                                 mat->ipiv + j, j));
+    const int nt = N + 1 - j - jb;
+    if(nt <= 0) continue;
+    double* A12 = A + j + static_cast<size_t>(j + jb) * lda;
+    HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
+    HIPCHK(c, hplx::launch_linv(s, jb, P, lda, c.Linv, nb));
+    HIPCHK(c, hplx::launch_swap_gather(s, jb, nt, A12, lda, c.permU, ws->W, nb));
+    HIPCHK(c, hplx::launch_dgemm(s, jb, nt, jb, 1.0, c.Linv, nb, ws->W, nb, 0.0,
+                                 A12, lda, true));
+    if(m - jb > 0) {
+      hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
+      HIPCHK(c, hipEventRecord(e0, s));
+      HIPCHK(c, hplx::launch_dgemm(s, m - jb, nt, jb, -1.0, P + jb, lda, A12,
+                                   lda, 1.0, A12 + jb, lda, false));
+      HIPCHK(c, hipEventRecord(e1, s));
+      c.ev_pairs++;
+      c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
+    }
+  }
+  // zero pivot or barrier failure?  (one synchronisation, before the solve)
+  int flags[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
+                           s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if(flags[1])
+    return fail(c, HPLX_ERR_HIP,
+                "panel kernel grid barrier timed out (workgroups not "
+                "co-resident)");
+  mat->info = flags[0];
+  if(flags[0]) return flags[0];
+  HIPCHK(c, hipMemcpyAsync(mat->X, A + static_cast<size_t>(N) * lda,
+                           static_cast<size_t>(N) * sizeof(double),
+                           hipMemcpyDeviceToDevice, s));
+  HIPCHK(c, hplx::launch_trsv(s, N, nb, A, lda, mat->X));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return HPLX_SUCCESS;
+}
+
+int hplx_pdtest_check(hplx_ctx* ctx, const hplx_pmat* mat, int seed,
+                      double diag_shift, double* out) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(!mat || !mat->X || !mat->ws || !out)
+    return fail(c, HPLX_ERR_INVALID, "pdtest_check: bad arguments");
+  hplx::MatWs* ws = static_cast<hplx::MatWs*>(mat->ws);
+  HIPCHK(c, hplx::launch_check(c.stream, mat->n, seed, diag_shift, mat->X,
+                               ws->W, ws->W_len, c.dout));
+  HIPCHK(c, hipMemcpyAsync(out, c.dout, 9 * sizeof(double),
+                           hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_pdfact(hplx_ctx* ctx, int nbmin, int nbdiv, double* P, int lda, int m,
+                int jb, int* ipiv, int row_offset) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(!P || !ipiv || m < 1 || jb < 1 || jb > m || lda < m)
+    return fail(c, HPLX_ERR_INVALID, "pdfact: m=%d jb=%d lda=%d", m, jb, lda);
+  if(jb > hplx::kMaxNB || m > c.num_cus * hplx::kLeafThreads * hplx::kLeafMaxRPT)
+    return fail(c, HPLX_ERR_UNSUPPORTED, "pdfact: m=%d jb=%d too large", m, jb);
+  HIPCHK(c, hplx::reset_panel_state(c));
+  HIPCHK(c, hplx::panel_factor(c, nbmin, nbdiv, P, lda, m, jb, ipiv,
+                               row_offset));
+  int flags[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
+                           c.stream));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  if(flags[1])
+    return fail(c, HPLX_ERR_HIP, "panel kernel grid barrier timed out");
+  return flags[0] ? flags[0] - row_offset : HPLX_SUCCESS;
+}
+
+int hplx_pipid(hplx_ctx* ctx, int jb, const int* ipiv, int* permU) {
+  int r = use(ctx);
+  if(r) return r;
+  if(jb < 0 || jb > hplx::kMaxNB || (jb > 0 && (!ipiv || !permU)))
+    return fail(ctx->c, HPLX_ERR_INVALID, "pipid: jb=%d", jb);
+  HIPCHK(ctx->c, hplx::launch_pipid(ctx->c.stream, jb, ipiv, 0, permU));
+  HIPCHK(ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_dlaswp00N(hplx_ctx* ctx, int jb, int n, double* A, int lda,
+                   const int* permU) {
+  int r = use(ctx);
+  if(r) return r;
+  if(jb < 0 || jb > hplx::kMaxNB || n < 0 || lda < jb)
+    return fail(ctx->c, HPLX_ERR_INVALID, "dlaswp00N: jb=%d n=%d", jb, n);
+  HIPCHK(ctx->c, hplx::launch_swap_inplace(ctx->c.stream, jb, n, A, lda, permU));
+  HIPCHK(ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_pdupdate(hplx_ctx* ctx, double* A, int lda, int m, int jb, int n,
+                  const int* permU, double* W) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(!A || !permU || !W || jb < 1 || jb > hplx::kMaxNB || m < jb || n < 0 ||
+     lda < m)
+    return fail(c, HPLX_ERR_INVALID, "pdupdate: m=%d jb=%d n=%d", m, jb, n);
+  if(n == 0) return HPLX_SUCCESS;
+  hipStream_t s   = c.stream;
+  double*     A12 = A + static_cast<size_t>(jb) * lda;
+  HIPCHK(c, hplx::launch_linv(s, jb, A, lda, c.Linv, jb));
+  HIPCHK(c, hplx::launch_swap_gather(s, jb, n, A12, lda, permU, W, jb));
+  HIPCHK(c, hplx::launch_dgemm(s, jb, n, jb, 1.0, c.Linv, jb, W, jb, 0.0, A12,
+                               lda, true));
+  if(m > jb)
+    HIPCHK(c, hplx::launch_dgemm(s, m - jb, n, jb, -1.0, A + jb, lda, A12, lda,
+                                 1.0, A12 + jb, lda, false));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return HPLX_SUCCESS;
+}
+
+int hplx_dgemm(hplx_ctx* ctx, int m, int n, int k, double alpha,
+               const double* A, int lda, const double* B, int ldb, double beta,
+               double* C, int ldc) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(m < 0 || n < 0 || k < 0 || lda < (m > 1 ? m : 1) ||
+     ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return fail(c, HPLX_ERR_INVALID, "dgemm: bad sizes");
+  if(!(alpha == 1.0 || alpha == -1.0) || !(beta == 0.0 || beta == 1.0))
+    return fail(c, HPLX_ERR_UNSUPPORTED, "dgemm: alpha must be +-1, beta 0 or 1");
+  HIPCHK(c, hplx::launch_dgemm(c.stream, m, n, k, alpha, A, lda, B, ldb, beta,
+                               C, ldc, false));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_pdtrsv(hplx_ctx* ctx, int n, int nb, const double* A, int lda,
+                double* x) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(n < 0 || nb < 1 || nb > hplx::kMaxNB || lda < n || (n > 0 && (!A || !x)))
+    return fail(c, HPLX_ERR_INVALID, "pdtrsv: n=%d nb=%d", n, nb);
+  if(n == 0) return HPLX_SUCCESS;
+  HIPCHK(c, hipMemcpyAsync(x, A + static_cast<size_t>(n) * lda,
+                           static_cast<size_t>(n) * sizeof(double),
+                           hipMemcpyDeviceToDevice, c.stream));
+  HIPCHK(c, hplx::launch_trsv(c.stream, n, nb, A, lda, x));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_update_stats(hplx_ctx* ctx, double* gemm_ms, int* launches,
+                      double* gemm_flops) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  double tot = 0.0;
+  for(int i = 0; i < c.ev_pairs; ++i) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c.ev[2 * i], c.ev[2 * i + 1]));
+    tot += ms;
+  }
+  if(gemm_ms) *gemm_ms = tot;
+  if(launches) *launches = c.ev_pairs;
+  if(gemm_flops) *gemm_flops = c.gemm_flops;
+  return HPLX_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// Internal declarations shared by the host ABI layer and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hplx.h"
+
+namespace hplx {
+
+// ---------------------------------------------------------------------------
+// Limits of the kernels.
+// ---------------------------------------------------------------------------
+constexpr int kLeafMax     = 32;    // widest leaf the register-resident kernel takes
+constexpr int kLeafThreads = 256;   // threads per leaf workgroup
+constexpr int kLeafMaxRPT  = 4;     // rows per thread in the leaf kernel
+constexpr int kMaxGrid     = 1024;  // workgroups a cooperative leaf may use
+constexpr int kMaxNB       = 1024;  // widest panel (pivot composition, Linv)
+constexpr int kSlotStride  = 40;    // doubles per pivot candidate slot
+
+// Per-context device state.  The small workspace (candidate slots, barrier
+// counter, flags, L11 inverse, composed pivots) lives with the context; the
+// n-sized workspace lives with the matrix (MatWs).
+struct Ctx {
+  int         device     = 0;
+  int         num_cus    = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream     = nullptr;  // where work is queued
+  std::string last_error;
+  uint64_t    barrier_epoch = 0;     // arrivals already counted on *counter
+
+  void*     small_base = nullptr;
+  double*   Linv       = nullptr;    // kMaxNB x kMaxNB
+  int*      permU      = nullptr;    // 2*kMaxNB
+  double*   slots      = nullptr;    // 2 x (kMaxGrid+1) x kSlotStride
+  uint32_t* counter    = nullptr;    // grid barrier arrivals
+  int*      flags      = nullptr;    // [0] first zero pivot (1-based), [1] error
+  double*   dout       = nullptr;    // 16 doubles of small results
+
+  // timing of the trailing-update GEMM launches of the last solve
+  std::vector<hipEvent_t> ev;
+  int                     ev_pairs = 0;
+  double                  gemm_flops = 0.0;
+};
+
+struct MatWs {
+  double* W     = nullptr;  // max(nb*(n+1), check scratch) doubles
+  size_t  W_len = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (defined in the .hip files).  All return hipError_t.
+// ---------------------------------------------------------------------------
+hipError_t launch_matgen(hipStream_t s, int m, int n, double* A, int lda,
+                         int seed, double diag_shift);
+
+// Zero the barrier counter and the flags (queued on c.stream).
+hipError_t reset_panel_state(Ctx& c);
+
+// Panel factorisation of an m x jb panel in place (recursive right-looking,
+// register-resident cooperative leaves).  ipiv receives panel-relative pivot
+// rows + row_offset; zero pivots are reported as row_offset + column + 1.
+hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, double* P, int lda,
+                        int m, int jb, int* ipiv, int row_offset);
+
+hipError_t launch_pipid(hipStream_t s, int jb, const int* ipiv, int ipiv_base,
+                        int* permU);
+hipError_t launch_swap_gather(hipStream_t s, int jb, int n, double* A,
+                              int lda, const int* permU, double* W, int ldw);
+hipError_t launch_swap_inplace(hipStream_t s, int jb, int n, double* A,
+                               int lda, const int* permU);
+hipError_t launch_linv(hipStream_t s, int jb, const double* L, int ldl,
+                       double* Linv, int ldi);
+hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const double* L,
+                                int ldl, double* B, int ldb);
+
+// C = beta*C + alpha*A*B; alpha in {+1,-1}, beta in {0,1}.  lower_a: A is
+// lower triangular (only k < row-tile end contributes).
+hipError_t launch_dgemm(hipStream_t s, int m, int n, int k, double alpha,
+                        const double* A, int lda, const double* B, int ldb,
+                        double beta, double* C, int ldc, bool lower_a);
+
+hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
+                       double* x);
+
+size_t     check_part_len(int n);  // doubles of scratch launch_check needs
+hipError_t launch_check(hipStream_t s, int n, int seed, double diag_shift,
+                        const double* X, double* part, size_t part_len,
+                        double* out_dev);
+
+}  // namespace hplx

@@ -1,0 +1,93 @@
+// Back-solve U x = b' (HPL_pdtrsv at P=Q=1, src/pgesv/HPL_pdtrsv_device.cpp:29-346).
+//
+// The reference walks the NB-blocks from the bottom with rocblas_dtrsv on the
+// diagonal block and two rocblas_dgemv calls for the rows above
+// (:169-180,195-332).  Here each block is one single-workgroup column sweep
+// on the diagonal block (x kept in LDS) followed by one HBM-bound GEMV over
+// all rows above, with the block's columns split across the 4 waves of a
+// workgroup so that enough loads are in flight.  Every element of the upper
+// triangle is read exactly once.
+#include "hplx_internal.hpp"
+
+namespace hplx {
+namespace {
+
+// x[0:kb] = U^{-1} x[0:kb], U upper, non-unit, kb <= kMaxNB.
+__global__ __launch_bounds__(1024) void diag_solve_kernel(
+    int kb, const double* __restrict__ U, int ldu, double* __restrict__ x) {
+  __shared__ double sx[kMaxNB];
+  __shared__ double sf[kMaxNB];
+  const int t = threadIdx.x;
+  for(int i = t; i < kb; i += blockDim.x) sx[i] = x[i];
+  __syncthreads();
+  for(int p = kb - 1; p >= 0; --p) {
+    const double* uc = U + static_cast<size_t>(p) * ldu;
+    const double  xp = sx[p] / uc[p];
+    for(int i = t; i < p; i += blockDim.x) sx[i] = fma(-uc[i], xp, sx[i]);
+    if(t == 0) sf[p] = xp;
+    __syncthreads();
+  }
+  for(int i = t; i < kb; i += blockDim.x) x[i] = sf[i];
+}
+
+// y[0:m] -= U[0:m, 0:kb] * xs[0:kb].  Workgroup = 64 rows; wave w takes a
+// quarter of the columns; partial sums meet in LDS.
+__global__ __launch_bounds__(256) void gemv_sub_kernel(
+    int m, int kb, const double* __restrict__ U, int ldu,
+    const double* __restrict__ xs, double* __restrict__ y) {
+  __shared__ double sxs[kMaxNB];
+  __shared__ double part[4][64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for(int i = t; i < kb; i += blockDim.x) sxs[i] = xs[i];
+  __syncthreads();
+  const int row = blockIdx.x * 64 + lane;
+  const int ck  = (kb + 3) / 4;
+  const int c0  = wave * ck;
+  const int c1  = min(kb, c0 + ck);
+  double    acc = 0.0;
+  if(row < m) {
+    const double* u = U + row + static_cast<size_t>(c0) * ldu;
+    int c = c0;
+    for(; c + 4 <= c1; c += 4) {
+      const double u0 = u[0];
+      const double u1 = u[static_cast<size_t>(ldu)];
+      const double u2 = u[2 * static_cast<size_t>(ldu)];
+      const double u3 = u[3 * static_cast<size_t>(ldu)];
+      acc = fma(u0, sxs[c], acc);
+      acc = fma(u1, sxs[c + 1], acc);
+      acc = fma(u2, sxs[c + 2], acc);
+      acc = fma(u3, sxs[c + 3], acc);
+      u += 4 * static_cast<size_t>(ldu);
+    }
+    for(; c < c1; ++c) {
+      acc = fma(u[0], sxs[c], acc);
+      u += ldu;
+    }
+  }
+  part[wave][lane] = acc;
+  __syncthreads();
+  if(wave == 0 && row < m)
+    y[row] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+}
+
+}  // namespace
+
+hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
+                       double* x) {
+  if(n <= 0) return hipSuccess;
+  if(nb <= 0 || nb > kMaxNB) nb = nb <= 0 ? 1 : kMaxNB;
+  const int nblk = (n + nb - 1) / nb;
+  for(int J = nblk - 1; J >= 0; --J) {
+    const int s0 = J * nb;
+    const int kb = (n - s0) < nb ? (n - s0) : nb;
+    hipLaunchKernelGGL(diag_solve_kernel, dim3(1), dim3(1024), 0, s, kb,
+                       A + s0 + static_cast<size_t>(s0) * lda, lda, x + s0);
+    if(s0 > 0)
+      hipLaunchKernelGGL(gemv_sub_kernel, dim3((s0 + 63) / 64), dim3(256), 0,
+                         s, s0, kb, A + static_cast<size_t>(s0) * lda, lda,
+                         x + s0, x);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace hplx

@@ -1,0 +1,65 @@
+"""The C-ABI library loads and exports every entry point include/hplx.h
+declares; the Python mirror covers the same set.  CPU only (no device calls)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import rochpl_amd as rp
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include",
+                      "hplx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(hplx_[a-z0-9_A-Z]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_expected_set():
+    assert declared_functions() == sorted(rp.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(rp.LIB_PATH):
+        pytest.skip("libhplx.so not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", rp.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (hplx_\w+)", out))
+    missing = [f for f in declared_functions() if f not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_binds_without_a_gpu():
+    if not os.path.exists(rp.LIB_PATH):
+        pytest.skip("libhplx.so not built")
+    L = rp.lib()
+    for name in rp.EXPORTS:
+        assert hasattr(L, name)
+    p = rp.palg_default()
+    # reference defaults, src/HPL_pdinfo.cpp:487-562
+    assert (p.pfact, p.rfact, p.nbmin, p.nbdiv, p.depth) == \
+        (rp.HPLX_RIGHT_LOOKING, rp.HPLX_RIGHT_LOOKING, 32, 2, 1)
+    assert (p.L1notran, p.Unotran, p.equil) == (0, 0, 0)
+
+
+def test_struct_layouts_match_header():
+    # hplx_palg: 12 ints, a double at offset 48, an int, padded -> 64 bytes
+    assert ctypes.sizeof(rp.PAlg) == 64
+    assert rp.PAlg.frac.offset == 48
+    # hplx_pmat: 3 pointers, 6 ints, a pointer -> 56 bytes on LP64
+    assert ctypes.sizeof(rp.PMat) == 56
+    assert rp.PMat.ws.offset == 48
+
+
+def test_errors_without_device_are_codes_not_exits():
+    if not os.path.exists(rp.LIB_PATH):
+        pytest.skip("libhplx.so not built")
+    h = ctypes.c_void_p()
+    rc = rp.lib().hplx_ctx_create(10 ** 6, ctypes.byref(h))
+    assert rc < 0 and not h.value
+    assert rp.lib().hplx_last_error(None) == b"null context"
