@@ -135,6 +135,29 @@ int hplx_pdrandmat(hplx_ctx* ctx, int m, int n, int nb, double* A, int lda,
  * positive zero-pivot column (then X is not computed). */
 int hplx_pdgesv(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat);
 
+/* ---- factor only: the panel loop of hplx_pdgesv over the n x n part of
+ * mat->A (column n is not touched, no back-solve).  Keeps the factors and
+ * pivots for hplx_pdgetrs / hplx_pdgesv_adjoint.  Returns like
+ * hplx_pdgesv.  No counterpart in the reference (it always factors [A|b]). */
+int hplx_pdgetrf(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat);
+
+/* ---- solve with the factors of hplx_pdgetrf / hplx_pdgesv: trans = 0
+ * solves A X = B, trans = 1 solves A^T X = B, for nrhs columns of B (device,
+ * ldb >= n) in place.  The L solve applies each panel's interchanges the way
+ * HPL_pdgesv carries b through the factorisation (HPL's L is left unpivoted
+ * by later panels); the U solve is HPL_pdtrsv's. */
+int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
+                 double* B, int ldb);
+
+/* ---- adjoint of X = A^{-1} B (new): with the factors of A in mat, the
+ * forward solution X (n x nrhs, ldx) and the upstream gradient G (ldg):
+ *   gradB = A^{-T} G          (n x nrhs, ldgb)
+ *   gradA = -gradB X^T        (n x n, ldga; skipped when gradA is NULL)
+ * All device pointers. */
+int hplx_pdgesv_adjoint(hplx_ctx* ctx, const hplx_pmat* mat, int nrhs,
+                        const double* X, int ldx, const double* G, int ldg,
+                        double* gradB, int ldgb, double* gradA, int ldga);
+
 /* ---- HPL_pdtest residual check (src/HPL_pdtest.cpp:303-432) with [A|b]
  * regenerated on the fly from `seed` (A is not re-read; its storage may hold
  * the factors).  Uses mat->X and mat's workspace.  out[9]:

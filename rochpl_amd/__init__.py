@@ -39,6 +39,7 @@ EXPORTS = [
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
     "hplx_pdtest_check", "hplx_pdfact", "hplx_pipid", "hplx_dlaswp00N",
     "hplx_pdupdate", "hplx_dgemm", "hplx_pdtrsv", "hplx_update_stats",
+    "hplx_pdgetrf", "hplx_pdgetrs", "hplx_pdgesv_adjoint",
 ]
 
 
@@ -118,6 +119,10 @@ def lib():
                              vp, i32]),
         "hplx_pdtrsv": (i32, [vp, i32, i32, vp, i32, vp]),
         "hplx_update_stats": (i32, [vp, P(f64), P(i32), P(f64)]),
+        "hplx_pdgetrf": (i32, [vp, P(PAlg), P(PMat)]),
+        "hplx_pdgetrs": (i32, [vp, P(PMat), i32, i32, vp, i32]),
+        "hplx_pdgesv_adjoint": (i32, [vp, P(PMat), i32, vp, i32, vp, i32, vp,
+                                      i32, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -240,6 +245,24 @@ class Context:
         """Returns 0 or the 1-based column of the first exact zero pivot."""
         return self._chk(lib().hplx_pdgesv(self.h, ctypes.byref(palg),
                                            ctypes.byref(mat)))
+
+    # ---- factor / solve / adjoint (new entry points) ---------------------------
+    def pdgetrf(self, palg: PAlg, mat: PMat) -> int:
+        return self._chk(lib().hplx_pdgetrf(self.h, ctypes.byref(palg),
+                                            ctypes.byref(mat)))
+
+    def pdgetrs(self, mat: PMat, trans: int, nrhs: int, B: int,
+                ldb: int) -> None:
+        self._chk(lib().hplx_pdgetrs(self.h, ctypes.byref(mat), trans, nrhs,
+                                     ctypes.c_void_p(B), ldb))
+
+    def pdgesv_adjoint(self, mat: PMat, nrhs: int, X: int, ldx: int, G: int,
+                       ldg: int, gradB: int, ldgb: int, gradA: int | None,
+                       ldga: int) -> None:
+        self._chk(lib().hplx_pdgesv_adjoint(
+            self.h, ctypes.byref(mat), nrhs, ctypes.c_void_p(X), ldx,
+            ctypes.c_void_p(G), ldg, ctypes.c_void_p(gradB), ldgb,
+            ctypes.c_void_p(gradA) if gradA else None, ldga))
 
     # ---- HPL_pdtest verification --------------------------------------------
     def pdtest_check(self, mat: PMat, seed: int = HPL_ISEED,

@@ -18,6 +18,8 @@
 // loop the back-solve U x = b' finishes the solve (HPL_pdtrsv).
 #include <cstdarg>
 #include <cstdio>
+#include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -104,6 +106,16 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
     return HPLX_ERR_HIP;
   }
   c.stream = c.own_stream;
+  {
+    int per_cu = 0;
+    if(hplx::leaf_occupancy(&per_cu) != hipSuccess || per_cu < 1) {
+      hipStreamDestroy(c.own_stream);
+      delete ctx;
+      return HPLX_ERR_HIP;
+    }
+    c.leaf_resident = per_cu * c.num_cus;
+  }
+  if(const char* pe = std::getenv("HPLX_PROGRESS")) c.progress_every = std::atoi(pe);
   // small workspace
   const size_t nLinv  = static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB * 8;
   const size_t nperm  = 2 * hplx::kMaxNB * sizeof(int);
@@ -111,8 +123,7 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
                         hplx::kSlotStride * sizeof(double);
   size_t off_perm  = align_up(nLinv, 256);
   size_t off_slots = align_up(off_perm + nperm, 256);
-  size_t off_cnt   = align_up(off_slots + nslots, 256);
-  size_t off_flags = off_cnt + 256;
+  size_t off_flags = align_up(off_slots + nslots, 256);
   size_t off_dout  = off_flags + 256;
   size_t total     = off_dout + 256;
   if(hipMalloc(&c.small_base, total) != hipSuccess) {
@@ -124,7 +135,6 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   c.Linv    = reinterpret_cast<double*>(b);
   c.permU   = reinterpret_cast<int*>(b + off_perm);
   c.slots   = reinterpret_cast<double*>(b + off_slots);
-  c.counter = reinterpret_cast<uint32_t*>(b + off_cnt);
   c.flags   = reinterpret_cast<int*>(b + off_flags);
   c.dout    = reinterpret_cast<double*>(b + off_dout);
   if(hipMemset(c.small_base, 0, total) != hipSuccess ||
@@ -144,6 +154,7 @@ int hplx_ctx_destroy(hplx_ctx* ctx) {
   hipSetDevice(c.device);
   hipStreamSynchronize(c.stream);
   for(hipEvent_t e : c.ev) hipEventDestroy(e);
+  for(hipEvent_t e : c.ev_prog) hipEventDestroy(e);
   if(c.small_base) hipFree(c.small_base);
   if(c.own_stream) hipStreamDestroy(c.own_stream);
   delete ctx;
@@ -294,18 +305,20 @@ static int check_palg(Ctx& c, const hplx_palg* p) {
   return HPLX_SUCCESS;
 }
 
-int hplx_pdgesv(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat) {
-  int r = use(ctx);
-  if(r) return r;
-  Ctx& c = ctx->c;
+// The panel loop over the first `ncols` columns of mat->A (n+1 with b for
+// hplx_pdgesv, n for hplx_pdgetrf).  Returns 0, an error code, or the
+// 1-based column of the first exact zero pivot.
+static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
+                       int ncols) {
+  int r;
   if((r = check_palg(c, palg))) return r;
   if(!mat || !mat->A || !mat->ws) return fail(c, HPLX_ERR_INVALID, "bad pmat");
   const int N   = mat->n;
   const int nb  = mat->nb;
   const int lda = mat->ld;
-  if(N > c.num_cus * hplx::kLeafThreads * hplx::kLeafMaxRPT)
+  if(N > hplx::max_leaf_rows(c))
     return fail(c, HPLX_ERR_UNSUPPORTED, "N=%d exceeds the panel kernel's %d rows",
-                N, c.num_cus * hplx::kLeafThreads * hplx::kLeafMaxRPT);
+                N, hplx::max_leaf_rows(c));
   hplx::MatWs* ws = static_cast<hplx::MatWs*>(mat->ws);
   double*      A  = mat->A;
   hipStream_t  s  = c.stream;
@@ -318,6 +331,14 @@ int hplx_pdgesv(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat) {
   }
   c.ev_pairs   = 0;
   c.gemm_flops = 0.0;
+  constexpr int kProgLag = 8;  // panels queued ahead of the reported one
+  if(c.progress_every > 0)
+    while(static_cast<int>(c.ev_prog.size()) < 2 * kProgLag) {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      c.ev_prog.push_back(e);
+    }
+  const auto t_start = std::chrono::steady_clock::now();
 
   HIPCHK(c, hplx::reset_panel_state(c));
   for(int j = 0; j < N; j += nb) {
@@ -326,40 +347,172 @@ int hplx_pdgesv(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat) {
     double*   P  = A + j + static_cast<size_t>(j) * lda;
     HIPCHK(c, hplx::panel_factor(c, palg->nbmin, palg->nbdiv, P, lda, m, jb,
                                  mat->ipiv + j, j));
-    const int nt = N + 1 - j - jb;
-    if(nt <= 0) continue;
-    double* A12 = A + j + static_cast<size_t>(j + jb) * lda;
-    HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
-    HIPCHK(c, hplx::launch_linv(s, jb, P, lda, c.Linv, nb));
-    HIPCHK(c, hplx::launch_swap_gather(s, jb, nt, A12, lda, c.permU, ws->W, nb));
-    HIPCHK(c, hplx::launch_dgemm(s, jb, nt, jb, 1.0, c.Linv, nb, ws->W, nb, 0.0,
-                                 A12, lda, true));
-    if(m - jb > 0) {
-      hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
-      HIPCHK(c, hipEventRecord(e0, s));
-      HIPCHK(c, hplx::launch_dgemm(s, m - jb, nt, jb, -1.0, P + jb, lda, A12,
-                                   lda, 1.0, A12 + jb, lda, false));
-      HIPCHK(c, hipEventRecord(e1, s));
-      c.ev_pairs++;
-      c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
+    const int nt = ncols - j - jb;
+    if(nt > 0) {
+      double* A12 = A + j + static_cast<size_t>(j + jb) * lda;
+      HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
+      HIPCHK(c, hplx::launch_linv(s, jb, P, lda, c.Linv, nb));
+      HIPCHK(c, hplx::launch_swap_gather(s, jb, nt, A12, lda, c.permU, ws->W, nb));
+      HIPCHK(c, hplx::launch_dgemm(s, jb, nt, jb, 1.0, c.Linv, nb, ws->W, nb,
+                                   0.0, A12, lda, true));
+      if(m - jb > 0) {
+        hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
+        HIPCHK(c, hipEventRecord(e0, s));
+        HIPCHK(c, hplx::launch_dgemm(s, m - jb, nt, jb, -1.0, P + jb, lda, A12,
+                                     lda, 1.0, A12 + jb, lda, false, 1));
+        HIPCHK(c, hipEventRecord(e1, s));
+        c.ev_pairs++;
+        c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
+      }
+    }
+    if(c.progress_every > 0) {
+      // report panels the device has finished, never more than kProgLag
+      // behind what is queued
+      const int k = j / nb;
+      HIPCHK(c, hipEventRecord(c.ev_prog[k % (2 * kProgLag)], s));
+      const int done = k - kProgLag;
+      if(done >= 0) {
+        HIPCHK(c, hipEventSynchronize(c.ev_prog[done % (2 * kProgLag)]));
+        if(done % c.progress_every == 0) {
+          const double el = std::chrono::duration<double>(
+                                std::chrono::steady_clock::now() - t_start)
+                                .count();
+          std::fprintf(stderr, "[hplx] panel %d/%d done (column %d of %d) "
+                               "%.2f s\n",
+                       done + 1, npanels, (done + 1) * nb, N, el);
+          std::fflush(stderr);
+        }
+      }
     }
   }
-  // zero pivot or barrier failure?  (one synchronisation, before the solve)
+  // zero pivot or barrier failure?  (one synchronisation)
   int flags[2] = {0, 0};
   HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
                            s));
   HIPCHK(c, hipStreamSynchronize(s));
   if(flags[1])
     return fail(c, HPLX_ERR_HIP,
-                "panel kernel grid barrier timed out (workgroups not "
-                "co-resident)");
+                "panel kernel wait timed out (workgroups not co-resident)");
   mat->info = flags[0];
-  if(flags[0]) return flags[0];
-  HIPCHK(c, hipMemcpyAsync(mat->X, A + static_cast<size_t>(N) * lda,
+  return flags[0];
+}
+
+int hplx_pdgesv(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  r      = factor_loop(c, palg, mat, mat ? mat->n + 1 : 0);
+  if(r) return r;
+  const int N = mat->n;
+  hipStream_t s = c.stream;
+  HIPCHK(c, hipMemcpyAsync(mat->X, mat->A + static_cast<size_t>(N) * mat->ld,
                            static_cast<size_t>(N) * sizeof(double),
                            hipMemcpyDeviceToDevice, s));
-  HIPCHK(c, hplx::launch_trsv(s, N, nb, A, lda, mat->X));
+  HIPCHK(c, hplx::launch_trsv(s, N, mat->nb, mat->A, mat->ld, mat->X));
   HIPCHK(c, hipStreamSynchronize(s));
+  return HPLX_SUCCESS;
+}
+
+int hplx_pdgetrf(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat) {
+  int r = use(ctx);
+  if(r) return r;
+  return factor_loop(ctx->c, palg, mat, mat ? mat->n : 0);
+}
+
+int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
+                 double* B, int ldb) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(!mat || !mat->A || nrhs < 0 || (nrhs > 0 && (!B || ldb < mat->n)) ||
+     (trans != 0 && trans != 1))
+    return fail(c, HPLX_ERR_INVALID, "pdgetrs: bad arguments");
+  if(nrhs == 0) return HPLX_SUCCESS;
+  const int    N = mat->n, nb = mat->nb, lda = mat->ld;
+  const double* A = mat->A;
+  hipStream_t  s = c.stream;
+  const int    np = (N + nb - 1) / nb;
+  if(!trans) {
+    // L solve with the panel-by-panel interchanges, as b is carried through
+    // the factorisation (HPL_pdgesv), then U x = y (HPL_pdtrsv)
+    for(int p = 0; p < np; ++p) {
+      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb;
+      HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
+      HIPCHK(c, hplx::launch_swap_inplace(s, jb, nrhs, B + j, ldb, c.permU));
+      HIPCHK(c, hplx::launch_tri_solve(s, true, false, true, jb, nrhs,
+                                       A + j + static_cast<size_t>(j) * lda,
+                                       lda, B + j, ldb));
+      if(N - j - jb > 0)
+        HIPCHK(c, hplx::launch_dgemm(s, N - j - jb, nrhs, jb, -1.0,
+                                     A + j + jb + static_cast<size_t>(j) * lda,
+                                     lda, B + j, ldb, 1.0, B + j + jb, ldb,
+                                     false));
+    }
+    for(int p = np - 1; p >= 0; --p) {
+      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb;
+      HIPCHK(c, hplx::launch_tri_solve(s, false, false, false, jb, nrhs,
+                                       A + j + static_cast<size_t>(j) * lda,
+                                       lda, B + j, ldb));
+      if(j > 0)
+        HIPCHK(c, hplx::launch_dgemm(s, j, nrhs, jb, -1.0,
+                                     A + static_cast<size_t>(j) * lda, lda,
+                                     B + j, ldb, 1.0, B, ldb, false));
+    }
+  } else {
+    // A = P0^T L0 P1^T L1 ... U  =>  A^T z = g  is solved as
+    // w = U^{-T} g, then for panels from the last: z_p -= L21^T z_below,
+    // z_p = L11^{-T} z_p, z = P_p^T z.
+    for(int p = 0; p < np; ++p) {
+      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb, e = j + jb;
+      HIPCHK(c, hplx::launch_tri_solve(s, true, true, false, jb, nrhs,
+                                       A + j + static_cast<size_t>(j) * lda,
+                                       lda, B + j, ldb));
+      if(e < N)
+        HIPCHK(c, hplx::launch_dgemm(s, N - e, nrhs, jb, -1.0,
+                                     A + j + static_cast<size_t>(e) * lda, lda,
+                                     B + j, ldb, 1.0, B + e, ldb, false, 0,
+                                     true, false));
+    }
+    for(int p = np - 1; p >= 0; --p) {
+      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb, e = j + jb;
+      if(e < N)
+        HIPCHK(c, hplx::launch_dgemm(s, jb, nrhs, N - e, -1.0,
+                                     A + e + static_cast<size_t>(j) * lda, lda,
+                                     B + e, ldb, 1.0, B + j, ldb, false, 0,
+                                     true, false));
+      HIPCHK(c, hplx::launch_tri_solve(s, false, true, true, jb, nrhs,
+                                       A + j + static_cast<size_t>(j) * lda,
+                                       lda, B + j, ldb));
+      HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
+      HIPCHK(c, hplx::launch_swap_inverse(s, jb, nrhs, B + j, ldb, c.permU));
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(s));
+  return HPLX_SUCCESS;
+}
+
+int hplx_pdgesv_adjoint(hplx_ctx* ctx, const hplx_pmat* mat, int nrhs,
+                        const double* X, int ldx, const double* G, int ldg,
+                        double* gradB, int ldgb, double* gradA, int ldga) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(!mat || !mat->A || nrhs < 1 || !X || !G || !gradB || ldx < mat->n ||
+     ldg < mat->n || ldgb < mat->n || (gradA && ldga < mat->n))
+    return fail(c, HPLX_ERR_INVALID, "pdgesv_adjoint: bad arguments");
+  const int N = mat->n;
+  HIPCHK(c, hipMemcpy2DAsync(gradB, static_cast<size_t>(ldgb) * sizeof(double),
+                             G, static_cast<size_t>(ldg) * sizeof(double),
+                             static_cast<size_t>(N) * sizeof(double), nrhs,
+                             hipMemcpyDeviceToDevice, c.stream));
+  if((r = hplx_pdgetrs(ctx, mat, 1, nrhs, gradB, ldgb))) return r;
+  if(gradA) {
+    // grad_A = -grad_B X^T: N x N from rank nrhs (HBM-bound store)
+    HIPCHK(c, hplx::launch_dgemm(c.stream, N, N, nrhs, -1.0, gradB, ldgb, X,
+                                 ldx, 0.0, gradA, ldga, false, 0, false,
+                                 true));
+    HIPCHK(c, hipStreamSynchronize(c.stream));
+  }
   return HPLX_SUCCESS;
 }
 
@@ -386,7 +539,7 @@ int hplx_pdfact(hplx_ctx* ctx, int nbmin, int nbdiv, double* P, int lda, int m,
   Ctx& c = ctx->c;
   if(!P || !ipiv || m < 1 || jb < 1 || jb > m || lda < m)
     return fail(c, HPLX_ERR_INVALID, "pdfact: m=%d jb=%d lda=%d", m, jb, lda);
-  if(jb > hplx::kMaxNB || m > c.num_cus * hplx::kLeafThreads * hplx::kLeafMaxRPT)
+  if(jb > hplx::kMaxNB || m > hplx::max_leaf_rows(c))
     return fail(c, HPLX_ERR_UNSUPPORTED, "pdfact: m=%d jb=%d too large", m, jb);
   HIPCHK(c, hplx::reset_panel_state(c));
   HIPCHK(c, hplx::panel_factor(c, nbmin, nbdiv, P, lda, m, jb, ipiv,

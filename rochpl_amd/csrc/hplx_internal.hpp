@@ -16,9 +16,10 @@ namespace hplx {
 // Limits of the kernels.
 // ---------------------------------------------------------------------------
 constexpr int kLeafMax     = 32;    // widest leaf the register-resident kernel takes
-constexpr int kLeafThreads = 256;   // threads per leaf workgroup
-constexpr int kLeafMaxRPT  = 4;     // rows per thread in the leaf kernel
-constexpr int kMaxGrid     = 1024;  // workgroups a cooperative leaf may use
+constexpr int kLeafThreads = 512;   // threads per leaf workgroup
+constexpr int kLeafRPT     = 3;     // rows per thread in the leaf kernel
+constexpr int kMaxLeafGrid = 256;   // workgroups a cooperative leaf may use
+constexpr int kMaxGrid     = kMaxLeafGrid;  // candidate slots per parity
 constexpr int kMaxNB       = 1024;  // widest panel (pivot composition, Linv)
 constexpr int kSlotStride  = 40;    // doubles per pivot candidate slot
 
@@ -31,16 +32,20 @@ struct Ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream     = nullptr;  // where work is queued
   std::string last_error;
-  uint64_t    barrier_epoch = 0;     // arrivals already counted on *counter
+  uint32_t    leaf_launches = 0;     // sequence numbers of the leaf slots
+  int         leaf_resident = 0;     // leaf workgroups resident at once
 
   void*     small_base = nullptr;
   double*   Linv       = nullptr;    // kMaxNB x kMaxNB
   int*      permU      = nullptr;    // 2*kMaxNB
   double*   slots      = nullptr;    // 2 x (kMaxGrid+1) x kSlotStride
-  uint32_t* counter    = nullptr;    // grid barrier arrivals
   int*      flags      = nullptr;    // [0] first zero pivot (1-based), [1] error
   double*   dout       = nullptr;    // 16 doubles of small results
 
+  // progress lines from the panel loop (HPLX_PROGRESS=<panels>, the
+  // counterpart of the reference's HPL_PROGRESS_REPORT build option)
+  int                     progress_every = 0;
+  std::vector<hipEvent_t> ev_prog;
   // timing of the trailing-update GEMM launches of the last solve
   std::vector<hipEvent_t> ev;
   int                     ev_pairs = 0;
@@ -58,7 +63,16 @@ struct MatWs {
 hipError_t launch_matgen(hipStream_t s, int m, int n, double* A, int lda,
                          int seed, double diag_shift);
 
-// Zero the barrier counter and the flags (queued on c.stream).
+// Largest panel height the register-resident leaf holds on this device.
+inline int max_leaf_rows(const Ctx& c) {
+  const int g = c.leaf_resident < kMaxLeafGrid ? c.leaf_resident : kMaxLeafGrid;
+  return g * kLeafThreads * kLeafRPT;
+}
+
+// Workgroups of the leaf kernel one CU holds (occupancy query).
+hipError_t leaf_occupancy(int* blocks_per_cu);
+
+// Zero the flags (queued on c.stream).
 hipError_t reset_panel_state(Ctx& c);
 
 // Panel factorisation of an m x jb panel in place (recursive right-looking,
@@ -73,16 +87,27 @@ hipError_t launch_swap_gather(hipStream_t s, int jb, int n, double* A,
                               int lda, const int* permU, double* W, int ldw);
 hipError_t launch_swap_inplace(hipStream_t s, int jb, int n, double* A,
                                int lda, const int* permU);
+hipError_t launch_swap_inverse(hipStream_t s, int jb, int n, double* A,
+                               int lda, const int* permU);
 hipError_t launch_linv(hipStream_t s, int jb, const double* L, int ldl,
                        double* Linv, int ldi);
+// op(T) x = b for nrhs columns of X; op(T) lower or upper as `lower` says
+// (T itself is upper when lower && trans, lower when !lower && trans).
+hipError_t launch_tri_solve(hipStream_t s, bool lower, bool trans, bool unit,
+                            int k, int nrhs, const double* T, int ldt,
+                            double* X, int ldx);
 hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const double* L,
                                 int ldl, double* B, int ldb);
 
-// C = beta*C + alpha*A*B; alpha in {+1,-1}, beta in {0,1}.  lower_a: A is
-// lower triangular (only k < row-tile end contributes).
+// C = beta*C + alpha*op(A)*op(B); alpha in {+1,-1}, beta in {0,1}.
+// lower_a: A is lower triangular (only k < row-tile end contributes).  kind 1
+// selects the separately named instance used for the trailing update.
+// trans_a / trans_b (one at a time, alpha = -1): op(X) = X^T.
 hipError_t launch_dgemm(hipStream_t s, int m, int n, int k, double alpha,
                         const double* A, int lda, const double* B, int ldb,
-                        double beta, double* C, int ldc, bool lower_a);
+                        double beta, double* C, int ldc, bool lower_a,
+                        int kind = 0, bool trans_a = false,
+                        bool trans_b = false);
 
 hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                        double* x);

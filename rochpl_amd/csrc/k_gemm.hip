@@ -12,12 +12,19 @@
 //   - workgroup tile 128 x 128, 4 waves in 2 x 2, each wave 64 x 64 as
 //     4 x 4 blocks of V_MFMA_F64_16X16X4_F64;
 //   - K streamed in 16-deep stages through double-buffered LDS (66 KB per
-//     workgroup, two workgroups per CU), one barrier per stage;
+//     workgroup, two workgroups per CU), one barrier per stage; the next
+//     stage's global loads are in flight during the current stage's MFMAs
+//     and the MFMA operands of the next k-step are read from LDS while the
+//     current k-step's MFMAs run;
+//   - global loads are unconditional (edge tiles clamp the address and zero
+//     the value), so no divergent branch forces an early wait on them;
+//     16-byte loads are used when A and B are 16-byte aligned with even
+//     leading dimensions and M, K even (always true for HPL's even NB);
 //   - the MFMA computes the transposed block (D = B^T A^T) so that each
 //     accumulator register of 16 lanes is 16 consecutive rows of one C
 //     column: C is read and written in 128-byte runs;
-//   - for C -= A*B the accumulators start from C and A is negated on its
-//     way into LDS, so the epilogue is a plain store;
+//   - for C -= A*B, A is negated on its way into LDS and the epilogue adds
+//     the accumulators to C;
 //   - workgroup -> tile mapping is XCD aware: the dispatcher deals
 //     workgroups to the 8 XCDs round robin, so workgroup ids are remapped to
 //     give each XCD a contiguous range of tiles, walked in groups of
@@ -36,11 +43,12 @@ constexpr int WM      = 64;
 constexpr int WN      = 64;
 constexpr int TM      = WM / 16;
 constexpr int TN      = WN / 16;
-constexpr int LDS_LD  = BM + 1;  // odd stride: conflict-free transposed B stores
+constexpr int LDS_LD  = BM + 2;  // even (16-byte rows), staggers LDS banks
 constexpr int kGroupM = 8;
 constexpr int kXcds   = 8;
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef double double2_t __attribute__((ext_vector_type(2)));
 
 // V_MFMA_F64_16X16X4_F64 operand layout (lane l of 64):
 //   a-operand: X[l % 16][l / 16]   (16 x 4)
@@ -48,19 +56,123 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 //   result:    D[4*r + l / 16][l % 16], r = 0..3
 __device__ inline int dl_row(int r, int lane) { return 4 * r + (lane >> 4); }
 
-template <bool BETA0, bool NEG, bool LOWER_A>
-__global__ __launch_bounds__(NT, 2) void dgemm_nn_kernel(
+// Stages one 128 x 16 tile of an operand (outer index o = row of A or
+// column of B, inner index k) from global memory into LDS as S[k][o].
+//   KFAST = false: o is the contiguous index (A, or B^T when B is transposed)
+//   KFAST = true:  k is the contiguous index (B, or A^T when A is transposed)
+// Loads are unconditional: out-of-range elements are read from a clamped
+// address and replaced by zero at the LDS store.
+template <bool KFAST, bool VEC2>
+struct TileLoader {
+  static constexpr int NQ = VEC2 ? 4 : 8;
+  double r[8];
+  bool   v[NQ];
+
+  __device__ void load(const double* __restrict__ P, int ld, int o0, int O,
+                       int k0, int kend) {
+    const int t = threadIdx.x;
+    if(!KFAST && VEC2) {  // pairs of o at k = (t>>6) + 4q
+      const int op = o0 + 2 * (t & 63);
+      const int oc = min(op, O - 2);
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const int k  = k0 + (t >> 6) + 4 * q;
+        v[q]         = op < O && k < kend;
+        const int kc = min(k, kend - 1);
+        const double2_t x = *reinterpret_cast<const double2_t*>(
+            P + oc + static_cast<size_t>(kc) * ld);
+        r[2 * q]     = x.x;
+        r[2 * q + 1] = x.y;
+      }
+    } else if(!KFAST) {  // o = t&127 at k = (t>>7) + 2q
+      const int o  = o0 + (t & 127);
+      const int oc = min(o, O - 1);
+#pragma unroll
+      for(int q = 0; q < 8; ++q) {
+        const int k  = k0 + (t >> 7) + 2 * q;
+        v[q]         = o < O && k < kend;
+        const int kc = min(k, kend - 1);
+        r[q]         = P[oc + static_cast<size_t>(kc) * ld];
+      }
+    } else if(VEC2) {  // pairs of k (t&7) in o = (t>>3) + 32q
+      const int k  = k0 + 2 * (t & 7);
+      const int kc = min(k, kend - 2);
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const int o  = o0 + (t >> 3) + 32 * q;
+        v[q]         = k < kend && o < O;
+        const int oc = min(o, O - 1);
+        const double2_t x = *reinterpret_cast<const double2_t*>(
+            P + kc + static_cast<size_t>(oc) * ld);
+        r[2 * q]     = x.x;
+        r[2 * q + 1] = x.y;
+      }
+    } else {  // k = t&15 in o = (t>>4) + 16q
+      const int k  = k0 + (t & 15);
+      const int kc = min(k, kend - 1);
+#pragma unroll
+      for(int q = 0; q < 8; ++q) {
+        const int o  = o0 + (t >> 4) + 16 * q;
+        v[q]         = k < kend && o < O;
+        const int oc = min(o, O - 1);
+        r[q]         = P[kc + static_cast<size_t>(oc) * ld];
+      }
+    }
+  }
+
+  __device__ void store(double* S, bool neg) const {
+    const int t = threadIdx.x;
+    if(!KFAST && VEC2) {
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const int k = (t >> 6) + 4 * q;
+        double2_t x;
+        x.x = v[q] ? (neg ? -r[2 * q] : r[2 * q]) : 0.0;
+        x.y = v[q] ? (neg ? -r[2 * q + 1] : r[2 * q + 1]) : 0.0;
+        *reinterpret_cast<double2_t*>(S + k * LDS_LD + 2 * (t & 63)) = x;
+      }
+    } else if(!KFAST) {
+#pragma unroll
+      for(int q = 0; q < 8; ++q) {
+        const int k = (t >> 7) + 2 * q;
+        S[k * LDS_LD + (t & 127)] = v[q] ? (neg ? -r[q] : r[q]) : 0.0;
+      }
+    } else if(VEC2) {
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const int o = (t >> 3) + 32 * q;
+        const int k = 2 * (t & 7);
+        S[k * LDS_LD + o]       = v[q] ? (neg ? -r[2 * q] : r[2 * q]) : 0.0;
+        S[(k + 1) * LDS_LD + o] = v[q] ? (neg ? -r[2 * q + 1] : r[2 * q + 1])
+                                       : 0.0;
+      }
+    } else {
+#pragma unroll
+      for(int q = 0; q < 8; ++q) {
+        const int o = (t >> 4) + 16 * q;
+        S[(t & 15) * LDS_LD + o] = v[q] ? (neg ? -r[q] : r[q]) : 0.0;
+      }
+    }
+  }
+};
+
+// KIND only names the instance: 1 = the trailing update A22 -= L21*U12, so
+// profiles report it apart from the panel-internal products.
+// TA / TB: the operand is stored transposed (op(A) = A^T, op(B) = B^T).
+template <bool BETA0, bool NEG, bool LOWER_A, bool TA, bool TB, bool VA,
+          bool VB, int KIND>
+__global__ __launch_bounds__(NT, 2) void dgemm_kernel(
     int M, int N, int K, const double* __restrict__ A, int lda,
     const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc,
     int tiles_m, int tiles_n) {
-  __shared__ double sA[2][BK * LDS_LD];
-  __shared__ double sB[2][BK * LDS_LD];
+  __shared__ __attribute__((aligned(16))) double sA[2][BK * LDS_LD];
+  __shared__ __attribute__((aligned(16))) double sB[2][BK * LDS_LD];
 
   // ---- XCD-aware tile mapping
-  const int T     = tiles_m * tiles_n;
-  const int per   = gridDim.x / kXcds;
-  const int bid   = blockIdx.x;
-  const int tile  = (bid % kXcds) * per + bid / kXcds;
+  const int T    = tiles_m * tiles_n;
+  const int per  = gridDim.x / kXcds;
+  const int bid  = blockIdx.x;
+  const int tile = (bid % kXcds) * per + bid / kXcds;
   if(tile >= T) return;
   const int width  = kGroupM * tiles_n;
   const int group  = tile / width;
@@ -81,86 +193,58 @@ __global__ __launch_bounds__(NT, 2) void dgemm_nn_kernel(
   const int l16  = lane & 15;
   const int lk   = lane >> 4;
 
-  // ---- accumulators
+  TileLoader<TA, VA>  la;   // A: o = row of C; k-contiguous when TA
+  TileLoader<!TB, VB> lb;   // B: o = column of C; k-contiguous unless TB
+  auto load_stage = [&](int k0) {
+    la.load(A, lda, m0, M, k0, kend);
+    lb.load(B, ldb, n0, N, k0, kend);
+  };
+  auto store_stage = [&](int buf) {
+    la.store(sA[buf], NEG);
+    lb.store(sB[buf], false);
+  };
+
+  if(kend > 0) load_stage(0);
+
+  // ---- accumulators start at zero; C is added in the epilogue, so no load
+  // of C is outstanding while the main loop runs
   double4_t acc[TM][TN];
 #pragma unroll
   for(int i = 0; i < TM; ++i)
 #pragma unroll
-    for(int j = 0; j < TN; ++j) {
-      if(BETA0) {
-        acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
-      } else {
-        const int row = m0 + wm0 + i * 16 + l16;
-#pragma unroll
-        for(int r = 0; r < 4; ++r) {
-          const int col = n0 + wn0 + j * 16 + dl_row(r, lane);
-          acc[i][j][r]  = (row < M && col < N)
-                              ? C[row + static_cast<size_t>(col) * ldc]
-                              : 0.0;
-        }
-      }
-    }
+    for(int j = 0; j < TN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
 
-  // ---- global -> register staging (8 doubles of A and of B per thread)
-  double ra[8], rb[8];
-  auto load_stage = [&](int k0) {
-#pragma unroll
-    for(int q = 0; q < 8; ++q) {
-      const int idx = t + NT * q;
-      // A: 128 rows x 16 k, rows fastest
-      const int r  = idx & (BM - 1);
-      const int kk = idx >> 7;
-      const int gr = m0 + r, gk = k0 + kk;
-      ra[q] = (gr < M && gk < kend) ? A[gr + static_cast<size_t>(gk) * lda]
-                                    : 0.0;
-      // B: 16 k x 128 columns, k fastest (16 threads read one 128-byte run)
-      const int bk = idx & (BK - 1);
-      const int bc = idx >> 4;
-      const int gk2 = k0 + bk, gc = n0 + bc;
-      rb[q] = (gk2 < kend && gc < N) ? B[gk2 + static_cast<size_t>(gc) * ldb]
-                                     : 0.0;
-    }
-  };
-  auto store_stage = [&](int buf) {
-#pragma unroll
-    for(int q = 0; q < 8; ++q) {
-      const int idx = t + NT * q;
-      const int r   = idx & (BM - 1);
-      const int kk  = idx >> 7;
-      sA[buf][kk * LDS_LD + r] = NEG ? -ra[q] : ra[q];
-      const int bk = idx & (BK - 1);
-      const int bc = idx >> 4;
-      sB[buf][bk * LDS_LD + bc] = rb[q];
-    }
-  };
-
-  if(kend > 0) {
-    load_stage(0);
-    store_stage(0);
-  }
+  if(kend > 0) store_stage(0);
   __syncthreads();
 
   int buf = 0;
   for(int k0 = 0; k0 < kend; k0 += BK) {
     const bool more = (k0 + BK) < kend;
     if(more) load_stage(k0 + BK);
-    const double* a_s = sA[buf];
-    const double* b_s = sB[buf];
+    const double* a_s = sA[buf] + lk * LDS_LD + wm0 + l16;
+    const double* b_s = sB[buf] + lk * LDS_LD + wn0 + l16;
+    double        fa[2][TM], fb[2][TN];
+#pragma unroll
+    for(int i = 0; i < TM; ++i) fa[0][i] = a_s[i * 16];
+#pragma unroll
+    for(int j = 0; j < TN; ++j) fb[0][j] = b_s[j * 16];
 #pragma unroll
     for(int ks = 0; ks < BK / 4; ++ks) {
-      double fa[TM], fb[TN];
-      const int krow = (ks * 4 + lk) * LDS_LD;
+      const int cur = ks & 1;
+      if(ks + 1 < BK / 4) {
+        const int off = (ks + 1) * 4 * LDS_LD;
 #pragma unroll
-      for(int i = 0; i < TM; ++i) fa[i] = a_s[krow + wm0 + i * 16 + l16];
+        for(int i = 0; i < TM; ++i) fa[cur ^ 1][i] = a_s[off + i * 16];
 #pragma unroll
-      for(int j = 0; j < TN; ++j) fb[j] = b_s[krow + wn0 + j * 16 + l16];
+        for(int j = 0; j < TN; ++j) fb[cur ^ 1][j] = b_s[off + j * 16];
+      }
 #pragma unroll
       for(int i = 0; i < TM; ++i)
 #pragma unroll
         for(int j = 0; j < TN; ++j)
           // D = B^T-block x A^T-block: D[n][m] accumulates C[m][n]
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i],
-                                                           acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              fb[cur][j], fa[cur][i], acc[i][j], 0, 0, 0);
     }
     if(more) store_stage(buf ^ 1);
     __syncthreads();
@@ -178,13 +262,18 @@ __global__ __launch_bounds__(NT, 2) void dgemm_nn_kernel(
       for(int r = 0; r < 4; ++r) {
         const int col = n0 + wn0 + j * 16 + dl_row(r, lane);
         if(col < N) {
-          C[row + static_cast<size_t>(col) * ldc] = acc[i][j][r];
+          double* c = C + row + static_cast<size_t>(col) * ldc;
+          *c        = BETA0 ? acc[i][j][r] : *c + acc[i][j][r];
         }
       }
   }
 }
 
-template <bool BETA0, bool NEG, bool LOWER_A>
+bool aligned16(const double* p) {
+  return reinterpret_cast<uintptr_t>(p) % 16 == 0;
+}
+
+template <bool BETA0, bool NEG, bool LOWER_A, bool TA, bool TB, int KIND>
 hipError_t run(hipStream_t s, int m, int n, int k, const double* A, int lda,
                const double* B, int ldb, double* C, int ldc) {
   const int  tiles_m = (m + BM - 1) / BM;
@@ -192,9 +281,24 @@ hipError_t run(hipStream_t s, int m, int n, int k, const double* A, int lda,
   const long T       = static_cast<long>(tiles_m) * tiles_n;
   const long grid    = ((T + kXcds - 1) / kXcds) * kXcds;
   if(grid > 0x7fffffffL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((dgemm_nn_kernel<BETA0, NEG, LOWER_A>),
-                     dim3(static_cast<unsigned>(grid)), dim3(NT), 0, s, m, n,
-                     k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
+  // 16-byte staging: aligned operand, even leading dimension and an even
+  // extent along the pair direction (o for o-contiguous tiles, K for
+  // k-contiguous ones; a LOWER_A row tile's K bound m0 + BM is even too)
+  const bool va = aligned16(A) && lda % 2 == 0 && (TA ? k % 2 == 0 : m % 2 == 0);
+  const bool vb = aligned16(B) && ldb % 2 == 0 && (TB ? n % 2 == 0 : k % 2 == 0);
+  const dim3 g(static_cast<unsigned>(grid)), b(NT);
+  if(va && vb)
+    hipLaunchKernelGGL((dgemm_kernel<BETA0, NEG, LOWER_A, TA, TB, true, true, KIND>),
+                       g, b, 0, s, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
+  else if(va)
+    hipLaunchKernelGGL((dgemm_kernel<BETA0, NEG, LOWER_A, TA, TB, true, false, KIND>),
+                       g, b, 0, s, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
+  else if(vb)
+    hipLaunchKernelGGL((dgemm_kernel<BETA0, NEG, LOWER_A, TA, TB, false, true, KIND>),
+                       g, b, 0, s, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
+  else
+    hipLaunchKernelGGL((dgemm_kernel<BETA0, NEG, LOWER_A, TA, TB, false, false, KIND>),
+                       g, b, 0, s, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
   return hipGetLastError();
 }
 
@@ -202,28 +306,40 @@ hipError_t run(hipStream_t s, int m, int n, int k, const double* A, int lda,
 
 hipError_t launch_dgemm(hipStream_t s, int m, int n, int k, double alpha,
                         const double* A, int lda, const double* B, int ldb,
-                        double beta, double* C, int ldc, bool lower_a) {
+                        double beta, double* C, int ldc, bool lower_a,
+                        int kind, bool trans_a, bool trans_b) {
   if(m <= 0 || n <= 0) return hipSuccess;
   if(!(alpha == 1.0 || alpha == -1.0) || !(beta == 0.0 || beta == 1.0))
     return hipErrorInvalidValue;
   const bool beta0 = beta == 0.0;
   const bool neg   = alpha == -1.0;
-  if(k <= 0) {
-    if(!beta0) return hipSuccess;
-    // C = 0: run with K = 0 (accumulators stay zero)
+  if(k < 0) return hipErrorInvalidValue;
+  if(k == 0 && !beta0) return hipSuccess;  // C unchanged
+  if(trans_a || trans_b) {
+    // products of the adjoint solves: C -= op(A) op(B) or C = -op(A) op(B)
+    if(lower_a || !neg) return hipErrorInvalidValue;
+    if(trans_a && !trans_b)
+      return beta0 ? run<true, true, false, true, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
+                   : run<false, true, false, true, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
+    if(!trans_a && trans_b)
+      return beta0 ? run<true, true, false, false, true, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
+                   : run<false, true, false, false, true, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
+    return hipErrorInvalidValue;
   }
   if(beta0) {
     if(neg)
-      return lower_a ? run<true, true, true>(s, m, n, k, A, lda, B, ldb, C, ldc)
-                     : run<true, true, false>(s, m, n, k, A, lda, B, ldb, C, ldc);
-    return lower_a ? run<true, false, true>(s, m, n, k, A, lda, B, ldb, C, ldc)
-                   : run<true, false, false>(s, m, n, k, A, lda, B, ldb, C, ldc);
+      return lower_a ? run<true, true, true, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
+                     : run<true, true, false, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
+    return lower_a ? run<true, false, true, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
+                   : run<true, false, false, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
   }
+  if(neg && !lower_a && kind == 1)
+    return run<false, true, false, false, false, 1>(s, m, n, k, A, lda, B, ldb, C, ldc);
   if(neg)
-    return lower_a ? run<false, true, true>(s, m, n, k, A, lda, B, ldb, C, ldc)
-                   : run<false, true, false>(s, m, n, k, A, lda, B, ldb, C, ldc);
-  return lower_a ? run<false, false, true>(s, m, n, k, A, lda, B, ldb, C, ldc)
-                 : run<false, false, false>(s, m, n, k, A, lda, B, ldb, C, ldc);
+    return lower_a ? run<false, true, true, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
+                   : run<false, true, false, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
+  return lower_a ? run<false, false, true, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
+                 : run<false, false, false, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
 }
 
 }  // namespace hplx

@@ -40,45 +40,36 @@
 namespace hplx {
 namespace {
 
-__global__ void reset_kernel(uint32_t* counter, int* info, int* err) {
-  *counter = 0;
-  *info    = 0;
-  *err     = 0;
+__global__ void zero_kernel(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = 0.0;
 }
 
-// Solve L X = B in place; L unit lower k x k.  One workgroup per column of
-// B; with `identity` B starts as the identity (explicit inverse of L).
-__global__ __launch_bounds__(256) void trsm_colsweep_kernel(
-    int k, const double* __restrict__ L, int ldl, double* __restrict__ B,
-    int ldb, int identity) {
-  __shared__ double sx[kMaxNB];
-  const int col = blockIdx.x;
-  double*   bc  = B + static_cast<size_t>(col) * ldb;
-  for(int i = threadIdx.x; i < k; i += blockDim.x)
-    sx[i] = identity ? (i == col ? 1.0 : 0.0) : bc[i];
-  __syncthreads();
-  const int p0 = identity ? col : 0;
-  for(int p = p0; p < k - 1; ++p) {
-    const double xp = sx[p];
-    if(xp != 0.0) {
-      const double* lc = L + static_cast<size_t>(p) * ldl;
-      for(int i = p + 1 + threadIdx.x; i < k; i += blockDim.x)
-        sx[i] = fma(-lc[i], xp, sx[i]);
-    }
-    __syncthreads();
-  }
-  for(int i = threadIdx.x; i < k; i += blockDim.x) bc[i] = sx[i];
+__global__ void reset_kernel(int* info, int* err) {
+  *info = 0;
+  *err  = 0;
 }
 
 // Device HPL_pipid: compose jb interchanges (row i <-> ipiv[i]-base) into
 // permU[0:jb] (source of each top row) and permU[jb:2jb] (destination of
-// each displaced top row, or -1).
+// each displaced top row, or -1), without a serial pass:
+//   position i < jb is last written at step i, with what position p_i held
+//   just before step i; a position q is written before step i only by the
+//   latest earlier step i' with p_i' = q, which moved there what position i'
+//   held just before step i'.  With g(i) = content of position i just before
+//   step i:
+//     g(i)       = g(last i' < i with p_i' = i), or i if none
+//     top[i]     = g(i) if p_i == i, else g(last i' < i with p_i' = p_i), or
+//                  p_i if none
+//     a position q >= jb ends with g(last i with p_i = q).
+//   Chains of g only go to smaller indices and are followed per thread.
 __global__ __launch_bounds__(1024) void pipid_kernel(
     int jb, const int* __restrict__ ipiv, int base, int* __restrict__ permU) {
   __shared__ int sp[kMaxNB];
-  __shared__ int sfirst[kMaxNB];
-  __shared__ int slab[kMaxNB];
-  __shared__ int splab[kMaxNB];
+  __shared__ int sprev[kMaxNB];   // last i' < i with p_i' == p_i, or -1
+  __shared__ int sto[kMaxNB];     // last i' < i with p_i' == i, or -1
+  __shared__ int slast[kMaxNB];   // 1 if no later step targets p_i
+  __shared__ int sg[kMaxNB];
   __shared__ int sout[2 * kMaxNB];
   const int t = threadIdx.x;
   // an interchange must target a row at or below its own; anything else is
@@ -88,48 +79,42 @@ __global__ __launch_bounds__(1024) void pipid_kernel(
     sp[i]       = p < i ? i : p;
   }
   __syncthreads();
-  // slot of an outside position = index of its first occurrence
   for(int i = t; i < jb; i += blockDim.x) {
-    int f = -1;
-    if(sp[i] >= jb) {
-      f = i;
-      for(int k = 0; k < i; ++k)
-        if(sp[k] == sp[i]) {
-          f = k;
-          break;
-        }
-    }
-    sfirst[i] = f;
-  }
-  __syncthreads();
-  if(t == 0) {
-    for(int i = 0; i < jb; ++i) {
-      slab[i] = i;
-      if(sfirst[i] == i) splab[i] = sp[i];
-    }
-    for(int i = 0; i < jb; ++i) {
-      const int p = sp[i];
-      if(p < jb) {
-        const int x = slab[i];
-        slab[i]     = slab[p];
-        slab[p]     = x;
-      } else {
-        const int s = sfirst[i];
-        const int x = slab[i];
-        slab[i]     = splab[s];
-        splab[s]    = x;
+    int prev = -1, to = -1, last = 1;
+    const int p = sp[i];
+    for(int k2 = 0; k2 < jb; ++k2) {
+      const int q = sp[k2];
+      if(k2 < i) {
+        if(q == p) prev = k2;
+        if(q == i) to = k2;
+      } else if(k2 > i && q == p) {
+        last = 0;
       }
     }
+    sprev[i] = prev;
+    sto[i]   = to;
+    slast[i] = last;
   }
   __syncthreads();
   for(int i = t; i < jb; i += blockDim.x) {
-    sout[i]      = slab[i];
+    int j = i;
+    while(sto[j] >= 0) j = sto[j];  // strictly decreasing chain
+    sg[i] = j;
+  }
+  __syncthreads();
+  for(int i = t; i < jb; i += blockDim.x) {
+    const int p = sp[i];
+    int       top;
+    if(p == i)
+      top = sg[i];
+    else
+      top = sprev[i] >= 0 ? sg[sprev[i]] : p;
+    sout[i]      = top;
     sout[jb + i] = -1;
   }
   __syncthreads();
-  // positions >= jb only ever receive rows that started in the top jb
   for(int i = t; i < jb; i += blockDim.x)
-    if(sfirst[i] == i) sout[jb + splab[i]] = sp[i];
+    if(sp[i] >= jb && slast[i]) sout[jb + sg[i]] = sp[i];
   __syncthreads();
   for(int i = t; i < 2 * jb; i += blockDim.x) permU[i] = sout[i];
 }
@@ -190,21 +175,58 @@ __global__ __launch_bounds__(256) void swap_inplace_kernel(
   }
 }
 
+// Inverse of the composed interchanges (the transpose P^T of the row
+// permutation, applied by the adjoint solve): what the forward swap moved
+// from row src(i) to top row i goes back, and each displaced top row t
+// returns from dst(t).
+__global__ __launch_bounds__(256) void swap_inverse_kernel(
+    int jb, double* __restrict__ A, int lda, const int* __restrict__ permU) {
+  constexpr int Q = kMaxNB / 256;
+  __shared__ double sv[kMaxNB];
+  __shared__ double sd[kMaxNB];
+  const int col = blockIdx.x;
+  double*   ac  = A + static_cast<size_t>(col) * lda;
+#pragma unroll
+  for(int q = 0; q < Q; ++q) {
+    const int i = threadIdx.x + 256 * q;
+    if(i < jb) {
+      sv[i]       = ac[i];
+      const int d = permU[jb + i];
+      sd[i]       = d >= 0 ? ac[d] : 0.0;
+    }
+  }
+  __syncthreads();  // every value is read before any is written
+#pragma unroll
+  for(int q = 0; q < Q; ++q) {
+    const int i = threadIdx.x + 256 * q;
+    if(i < jb) {
+      ac[permU[i]] = sv[i];
+      if(permU[jb + i] >= 0) ac[i] = sd[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
-hipError_t launch_leaf(Ctx& c, double* P, int lda, int m,
-                       int JB, int c0, int w, int* ipiv, int row_offset,
-                       int info_base) {
-  const int R    = m - c0;
-  const int Gmax = c.num_cus < kMaxGrid ? c.num_cus : kMaxGrid;
-  int       rpt  = 1;
-  while(rpt < kLeafMaxRPT &&
-        (R + kLeafThreads * rpt - 1) / (kLeafThreads * rpt) > Gmax)
-    rpt *= 2;
-  const int G = (R + kLeafThreads * rpt - 1) / (kLeafThreads * rpt);
-  if(G > Gmax || G < 1 || w < 1 || w > kLeafMax) return hipErrorInvalidValue;
+hipError_t launch_leaf(Ctx& c, double* P, int lda, int m, int JB, int c0,
+                       int w, int* ipiv, int row_offset, int info_base) {
+  constexpr int CH   = kLeafThreads * kLeafRPT;
+  const int     R    = m - c0;
+  const int     G    = (R + CH - 1) / CH;
+  if(G > kMaxLeafGrid || G < 1 || w < 1 || w > kLeafMax)
+    return hipErrorInvalidValue;
+  // sequence numbers: launch << 6 | column; slots restart when they wrap
+  if(c.leaf_launches >= (1u << 26) - 1) {
+    const int n = 2 * (kMaxGrid + 1) * kSlotStride;
+    hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                       c.stream, c.slots, n);
+    hipError_t e = hipGetLastError();
+    if(e != hipSuccess) return e;
+    c.leaf_launches = 0;
+  }
+  ++c.leaf_launches;
   leaf::LeafArgs la;
   la.P          = P;
   la.lda        = lda;
@@ -217,19 +239,20 @@ hipError_t launch_leaf(Ctx& c, double* P, int lda, int m,
   la.info       = c.flags;
   la.info_base  = info_base;
   la.slots      = c.slots;
-  la.counter    = c.counter;
-  la.base       = static_cast<uint32_t>(c.barrier_epoch);
+  la.seq_base   = c.leaf_launches << 6;
   la.err        = c.flags + 1;
-  void*       args[] = {&la};
-  const void* fn     = rpt == 1   ? leaf_kernel_ptr<1>()
-                       : rpt == 2 ? leaf_kernel_ptr<2>()
-                                  : leaf_kernel_ptr<4>();
-  hipError_t e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(kLeafThreads),
-                                            args, 0, c.stream);
-  if(e == hipSuccess)
-    c.barrier_epoch += static_cast<uint64_t>(w) * static_cast<uint64_t>(G);
-  return e;
+  if(G > c.leaf_resident) return hipErrorInvalidValue;
+  // A plain launch: every workgroup is resident at once because the grid is
+  // capped by the occupancy of this kernel (one workgroup per CU) and the
+  // solve queues nothing else concurrently; the kernel's bounded waits turn
+  // any violation into an error instead of a hang.  (The cooperative-launch
+  // API adds nothing here and refuses to launch while another process holds
+  // the device.)
+  void* args[] = {&la};
+  return hipLaunchKernel(leaf_kernel_ptr<kLeafRPT>(), dim3(G),
+                         dim3(kLeafThreads), args, 0, c.stream);
 }
+
 
 // Columns [c0, c0+n) of the panel, rows [c0, m).
 hipError_t leaf_or_chunks(Ctx& c, double* P, int lda, int m,
@@ -290,10 +313,14 @@ hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, double* P,
 
 }  // namespace
 
+hipError_t leaf_occupancy(int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, leaf_kernel_ptr<kLeafRPT>(), kLeafThreads, 0);
+}
+
 hipError_t reset_panel_state(Ctx& c) {
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(1), 0, c.stream, c.counter,
-                     c.flags, c.flags + 1);
-  c.barrier_epoch = 0;
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(1), 0, c.stream, c.flags,
+                     c.flags + 1);
   return hipGetLastError();
 }
 
@@ -302,7 +329,7 @@ hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, double* P, int lda,
   if(nbmin < 1) nbmin = 1;
   if(nbdiv < 2) nbdiv = 2;
   if(jb > kMaxNB || jb > m) return hipErrorInvalidValue;
-  if(m > c.num_cus * kLeafThreads * kLeafMaxRPT) return hipErrorInvalidValue;
+  if(m > max_leaf_rows(c)) return hipErrorInvalidValue;
   return rpanel(c, nbmin, nbdiv, P, lda, m, jb, 0, jb, ipiv, row_offset,
                 row_offset);
 }
@@ -325,30 +352,21 @@ hipError_t launch_swap_gather(hipStream_t s, int jb, int n, double* A, int lda,
   return hipGetLastError();
 }
 
+hipError_t launch_swap_inverse(hipStream_t s, int jb, int n, double* A,
+                               int lda, const int* permU) {
+  if(jb <= 0 || n <= 0) return hipSuccess;
+  if(jb > kMaxNB) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(swap_inverse_kernel, dim3(n), dim3(256), 0, s, jb, A, lda,
+                     permU);
+  return hipGetLastError();
+}
+
 hipError_t launch_swap_inplace(hipStream_t s, int jb, int n, double* A,
                                int lda, const int* permU) {
   if(jb <= 0 || n <= 0) return hipSuccess;
   if(jb > kMaxNB) return hipErrorInvalidValue;
   hipLaunchKernelGGL(swap_inplace_kernel, dim3(n), dim3(256), 0, s, jb, A, lda,
                      permU);
-  return hipGetLastError();
-}
-
-hipError_t launch_linv(hipStream_t s, int jb, const double* L, int ldl,
-                       double* Linv, int ldi) {
-  if(jb <= 0) return hipSuccess;
-  if(jb > kMaxNB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(trsm_colsweep_kernel, dim3(jb), dim3(256), 0, s, jb, L,
-                     ldl, Linv, ldi, 1);
-  return hipGetLastError();
-}
-
-hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const double* L,
-                                int ldl, double* B, int ldb) {
-  if(k <= 0 || n <= 0) return hipSuccess;
-  if(k > kMaxNB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(trsm_colsweep_kernel, dim3(n), dim3(256), 0, s, k, L, ldl,
-                     B, ldb, 0);
   return hipGetLastError();
 }
 

@@ -2,33 +2,16 @@
 //
 // The reference walks the NB-blocks from the bottom with rocblas_dtrsv on the
 // diagonal block and two rocblas_dgemv calls for the rows above
-// (:169-180,195-332).  Here each block is one single-workgroup column sweep
-// on the diagonal block (x kept in LDS) followed by one HBM-bound GEMV over
-// all rows above, with the block's columns split across the 4 waves of a
-// workgroup so that enough loads are in flight.  Every element of the upper
-// triangle is read exactly once.
+// (:169-180,195-332).  Here each block is one single-workgroup blocked solve
+// of the diagonal block (k_trsm.hip) followed by one HBM-bound GEMV over all
+// rows above,
+// with the block's columns split across the 4 waves of a workgroup so that
+// enough loads are in flight.  Every element of the upper triangle is read
+// exactly once.
 #include "hplx_internal.hpp"
 
 namespace hplx {
 namespace {
-
-// x[0:kb] = U^{-1} x[0:kb], U upper, non-unit, kb <= kMaxNB.
-__global__ __launch_bounds__(1024) void diag_solve_kernel(
-    int kb, const double* __restrict__ U, int ldu, double* __restrict__ x) {
-  __shared__ double sx[kMaxNB];
-  __shared__ double sf[kMaxNB];
-  const int t = threadIdx.x;
-  for(int i = t; i < kb; i += blockDim.x) sx[i] = x[i];
-  __syncthreads();
-  for(int p = kb - 1; p >= 0; --p) {
-    const double* uc = U + static_cast<size_t>(p) * ldu;
-    const double  xp = sx[p] / uc[p];
-    for(int i = t; i < p; i += blockDim.x) sx[i] = fma(-uc[i], xp, sx[i]);
-    if(t == 0) sf[p] = xp;
-    __syncthreads();
-  }
-  for(int i = t; i < kb; i += blockDim.x) x[i] = sf[i];
-}
 
 // y[0:m] -= U[0:m, 0:kb] * xs[0:kb].  Workgroup = 64 rows; wave w takes a
 // quarter of the columns; partial sums meet in LDS.
@@ -80,8 +63,10 @@ hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
   for(int J = nblk - 1; J >= 0; --J) {
     const int s0 = J * nb;
     const int kb = (n - s0) < nb ? (n - s0) : nb;
-    hipLaunchKernelGGL(diag_solve_kernel, dim3(1), dim3(1024), 0, s, kb,
-                       A + s0 + static_cast<size_t>(s0) * lda, lda, x + s0);
+    hipError_t e = launch_tri_solve(s, false, false, false, kb, 1,
+                                    A + s0 + static_cast<size_t>(s0) * lda,
+                                    lda, x + s0, n);
+    if(e != hipSuccess) return e;
     if(s0 > 0)
       hipLaunchKernelGGL(gemv_sub_kernel, dim3((s0 + 63) / 64), dim3(256), 0,
                          s, s0, kb, A + static_cast<size_t>(s0) * lda, lda,

@@ -1,9 +1,29 @@
 // Register-resident cooperative leaf of the panel factorisation
 // (HPL_pdpanrlT, src/pfact/HPL_pdpanrlT_device.cpp:23-468): one launch per
-// leaf of <= 32 columns, one grid barrier per column, pivot search by wave64
-// shuffles and LDS, no host round-trip.  Included by k_leaf{1,2,4}.hip, each
-// of which instantiates one rows-per-thread variant (the fully unrolled
-// column steps make each instance a large compile unit).
+// leaf of <= 32 columns.  Included by k_leaf{1,2,3}.hip, each of which
+// instantiates one rows-per-thread variant (the fully unrolled column steps
+// make each instance a large compile unit).
+//
+// Every workgroup keeps its slice of the leaf (up to 1536 rows x 32 columns)
+// in registers for the whole leaf.  Per column:
+//   1. each workgroup finds its candidate max |a| (wave64 shuffles, LDS);
+//   2. it publishes the candidate's value, row index and 32-wide row segment
+//      in its slot, then a sequence number with release ordering; the
+//      workgroup holding the current top row publishes that row likewise;
+//   3. every workgroup polls all slots' sequence numbers, then reads all
+//      candidates (values and row segments) in one pass into LDS, reduces
+//      them itself and takes the winner's row from LDS;
+//   4. registers are updated: row interchange, division by the pivot,
+//      rank-1 update of the remaining leaf columns.
+// Slots are double-buffered by column parity, so a workgroup that runs ahead
+// never overwrites a slot another workgroup still reads.  A grid of one
+// workgroup skips steps 2-3.  The reference instead parks its leaf on a
+// pinned host flag once per column and lets a host thread run HPL_pdmxswp
+// (:94-117,455-463).
+//
+// Every wait is bounded: after kSpinLimit polls a workgroup raises *err and
+// all workgroups leave the kernel, so a grid that is not co-resident cannot
+// hang the device.
 #pragma once
 
 #include <climits>
@@ -14,85 +34,51 @@ namespace hplx {
 namespace leaf {
 
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int      kWaves     = kLeafThreads / 64;
+// slot layout (doubles): [0] |value|, [1] row, [2] sequence, [4..35] row
+constexpr int kSlotVal = 0, kSlotRow = 1, kSlotSeq = 2, kSlotData = 4;
 
 __device__ inline bool better(double v, int r, double bv, int br) {
   return v > bv || (v == bv && r < br);
 }
 
-__device__ inline void st_agent(double* p, double v) {
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
-                     static_cast<unsigned long long>(__double_as_longlong(v)),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline double ld_agent(const double* p) {
-  return __longlong_as_double(static_cast<long long>(__hip_atomic_load(
+__device__ inline unsigned long long ld_seq(const double* p) {
+  return __hip_atomic_load(
       reinterpret_cast<unsigned long long*>(const_cast<double*>(p)),
-      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid-wide barrier over a monotonically increasing arrival counter.  Every
-// wait is bounded: after kSpinLimit polls the workgroup raises *err and all
-// workgroups leave the kernel, so a grid that is not co-resident cannot hang
-// the device.  Returns false once *err is set.
-__device__ inline bool grid_sync(uint32_t* counter, uint32_t target, int* err,
-                          int* s_flag) {
-  __threadfence();
-  __syncthreads();
-  if(threadIdx.x == 0) {
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    int      ok    = 1;
-    uint32_t spins = 0;
-    while(static_cast<int32_t>(__hip_atomic_load(counter, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT) -
-                               target) < 0) {
-      ++spins;
-      if((spins & 1023u) == 0 &&
-         __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        ok = 0;
-        break;
-      }
-      if(spins > kSpinLimit) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    *s_flag = ok;
-  }
-  __syncthreads();
-  return *s_flag != 0;
+__device__ inline void st_seq(double* p, unsigned long long v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), v,
+                     __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct LeafArgs {
-  double*   P;          // panel top-left (panel row 0)
-  int       lda;
-  int       m;          // panel rows
-  int       JB;         // panel width (row interchanges span it)
-  int       c0;         // first leaf column (panel relative)
-  int       w;          // leaf width, <= 32
-  int*      ipiv;       // panel-relative pivots (+ row_offset) written here
-  int       row_offset;
-  int*      info;       // first exact zero pivot, 1-based global column
-  int       info_base;  // global column of panel column 0
-  double*   slots;      // 2 x (kMaxGrid+1) x kSlotStride
-  uint32_t* counter;
-  uint32_t  base;       // counter value when this launch starts
-  int*      err;
+  double*  P;          // panel top-left (panel row 0)
+  int      lda;
+  int      m;          // panel rows
+  int      JB;         // panel width (row interchanges span it)
+  int      c0;         // first leaf column (panel relative)
+  int      w;          // leaf width, <= 32
+  int*     ipiv;       // panel-relative pivots (+ row_offset) written here
+  int      row_offset;
+  int*     info;       // first exact zero pivot, 1-based global column
+  int      info_base;  // global column of panel column 0
+  double*  slots;      // 2 x (kMaxGrid+1) x kSlotStride
+  uint32_t seq_base;   // unique per launch; column j publishes seq_base+j+1
+  int*     err;
 };
-
 
 // Shared-memory scratch of the leaf kernel.
 struct LeafShared {
+  double cand[kMaxLeafGrid][kLeafMax];  // every workgroup's candidate row
+  double cv[kMaxLeafGrid];
+  int    cr[kMaxLeafGrid];
   double u[kLeafMax];
   double top[kLeafMax];
-  double wv[kLeafThreads / 64];
-  int    wr[kLeafThreads / 64];
+  double wv[kWaves];
+  int    wr[kWaves];
   int    piv[kLeafMax];
-  int    flag;
   int    lab[kLeafMax];
   int    ppos[kLeafMax];
   int    plab[kLeafMax];
@@ -101,6 +87,51 @@ struct LeafShared {
   int    np;
 };
 
+// Workgroup max-loc of (v, r); every thread gets the result.
+__device__ inline void block_maxloc(double& v, int& r, LeafShared& sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o);
+    const int    orr = __shfl_xor(r, o);
+    if(better(ov, orr, v, r)) {
+      v = ov;
+      r = orr;
+    }
+  }
+  if(lane == 0) {
+    sh.wv[wave] = v;
+    sh.wr[wave] = r;
+  }
+  __syncthreads();
+  v = sh.wv[0];
+  r = sh.wr[0];
+#pragma unroll
+  for(int k = 1; k < kWaves; ++k)
+    if(better(sh.wv[k], sh.wr[k], v, r)) {
+      v = sh.wv[k];
+      r = sh.wr[k];
+    }
+}
+
+// Wait until slot s shows sequence `want`; false on timeout or error.
+__device__ inline bool poll_slot(const double* s, unsigned long long want,
+                                 int* err) {
+  uint32_t spins = 0;
+  while(ld_seq(s + kSlotSeq) != want) {
+    ++spins;
+    if((spins & 255u) == 0 &&
+       __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return false;
+    if(spins > kSpinLimit) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
 // One pivot step of the leaf for compile-time column JJ, so that every
 // register of the row tile is addressed statically (no scratch spill).
 template <int RPT, int JJ>
@@ -108,13 +139,11 @@ __device__ __forceinline__ void leaf_step(const LeafArgs& a,
                                           double (&v)[RPT][kLeafMax],
                                           const int (&rows)[RPT],
                                           LeafShared& sh, bool& alive) {
-  constexpr int CH   = kLeafThreads * RPT;
-  const int     t    = threadIdx.x;
-  const int     lane = t & 63;
-  const int     wave = t >> 6;
-  const int     b    = blockIdx.x;
-  const int     G    = gridDim.x;
-  const int     gc   = a.c0 + JJ;  // current top row == current column
+  constexpr int CH = kLeafThreads * RPT;
+  const int     t  = threadIdx.x;
+  const int     b  = blockIdx.x;
+  const int     G  = gridDim.x;
+  const int     gc = a.c0 + JJ;  // current top row == current column
   // ---- workgroup candidate: max |a(r, gc)| over rows r >= gc
   double bv = -1.0;
   int    br = INT_MAX;
@@ -128,95 +157,89 @@ __device__ __forceinline__ void leaf_step(const LeafArgs& a,
       }
     }
   }
+  block_maxloc(bv, br, sh);
+  int p;
+  if(G == 1) {
+    // ---- single workgroup: the candidate is the pivot
 #pragma unroll
-  for(int o = 32; o > 0; o >>= 1) {
-    const double ov  = __shfl_xor(bv, o);
-    const int    orr = __shfl_xor(br, o);
-    if(better(ov, orr, bv, br)) {
-      bv = ov;
-      br = orr;
-    }
-  }
-  if(lane == 0) {
-    sh.wv[wave] = bv;
-    sh.wr[wave] = br;
-  }
-  __syncthreads();
-  bv = sh.wv[0];
-  br = sh.wr[0];
+    for(int q = 0; q < RPT; ++q) {
+      if(rows[q] == br) {
 #pragma unroll
-  for(int k = 1; k < kLeafThreads / 64; ++k)
-    if(better(sh.wv[k], sh.wr[k], bv, br)) {
-      bv = sh.wv[k];
-      br = sh.wr[k];
-    }
-  // ---- publish candidate (value, row, leaf segment) and the top row
-  const size_t par   = static_cast<size_t>(JJ & 1) * (kMaxGrid + 1);
-  double*      slot  = a.slots + (par + b) * kSlotStride;
-  double*      tslot = a.slots + (par + kMaxGrid) * kSlotStride;
-  if(t == 0) {
-    st_agent(slot + 0, bv);
-    st_agent(slot + 1, static_cast<double>(br));
-  }
+        for(int c = 0; c < kLeafMax; ++c) sh.u[c] = v[q][c];
+      }
+      if(rows[q] == gc) {
 #pragma unroll
-  for(int q = 0; q < RPT; ++q) {
-    if(rows[q] == br) {
+        for(int c = 0; c < kLeafMax; ++c) sh.top[c] = v[q][c];
+      }
+    }
+    p = br;
+    __syncthreads();
+  } else {
+    // ---- publish (value, row, segment), then the sequence number
+    const unsigned long long seq =
+        static_cast<unsigned long long>(a.seq_base) + JJ + 1;
+    const size_t par   = static_cast<size_t>(JJ & 1) * (kMaxGrid + 1);
+    double*      slot  = a.slots + (par + b) * kSlotStride;
+    double*      tslot = a.slots + (par + kMaxGrid) * kSlotStride;
+    if(t == 0) {
+      slot[kSlotVal] = bv;
+      slot[kSlotRow] = static_cast<double>(br);
+    }
 #pragma unroll
-      for(int c = 0; c < kLeafMax; ++c) st_agent(slot + 2 + c, v[q][c]);
-    }
-    if(rows[q] == gc) {
+    for(int q = 0; q < RPT; ++q) {
+      if(rows[q] == br) {
 #pragma unroll
-      for(int c = 0; c < kLeafMax; ++c) st_agent(tslot + 2 + c, v[q][c]);
-    }
-  }
-  alive = grid_sync(a.counter,
-                    a.base + static_cast<uint32_t>(JJ + 1) *
-                                 static_cast<uint32_t>(G),
-                    a.err, &sh.flag);
-  if(!alive) return;
-  // ---- every workgroup reduces all candidates itself
-  double cv = -1.0;
-  int    cr = INT_MAX;
-  for(int i = t; i < G; i += kLeafThreads) {
-    const double* si = a.slots + (par + i) * kSlotStride;
-    const double  x  = ld_agent(si);
-    const int     r  = static_cast<int>(ld_agent(si + 1));
-    if(better(x, r, cv, cr)) {
-      cv = x;
-      cr = r;
-    }
-  }
+        for(int c = 0; c < kLeafMax; ++c) slot[kSlotData + c] = v[q][c];
+      }
+      if(rows[q] == gc) {
 #pragma unroll
-  for(int o = 32; o > 0; o >>= 1) {
-    const double ov  = __shfl_xor(cv, o);
-    const int    orr = __shfl_xor(cr, o);
-    if(better(ov, orr, cv, cr)) {
-      cv = ov;
-      cr = orr;
+        for(int c = 0; c < kLeafMax; ++c) tslot[kSlotData + c] = v[q][c];
+      }
     }
-  }
-  if(lane == 0) {
-    sh.wv[wave] = cv;
-    sh.wr[wave] = cr;
-  }
-  __syncthreads();
-  cv = sh.wv[0];
-  cr = sh.wr[0];
-#pragma unroll
-  for(int k = 1; k < kLeafThreads / 64; ++k)
-    if(better(sh.wv[k], sh.wr[k], cv, cr)) {
-      cv = sh.wv[k];
-      cr = sh.wr[k];
+    __threadfence();  // this thread's slot stores reach the coherence point
+    __syncthreads();
+    if(t == 0) {
+      st_seq(slot + kSlotSeq, seq);
+      // the top row gc < c0 + 32 always lives in workgroup 0
+      if(b == 0) st_seq(tslot + kSlotSeq, seq);
     }
-  const int     p  = cr;
-  const int     wb = (p - a.c0) / CH;
-  const double* ws = a.slots + (par + wb) * kSlotStride;
-  if(t < kLeafMax)
-    sh.u[t] = ld_agent(ws + 2 + t);
-  else if(t < 2 * kLeafMax)
-    sh.top[t - kLeafMax] = ld_agent(tslot + 2 + (t - kLeafMax));
+    // ---- wait for every candidate and the top row, then read them all
+    int ok = 1;
+    if(t < G)
+      ok = poll_slot(a.slots + (par + t) * kSlotStride, seq, a.err);
+    else if(t == G)
+      ok = poll_slot(tslot, seq, a.err);
+    if(!__syncthreads_and(ok)) {
+      alive = false;
+      return;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for(int i = t; i < G * kLeafMax; i += kLeafThreads) {
+      const int g = i / kLeafMax, c = i % kLeafMax;
+      sh.cand[g][c] = a.slots[(par + g) * kSlotStride + kSlotData + c];
+    }
+    if(t < G) {
+      const double* s = a.slots + (par + t) * kSlotStride;
+      sh.cv[t]        = s[kSlotVal];
+      sh.cr[t]        = static_cast<int>(s[kSlotRow]);
+    } else if(t >= kLeafThreads - kLeafMax) {
+      const int c = t - (kLeafThreads - kLeafMax);
+      sh.top[c]   = tslot[kSlotData + c];
+    }
+    __syncthreads();
+    double cv = -1.0;
+    int    cr = INT_MAX;
+    if(t < G) {
+      cv = sh.cv[t];
+      cr = sh.cr[t];
+    }
+    block_maxloc(cv, cr, sh);
+    p            = cr;
+    const int wb = (p - a.c0) / CH;
+    if(t < kLeafMax) sh.u[t] = sh.cand[wb][t];
+    __syncthreads();
+  }
   if(t == 0) sh.piv[JJ] = p;
-  __syncthreads();
   const double piv = sh.u[JJ];
   if(b == 0 && t == 0) {
     a.ipiv[gc] = p + a.row_offset;
@@ -268,7 +291,7 @@ struct LeafSteps<RPT, kLeafMax> {
 
 template <int RPT>
 __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
-  constexpr int CH = kLeafThreads * RPT;  // rows per workgroup
+  constexpr int CH   = kLeafThreads * RPT;  // rows per workgroup
   const int     t    = threadIdx.x;
   const int     lane = t & 63;
   const int     wave = t >> 6;
@@ -315,9 +338,9 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
     for(int jj = 0; jj < a.w; ++jj) {
       const int p = sh.piv[jj];
       if(p < top_end) {
-        const int x         = sh.lab[jj];
-        sh.lab[jj]           = sh.lab[p - a.c0];
-        sh.lab[p - a.c0]     = x;
+        const int x      = sh.lab[jj];
+        sh.lab[jj]       = sh.lab[p - a.c0];
+        sh.lab[p - a.c0] = x;
       } else {
         int s = -1;
         for(int k = 0; k < nout; ++k)
@@ -326,13 +349,13 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
             break;
           }
         if(s < 0) {
-          s         = nout++;
+          s          = nout++;
           sh.ppos[s] = p;
           sh.plab[s] = p;
         }
         const int x = sh.lab[jj];
-        sh.lab[jj]   = sh.plab[s];
-        sh.plab[s]   = x;
+        sh.lab[jj]  = sh.plab[s];
+        sh.plab[s]  = x;
       }
     }
     int np = 0;
@@ -354,8 +377,7 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
   const int np     = sh.np;
   const int others = a.JB - a.w;
   if(np > 0) {
-    for(int idx = b * (kLeafThreads / 64) + wave; idx < others;
-        idx += G * (kLeafThreads / 64)) {
+    for(int idx = b * kWaves + wave; idx < others; idx += G * kWaves) {
       const int c   = idx < a.c0 ? idx : idx + a.w;
       double*   col = a.P + static_cast<size_t>(c) * a.lda;
       if(lane < np) {
