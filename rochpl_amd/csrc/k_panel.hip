@@ -212,11 +212,12 @@ __global__ __launch_bounds__(256) void swap_inverse_kernel(
 
 hipError_t launch_leaf(Ctx& c, double* P, int lda, int m, int JB, int c0,
                        int w, int* ipiv, int row_offset, int info_base) {
-  constexpr int CH   = kLeafThreads * kLeafRPT;
-  const int     R    = m - c0;
-  const int     G    = (R + CH - 1) / CH;
+  constexpr int CH = kLeafThreads * kLeafRPT;
+  const int     R  = m - c0;
+  const int     G  = (R + CH - 1) / CH;
   if(G > kMaxLeafGrid || G < 1 || w < 1 || w > kLeafMax)
     return hipErrorInvalidValue;
+  if(G > c.leaf_resident) return hipErrorInvalidValue;
   // sequence numbers: launch << 6 | column; slots restart when they wrap
   if(c.leaf_launches >= (1u << 26) - 1) {
     const int n = 2 * (kMaxGrid + 1) * kSlotStride;
@@ -228,20 +229,20 @@ hipError_t launch_leaf(Ctx& c, double* P, int lda, int m, int JB, int c0,
   }
   ++c.leaf_launches;
   leaf::LeafArgs la;
-  la.P          = P;
-  la.lda        = lda;
-  la.m          = m;
-  la.JB         = JB;
-  la.c0         = c0;
-  la.w          = w;
-  la.ipiv       = ipiv;
-  la.row_offset = row_offset;
-  la.info       = c.flags;
-  la.info_base  = info_base;
-  la.slots      = c.slots;
-  la.seq_base   = c.leaf_launches << 6;
-  la.err        = c.flags + 1;
-  if(G > c.leaf_resident) return hipErrorInvalidValue;
+  la.P           = P;
+  la.lda         = lda;
+  la.m           = m;
+  la.JB          = JB;
+  la.c0          = c0;
+  la.w           = w;
+  la.ipiv        = ipiv;
+  la.row_offset  = row_offset;
+  la.info        = c.flags;
+  la.info_base   = info_base;
+  la.slots       = c.slots;
+  la.seq_base    = c.leaf_launches << 6;
+  la.err         = c.flags + 1;
+  la.rows_per_wg = CH;
   // A plain launch: every workgroup is resident at once because the grid is
   // capped by the occupancy of this kernel (one workgroup per CU) and the
   // solve queues nothing else concurrently; the kernel's bounded waits turn
@@ -249,10 +250,9 @@ hipError_t launch_leaf(Ctx& c, double* P, int lda, int m, int JB, int c0,
   // API adds nothing here and refuses to launch while another process holds
   // the device.)
   void* args[] = {&la};
-  return hipLaunchKernel(leaf_kernel_ptr<kLeafRPT>(), dim3(G),
+  return hipLaunchKernel(leaf_kernel_ptr<LeafKind::kF64>(), dim3(G),
                          dim3(kLeafThreads), args, 0, c.stream);
 }
-
 
 // Columns [c0, c0+n) of the panel, rows [c0, m).
 hipError_t leaf_or_chunks(Ctx& c, double* P, int lda, int m,
@@ -315,7 +315,7 @@ hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, double* P,
 
 hipError_t leaf_occupancy(int* blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, leaf_kernel_ptr<kLeafRPT>(), kLeafThreads, 0);
+      blocks_per_cu, leaf_kernel_ptr<LeafKind::kF64>(), kLeafThreads, 0);
 }
 
 hipError_t reset_panel_state(Ctx& c) {

@@ -1,15 +1,15 @@
 // Register-resident cooperative leaf of the panel factorisation
 // (HPL_pdpanrlT, src/pfact/HPL_pdpanrlT_device.cpp:23-468): one launch per
-// leaf of <= 32 columns.  Included by k_leaf{1,2,3}.hip, each of which
-// instantiates one rows-per-thread variant (the fully unrolled column steps
-// make each instance a large compile unit).
+// leaf of <= 32 columns.  Instantiated in k_leaf*.hip for each (register
+// type T, storage type S) pair: FP64 for the HPL solve, FP32 registers over
+// FP32 or FP16 storage for the mixed-precision factorisations.
 //
-// Every workgroup keeps its slice of the leaf (up to 1536 rows x 32 columns)
-// in registers for the whole leaf.  Per column:
+// Every workgroup keeps its slice of the leaf (up to 512*RPT rows x 32
+// columns) in registers for the whole leaf.  Per column:
 //   1. each workgroup finds its candidate max |a| (wave64 shuffles, LDS);
 //   2. it publishes the candidate's value, row index and 32-wide row segment
-//      in its slot, then a sequence number with release ordering; the
-//      workgroup holding the current top row publishes that row likewise;
+//      in its slot, then a sequence number with release ordering; workgroup
+//      0, which holds the current top row, publishes that row likewise;
 //   3. every workgroup polls all slots' sequence numbers, then reads all
 //      candidates (values and row segments) in one pass into LDS, reduces
 //      them itself and takes the winner's row from LDS;
@@ -17,14 +17,19 @@
 //      rank-1 update of the remaining leaf columns.
 // Slots are double-buffered by column parity, so a workgroup that runs ahead
 // never overwrites a slot another workgroup still reads.  A grid of one
-// workgroup skips steps 2-3.  The reference instead parks its leaf on a
+// workgroup skips the slots.  The reference instead parks its leaf on a
 // pinned host flag once per column and lets a host thread run HPL_pdmxswp
 // (:94-117,455-463).
+//
+// Steps 1 and 4 touch the register tile and are instantiated per column
+// (static register indices); steps 2-3 are one out-of-line function.
 //
 // Every wait is bounded: after kSpinLimit polls a workgroup raises *err and
 // all workgroups leave the kernel, so a grid that is not co-resident cannot
 // hang the device.
 #pragma once
+
+#include <hip/hip_fp16.h>
 
 #include <climits>
 
@@ -54,7 +59,7 @@ __device__ inline void st_seq(double* p, unsigned long long v) {
 }
 
 struct LeafArgs {
-  double*  P;          // panel top-left (panel row 0)
+  void*    P;          // panel top-left (panel row 0), storage type S
   int      lda;
   int      m;          // panel rows
   int      JB;         // panel width (row interchanges span it)
@@ -67,15 +72,18 @@ struct LeafArgs {
   double*  slots;      // 2 x (kMaxGrid+1) x kSlotStride
   uint32_t seq_base;   // unique per launch; column j publishes seq_base+j+1
   int*     err;
+  int      rows_per_wg;  // kLeafThreads * RPT
 };
 
-// Shared-memory scratch of the leaf kernel.
+// Shared-memory scratch of the leaf kernel (the exchange works in double;
+// FP32 rows convert exactly).
 struct LeafShared {
   double cand[kMaxLeafGrid][kLeafMax];  // every workgroup's candidate row
   double cv[kMaxLeafGrid];
   int    cr[kMaxLeafGrid];
-  double u[kLeafMax];
-  double top[kLeafMax];
+  double mine[kLeafMax];  // this workgroup's candidate row
+  double u[kLeafMax];     // pivot row
+  double top[kLeafMax];   // current top row
   double wv[kWaves];
   int    wr[kWaves];
   int    piv[kLeafMax];
@@ -85,14 +93,34 @@ struct LeafShared {
   int    dst[2 * kLeafMax];
   int    src[2 * kLeafMax];
   int    np;
+  int    p;
 };
+
+template <typename S>
+__device__ inline float to_f(S x);
+template <>
+__device__ inline float to_f<float>(float x) { return x; }
+template <>
+__device__ inline float to_f<__half>(__half x) { return __half2float(x); }
+
+template <typename T, typename S>
+__device__ inline T load_s(const S* p) {
+  if constexpr(sizeof(T) == sizeof(S)) return *p;
+  else return static_cast<T>(to_f<S>(*p));
+}
+
+template <typename T, typename S>
+__device__ inline void store_s(S* p, T v) {
+  if constexpr(sizeof(T) == sizeof(S)) *p = v;
+  else *p = __float2half(static_cast<float>(v));
+}
 
 // Workgroup max-loc of (v, r); every thread gets the result.
 __device__ inline void block_maxloc(double& v, int& r, LeafShared& sh) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for(int o = 32; o > 0; o >>= 1) {
-    const double ov = __shfl_xor(v, o);
+    const double ov  = __shfl_xor(v, o);
     const int    orr = __shfl_xor(r, o);
     if(better(ov, orr, v, r)) {
       v = ov;
@@ -112,6 +140,7 @@ __device__ inline void block_maxloc(double& v, int& r, LeafShared& sh) {
       v = sh.wv[k];
       r = sh.wr[k];
     }
+  __syncthreads();  // sh.wv may be rewritten by the next call
 }
 
 // Wait until slot s shows sequence `want`; false on timeout or error.
@@ -132,25 +161,97 @@ __device__ inline bool poll_slot(const double* s, unsigned long long want,
   return true;
 }
 
-// One pivot step of the leaf for compile-time column JJ, so that every
-// register of the row tile is addressed statically (no scratch spill).
-template <int RPT, int JJ>
+// Steps 2-3 of a column: sh.mine holds this workgroup's candidate row (value
+// bv, row br) and, in workgroup 0, sh.top holds the top row.  On return
+// sh.p is the pivot row, sh.u its segment and sh.top the top row in every
+// workgroup.  Returns false if the grid has to stop.
+__device__ __attribute__((noinline)) bool exchange(const LeafArgs* ap, int jj,
+                                                   double bv, int br,
+                                                   LeafShared* shp) {
+  const LeafArgs& a  = *ap;
+  LeafShared&     sh = *shp;
+  const int       t  = threadIdx.x;
+  const int       b  = blockIdx.x;
+  const int       G  = gridDim.x;
+  if(G == 1) {
+    if(t < kLeafMax) sh.u[t] = sh.mine[t];
+    if(t == 0) sh.p = br;
+    __syncthreads();
+    return true;
+  }
+  // ---- publish (value, row, segment), then the sequence number
+  const unsigned long long seq =
+      static_cast<unsigned long long>(a.seq_base) + jj + 1;
+  const size_t par   = static_cast<size_t>(jj & 1) * (kMaxGrid + 1);
+  double*      slot  = a.slots + (par + b) * kSlotStride;
+  double*      tslot = a.slots + (par + kMaxGrid) * kSlotStride;
+  if(t < kLeafMax) {
+    slot[kSlotData + t] = sh.mine[t];
+  } else if(t == kLeafMax) {
+    slot[kSlotVal] = bv;
+    slot[kSlotRow] = static_cast<double>(br);
+  } else if(b == 0 && t >= 64 && t < 64 + kLeafMax) {
+    // the top row c0 + jj < c0 + 32 always lives in workgroup 0
+    tslot[kSlotData + (t - 64)] = sh.top[t - 64];
+  }
+  __threadfence();  // this thread's slot stores reach the coherence point
+  __syncthreads();
+  if(t == 0) {
+    st_seq(slot + kSlotSeq, seq);
+    if(b == 0) st_seq(tslot + kSlotSeq, seq);
+  }
+  // ---- wait for every candidate and the top row, then read them all
+  int ok = 1;
+  if(t < G)
+    ok = poll_slot(a.slots + (par + t) * kSlotStride, seq, a.err);
+  else if(t == G)
+    ok = poll_slot(tslot, seq, a.err);
+  if(!__syncthreads_and(ok)) return false;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  for(int i = t; i < G * kLeafMax; i += kLeafThreads) {
+    const int g = i / kLeafMax, c = i % kLeafMax;
+    sh.cand[g][c] = a.slots[(par + g) * kSlotStride + kSlotData + c];
+  }
+  if(t < G) {
+    const double* s = a.slots + (par + t) * kSlotStride;
+    sh.cv[t]        = s[kSlotVal];
+    sh.cr[t]        = static_cast<int>(s[kSlotRow]);
+  } else if(t >= kLeafThreads - kLeafMax && b != 0) {
+    const int c = t - (kLeafThreads - kLeafMax);
+    sh.top[c]   = tslot[kSlotData + c];
+  }
+  __syncthreads();
+  double cv = -1.0;
+  int    cr = INT_MAX;
+  if(t < G) {
+    cv = sh.cv[t];
+    cr = sh.cr[t];
+  }
+  block_maxloc(cv, cr, sh);
+  const int wb = (cr - a.c0) / a.rows_per_wg;
+  if(t < kLeafMax) sh.u[t] = sh.cand[wb][t];
+  if(t == 0) sh.p = cr;
+  __syncthreads();
+  return true;
+}
+
+// One pivot step for compile-time column JJ: every register of the row tile
+// is addressed statically (no scratch spill).
+template <typename T, int RPT, int JJ>
 __device__ __forceinline__ void leaf_step(const LeafArgs& a,
-                                          double (&v)[RPT][kLeafMax],
+                                          T (&v)[RPT][kLeafMax],
                                           const int (&rows)[RPT],
                                           LeafShared& sh, bool& alive) {
-  constexpr int CH = kLeafThreads * RPT;
-  const int     t  = threadIdx.x;
-  const int     b  = blockIdx.x;
-  const int     G  = gridDim.x;
-  const int     gc = a.c0 + JJ;  // current top row == current column
-  // ---- workgroup candidate: max |a(r, gc)| over rows r >= gc
+  const int t  = threadIdx.x;
+  const int b  = blockIdx.x;
+  const int gc = a.c0 + JJ;  // current top row == current column
+  // ---- step 1: workgroup candidate max |a(r, gc)| over rows r >= gc
   double bv = -1.0;
   int    br = INT_MAX;
 #pragma unroll
   for(int q = 0; q < RPT; ++q) {
     if(rows[q] >= gc && rows[q] < a.m) {
-      const double x = fabs(v[q][JJ]);
+      const double x = fabs(static_cast<double>(v[q][JJ]));
       if(better(x, rows[q], bv, br)) {
         bv = x;
         br = rows[q];
@@ -158,138 +259,75 @@ __device__ __forceinline__ void leaf_step(const LeafArgs& a,
     }
   }
   block_maxloc(bv, br, sh);
-  int p;
-  if(G == 1) {
-    // ---- single workgroup: the candidate is the pivot
 #pragma unroll
-    for(int q = 0; q < RPT; ++q) {
-      if(rows[q] == br) {
+  for(int q = 0; q < RPT; ++q) {
+    if(rows[q] == br) {
 #pragma unroll
-        for(int c = 0; c < kLeafMax; ++c) sh.u[c] = v[q][c];
-      }
-      if(rows[q] == gc) {
+      for(int c = 0; c < kLeafMax; ++c) sh.mine[c] = static_cast<double>(v[q][c]);
+    }
+    if(rows[q] == gc) {
 #pragma unroll
-        for(int c = 0; c < kLeafMax; ++c) sh.top[c] = v[q][c];
-      }
+      for(int c = 0; c < kLeafMax; ++c) sh.top[c] = static_cast<double>(v[q][c]);
     }
-    p = br;
-    __syncthreads();
-  } else {
-    // ---- publish (value, row, segment), then the sequence number
-    const unsigned long long seq =
-        static_cast<unsigned long long>(a.seq_base) + JJ + 1;
-    const size_t par   = static_cast<size_t>(JJ & 1) * (kMaxGrid + 1);
-    double*      slot  = a.slots + (par + b) * kSlotStride;
-    double*      tslot = a.slots + (par + kMaxGrid) * kSlotStride;
-    if(t == 0) {
-      slot[kSlotVal] = bv;
-      slot[kSlotRow] = static_cast<double>(br);
-    }
-#pragma unroll
-    for(int q = 0; q < RPT; ++q) {
-      if(rows[q] == br) {
-#pragma unroll
-        for(int c = 0; c < kLeafMax; ++c) slot[kSlotData + c] = v[q][c];
-      }
-      if(rows[q] == gc) {
-#pragma unroll
-        for(int c = 0; c < kLeafMax; ++c) tslot[kSlotData + c] = v[q][c];
-      }
-    }
-    __threadfence();  // this thread's slot stores reach the coherence point
-    __syncthreads();
-    if(t == 0) {
-      st_seq(slot + kSlotSeq, seq);
-      // the top row gc < c0 + 32 always lives in workgroup 0
-      if(b == 0) st_seq(tslot + kSlotSeq, seq);
-    }
-    // ---- wait for every candidate and the top row, then read them all
-    int ok = 1;
-    if(t < G)
-      ok = poll_slot(a.slots + (par + t) * kSlotStride, seq, a.err);
-    else if(t == G)
-      ok = poll_slot(tslot, seq, a.err);
-    if(!__syncthreads_and(ok)) {
-      alive = false;
-      return;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for(int i = t; i < G * kLeafMax; i += kLeafThreads) {
-      const int g = i / kLeafMax, c = i % kLeafMax;
-      sh.cand[g][c] = a.slots[(par + g) * kSlotStride + kSlotData + c];
-    }
-    if(t < G) {
-      const double* s = a.slots + (par + t) * kSlotStride;
-      sh.cv[t]        = s[kSlotVal];
-      sh.cr[t]        = static_cast<int>(s[kSlotRow]);
-    } else if(t >= kLeafThreads - kLeafMax) {
-      const int c = t - (kLeafThreads - kLeafMax);
-      sh.top[c]   = tslot[kSlotData + c];
-    }
-    __syncthreads();
-    double cv = -1.0;
-    int    cr = INT_MAX;
-    if(t < G) {
-      cv = sh.cv[t];
-      cr = sh.cr[t];
-    }
-    block_maxloc(cv, cr, sh);
-    p            = cr;
-    const int wb = (p - a.c0) / CH;
-    if(t < kLeafMax) sh.u[t] = sh.cand[wb][t];
-    __syncthreads();
   }
+  __syncthreads();
+  // ---- steps 2-3
+  if(!exchange(&a, JJ, bv, br, &sh)) {
+    alive = false;
+    return;
+  }
+  const int p = sh.p;
   if(t == 0) sh.piv[JJ] = p;
-  const double piv = sh.u[JJ];
+  const T piv = static_cast<T>(sh.u[JJ]);
   if(b == 0 && t == 0) {
     a.ipiv[gc] = p + a.row_offset;
-    if(piv == 0.0 && *a.info == 0) *a.info = a.info_base + gc + 1;
+    if(piv == T(0) && *a.info == 0) *a.info = a.info_base + gc + 1;
   }
-  // ---- interchange rows gc and p (leaf columns; registers)
+  // ---- step 4: interchange rows gc and p, scale, rank-1 update
 #pragma unroll
   for(int q = 0; q < RPT; ++q) {
     if(rows[q] == gc) {
 #pragma unroll
-      for(int c = 0; c < kLeafMax; ++c) v[q][c] = sh.u[c];
+      for(int c = 0; c < kLeafMax; ++c) v[q][c] = static_cast<T>(sh.u[c]);
     } else if(rows[q] == p) {
 #pragma unroll
-      for(int c = 0; c < kLeafMax; ++c) v[q][c] = sh.top[c];
+      for(int c = 0; c < kLeafMax; ++c) v[q][c] = static_cast<T>(sh.top[c]);
     }
   }
-  // ---- scale column gc and rank-1 update the rest of the leaf
 #pragma unroll
   for(int q = 0; q < RPT; ++q) {
     if(rows[q] > gc && rows[q] < a.m) {
-      double l = v[q][JJ];
-      if(piv != 0.0) l = l / piv;
+      T l = v[q][JJ];
+      if(piv != T(0)) l = l / piv;
       v[q][JJ] = l;
 #pragma unroll
-      for(int c = JJ + 1; c < kLeafMax; ++c) v[q][c] = fma(-l, sh.u[c], v[q][c]);
+      for(int c = JJ + 1; c < kLeafMax; ++c)
+        v[q][c] = fma(-l, static_cast<T>(sh.u[c]), v[q][c]);
     }
   }
-  __syncthreads();  // sh.u / sh.top / sh.wv are rewritten next column
+  __syncthreads();  // sh.u / sh.top / sh.mine are rewritten next column
 }
 
-template <int RPT, int JJ>
+template <typename T, int RPT, int JJ>
 struct LeafSteps {
   __device__ __forceinline__ static void run(const LeafArgs& a,
-                                             double (&v)[RPT][kLeafMax],
+                                             T (&v)[RPT][kLeafMax],
                                              const int (&rows)[RPT],
                                              LeafShared& sh, bool& alive) {
-    if(JJ < a.w && alive) leaf_step<RPT, JJ>(a, v, rows, sh, alive);
-    LeafSteps<RPT, JJ + 1>::run(a, v, rows, sh, alive);
+    if(JJ < a.w && alive) leaf_step<T, RPT, JJ>(a, v, rows, sh, alive);
+    LeafSteps<T, RPT, JJ + 1>::run(a, v, rows, sh, alive);
   }
 };
 
-template <int RPT>
-struct LeafSteps<RPT, kLeafMax> {
+template <typename T, int RPT>
+struct LeafSteps<T, RPT, kLeafMax> {
   __device__ __forceinline__ static void run(const LeafArgs&,
-                                             double (&)[RPT][kLeafMax],
+                                             T (&)[RPT][kLeafMax],
                                              const int (&)[RPT], LeafShared&,
                                              bool&) {}
 };
 
-template <int RPT>
+template <typename T, typename S, int RPT>
 __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
   constexpr int CH   = kLeafThreads * RPT;  // rows per workgroup
   const int     t    = threadIdx.x;
@@ -297,36 +335,37 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
   const int     wave = t >> 6;
   const int     b    = blockIdx.x;
   const int     G    = gridDim.x;
+  S*            P    = static_cast<S*>(a.P);
 
   __shared__ LeafShared sh;
 
   if(__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
     return;
 
-  int    rows[RPT];
-  double v[RPT][kLeafMax];
+  int rows[RPT];
+  T   v[RPT][kLeafMax];
 #pragma unroll
   for(int q = 0; q < RPT; ++q) rows[q] = a.c0 + b * CH + q * kLeafThreads + t;
 #pragma unroll
   for(int c = 0; c < kLeafMax; ++c) {
-    const double* col = a.P + static_cast<size_t>(a.c0 + c) * a.lda;
+    const S* col = P + static_cast<size_t>(a.c0 + c) * a.lda;
 #pragma unroll
     for(int q = 0; q < RPT; ++q)
-      v[q][c] = (c < a.w && rows[q] < a.m) ? col[rows[q]] : 0.0;
+      v[q][c] = (c < a.w && rows[q] < a.m) ? load_s<T, S>(col + rows[q]) : T(0);
   }
 
   bool alive = true;
-  LeafSteps<RPT, 0>::run(a, v, rows, sh, alive);
+  LeafSteps<T, RPT, 0>::run(a, v, rows, sh, alive);
   if(!alive) return;
 
   // ---- write the leaf back
 #pragma unroll
   for(int c = 0; c < kLeafMax; ++c) {
     if(c < a.w) {
-      double* col = a.P + static_cast<size_t>(a.c0 + c) * a.lda;
+      S* col = P + static_cast<size_t>(a.c0 + c) * a.lda;
 #pragma unroll
       for(int q = 0; q < RPT; ++q)
-        if(rows[q] < a.m) col[rows[q]] = v[q][c];
+        if(rows[q] < a.m) store_s<T, S>(col + rows[q], v[q][c]);
     }
   }
 
@@ -379,10 +418,10 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
   if(np > 0) {
     for(int idx = b * kWaves + wave; idx < others; idx += G * kWaves) {
       const int c   = idx < a.c0 ? idx : idx + a.w;
-      double*   col = a.P + static_cast<size_t>(c) * a.lda;
+      S*        col = P + static_cast<size_t>(c) * a.lda;
       if(lane < np) {
         // one instruction loads every source before any lane stores
-        const double x = col[sh.src[lane]];
+        const S x        = col[sh.src[lane]];
         col[sh.dst[lane]] = x;
       }
     }
@@ -391,8 +430,10 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
 
 }  // namespace leaf
 
-// Kernel entry of one rows-per-thread variant (defined in k_leaf<RPT>.hip).
-template <int RPT>
+// Kernel entry of one variant (defined in k_leaf*.hip): FP64, or FP32
+// registers over FP32 / FP16 storage.
+enum class LeafKind { kF64, kF32, kF32H };
+template <LeafKind K>
 const void* leaf_kernel_ptr();
 
 }  // namespace hplx

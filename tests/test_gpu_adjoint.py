@@ -7,6 +7,10 @@ checked against (1) LAPACK (scipy) FP64 solves on the CPU and (2) central
 finite differences of loss = sum(G * X), h = 1e-6, on 32 random entries
 (the config: N = 16384, A = U(-0.5,0.5) + 2 sqrt(N) I from the HPL generator,
 B and G standard normal from numpy's PCG64 with the seeds below)."""
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 import scipy.linalg as sl
@@ -129,7 +133,7 @@ def test_adjoint_config5_vs_cpu_and_finite_differences(ctx):
         def loss(Xv):
             return float(np.sum(G * Xv))
 
-        fd_rel = []
+        fd_rel, fd_abs, gsel = [], [], []
         for e in range(32):
             i, j = int(rng.integers(n)), int(rng.integers(n))
             s.factor((i, j, +h))
@@ -138,6 +142,8 @@ def test_adjoint_config5_vs_cpu_and_finite_differences(ctx):
             lm = loss(s.solve(B))
             fd = (lp - lm) / (2 * h)
             fd_rel.append(abs(fd - gA[i, j]) / abs(gA[i, j]))
+            fd_abs.append(fd - gA[i, j])
+            gsel.append(gA[i, j])
         s.factor()
         for e in range(32):
             i, k = int(rng.integers(n)), int(rng.integers(nrhs))
@@ -148,74 +154,44 @@ def test_adjoint_config5_vs_cpu_and_finite_differences(ctx):
             lm = loss(s.solve(Bp))
             fd = (lp - lm) / (2 * h)
             fd_rel.append(abs(fd - gB[i, k]) / abs(gB[i, k]))
+            fd_abs.append(fd - gB[i, k])
+            gsel.append(gB[i, k])
     finally:
         s.free()
-    fd_rel = np.array(fd_rel)
-    print("[adjoint] FD relative error: max %.3e median %.3e (A entries max "
-          "%.3e, B entries max %.3e)" % (fd_rel.max(), np.median(fd_rel),
-                                         fd_rel[:32].max(), fd_rel[32:].max()),
-          flush=True)
-    assert np.median(fd_rel) <= 1e-7
-    assert fd_rel.max() <= 1e-5
+    fd_rel, fd_abs, gsel = map(np.array, (fd_rel, fd_abs, gsel))
+    # Central differences cannot resolve the gradient better than the
+    # rounding of the two perturbed losses allows.  A perturbed A is factored
+    # afresh, so each loss carries the factorisation's rounding,
+    # sum_ij grad_B[i] dA_ij X[j] with |dA_ij| ~ eps sqrt(n) (dot products of
+    # length up to n), i.e. |fd - g| <~ eps sqrt(n) ||grad_B||_F ||X||_F / h.
+    # At this size that floor is ~5e-7 against gradient entries of ~1e-4 (A)
+    # and ~4e-3 (B): FD pins each entry to ~5e-3 relative, far from the 1e-9
+    # that the FP64 reference gradient above is held to.  A wrong sign, index
+    # or transposition moves an entry by about its own size, ~100x the floor.
+    floor = np.finfo(np.float64).eps * n ** 0.5 * np.linalg.norm(gB) * \
+        np.linalg.norm(X) / h
+    rms_g = np.sqrt(np.mean(gsel ** 2))
+    normwise_A = np.linalg.norm(fd_abs[:32]) / np.linalg.norm(gsel[:32])
+    normwise_B = np.linalg.norm(fd_abs[32:]) / np.linalg.norm(gsel[32:])
+    print("[adjoint] FD: floor %.2e (rms grad %.2e); |fd-g| max %.2e; normwise"
+          " rel A %.3e B %.3e; per-entry rel max %.3e median %.3e" %
+          (floor, rms_g, np.abs(fd_abs).max(), normwise_A, normwise_B,
+           fd_rel.max(), np.median(fd_rel)), flush=True)
+    assert floor <= 1e-2 * np.sqrt(np.mean(gsel[:32] ** 2))
+    assert np.all(np.abs(fd_abs) <= floor)
 
 
-def test_autograd_function_wrapper(ctx):
-    """A test-only torch.autograd.Function around the C ABI: forward factors
-    and solves, backward calls hplx_pdgesv_adjoint; compared with torch's own
-    FP64 linear solve gradient."""
-    torch = pytest.importorskip("torch")
-    n, nb, nrhs = 2048, 256, 16
-    say("autograd wrapper n=%d" % n)
-    dev = torch.device("cuda:0")
-
-    class LUSolve(torch.autograd.Function):
-        @staticmethod
-        def forward(fctx, A, B):
-            mat = ctx.pmat_alloc(n, nb)
-            Ah = A.detach().double().cpu().numpy()
-            buf = np.zeros((mat.ld, n + 1), order="F")
-            buf[:n, :n] = Ah
-            rp.lib().hplx_memcpy_h2d(ctx.h, rp.ctypes.c_void_p(mat.A),
-                                     buf.ctypes.data_as(rp.ctypes.c_void_p),
-                                     buf.nbytes)
-            assert ctx.pdgetrf(rp.palg_default(), mat) == 0
-            Xd = B.detach().double().t().contiguous().t().clone()  # column-major
-            torch.cuda.synchronize()
-            ctx.pdgetrs(mat, 0, nrhs, Xd.data_ptr(), n)
-            fctx.mat = mat
-            fctx.save_for_backward(Xd)
-            return Xd.clone()
-
-        @staticmethod
-        def backward(fctx, Gout):
-            (Xd,) = fctx.saved_tensors
-            Gc = Gout.double().t().contiguous().t()
-            gB = torch.empty_like(Gc).t().contiguous().t()
-            gA = torch.empty((n, n), dtype=torch.float64,
-                             device=dev).t().contiguous().t()
-            torch.cuda.synchronize()
-            ctx.pdgesv_adjoint(fctx.mat, nrhs, Xd.data_ptr(), n, Gc.data_ptr(),
-                               n, gB.data_ptr(), n, gA.data_ptr(), n)
-            ctx.pmat_free(fctx.mat)
-            return gA, gB
-
-    An = o.matgen(n, n, diag_shift=2 * n ** 0.5)
-    A1 = torch.tensor(An, device=dev, requires_grad=True)
-    g = torch.Generator(device="cpu").manual_seed(7)
-    Bt = torch.randn(n, nrhs, generator=g, dtype=torch.float64).to(dev)
-    Gt = torch.randn(n, nrhs, generator=g, dtype=torch.float64).to(dev)
-    B1 = Bt.clone().requires_grad_(True)
-    X1 = LUSolve.apply(A1, B1)
-    (X1 * Gt).sum().backward()
-    A2 = torch.tensor(An, device=dev, requires_grad=True)
-    B2 = Bt.clone().requires_grad_(True)
-    X2 = torch.linalg.solve(A2, B2)
-    (X2 * Gt).sum().backward()
-    torch.cuda.synchronize()
-
-    def rel(a, b):
-        return (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()
-
-    assert rel(X1, X2) < 1e-12
-    assert rel(B1.grad, B2.grad) < 1e-12
-    assert rel(A1.grad, A2.grad) < 1e-12
+def test_autograd_function_wrapper():
+    """The test-only torch.autograd.Function (tests/autograd_case.py) against
+    torch.linalg.solve's FP64 gradient.  It runs in a child process that
+    imports torch before loading libhplx.so, so that both use torch's HIP
+    runtime (this process has already loaded the system one)."""
+    pytest.importorskip("torch")
+    say("autograd wrapper n=2048 (child process)")
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([sys.executable, "-u",
+                        os.path.join(here, "autograd_case.py")],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-4000:], flush=True)
+    assert r.returncode == 0
+    assert "AUTOGRAD-OK" in r.stdout
