@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-nb", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prewarm", type=int, default=None,
+                    help="N of an untimed warm-up solve (default: min(N,4096) "
+                         "for N <= 32768, none above)")
     ap.add_argument("--check-pivots", type=int, default=0,
                     help="compare the first K pivots with LAPACK dgetrf on the "
                          "N x K column prefix (prefix oracle)")
@@ -99,12 +102,18 @@ def main():
     palg = rp.palg_default()
     palg.frac = args.frac
 
-    # kernel pre-warm on a small matrix (cf. HPL_pdwarmup); not a step
-    wn = min(n, 4096)
-    wmat = ctx.pmat_alloc(wn, min(nb, wn))
-    ctx.pdrandmat(wn, wn + 1, wmat.nb, wmat.A, wmat.ld, rp.HPL_ISEED, 0.0)
-    ctx.pdgesv(palg, wmat)
-    ctx.pmat_free(wmat)
+    # kernel pre-warm on a small matrix (cf. HPL_pdwarmup); not a step.  Only
+    # for small N: at large N the one-off code-object load is lost in a
+    # minutes-long solve, and without it every trailing-update launch in a
+    # kernel trace of this command belongs to the timed solve.
+    if args.prewarm is None:
+        args.prewarm = min(n, 4096) if n <= 32768 else 0
+    if args.prewarm > 0:
+        wn = args.prewarm
+        wmat = ctx.pmat_alloc(wn, min(nb, wn))
+        ctx.pdrandmat(wn, wn + 1, wmat.nb, wmat.A, wmat.ld, rp.HPL_ISEED, 0.0)
+        ctx.pdgesv(palg, wmat)
+        ctx.pmat_free(wmat)
 
     log("allocating N=%d NB=%d (%.1f GiB)" % (n, nb, rp.hpl_ld(n) * (n + 1)
                                                * 8 / 2 ** 30))
@@ -171,7 +180,9 @@ def main():
                      "traffic": traffic,
                      "kernel": "trailing-update FP64 MFMA GEMM "
                                "(A22 -= L21*U12), %d launches, %.1f ms"
-                               % (gemm_launch, gemm_ms)},
+                               % (gemm_launch, gemm_ms),
+                     "avg_launch_ms": round(gemm_ms / max(gemm_launch, 1), 4),
+                     "flops_per_launch": gemm_flops / max(gemm_launch, 1)},
         "residual": round(max(c["resid"] for c in checks), 6),
         "residual_r0_form": round(max(c["resid_r0_form"] for c in checks), 6),
         "tflops_ref_formula": round(ref_flops(n) / t_mean / 1e12, 3),
