@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--n", type=int, default=193280)
     ap.add_argument("--nb", type=int, default=896)
     ap.add_argument("--frac", type=float, default=0.6)
-    ap.add_argument("--cpu-n", type=int, default=16384)
+    ap.add_argument("--cpu-n", type=int, default=24576)
     ap.add_argument("--cpu-nb", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prewarm", type=int, default=None,

@@ -1,6 +1,7 @@
 // Internal declarations shared by the host ABI layer and the HIP kernels.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -80,24 +81,38 @@ hipError_t reset_panel_state(Ctx& c);
 // rows + row_offset; zero pivots are reported as row_offset + column + 1.
 hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, double* P, int lda,
                         int m, int jb, int* ipiv, int row_offset);
+// The same recursion in FP32 (FP32 leaves, solves and MFMA GEMMs).
+hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, float* P, int lda,
+                        int m, int jb, int* ipiv, int row_offset);
 
 hipError_t launch_pipid(hipStream_t s, int jb, const int* ipiv, int ipiv_base,
                         int* permU);
 hipError_t launch_swap_gather(hipStream_t s, int jb, int n, double* A,
                               int lda, const int* permU, double* W, int ldw);
+hipError_t launch_swap_gather(hipStream_t s, int jb, int n, float* A, int lda,
+                              const int* permU, float* W, int ldw);
+hipError_t launch_swap_gather(hipStream_t s, int jb, int n, __half* A,
+                              int lda, const int* permU, float* W, int ldw);
 hipError_t launch_swap_inplace(hipStream_t s, int jb, int n, double* A,
                                int lda, const int* permU);
 hipError_t launch_swap_inverse(hipStream_t s, int jb, int n, double* A,
                                int lda, const int* permU);
 hipError_t launch_linv(hipStream_t s, int jb, const double* L, int ldl,
                        double* Linv, int ldi);
+hipError_t launch_linv(hipStream_t s, int jb, const float* L, int ldl,
+                       float* Linv, int ldi);
 // op(T) x = b for nrhs columns of X; op(T) lower or upper as `lower` says
 // (T itself is upper when lower && trans, lower when !lower && trans).
 hipError_t launch_tri_solve(hipStream_t s, bool lower, bool trans, bool unit,
                             int k, int nrhs, const double* T, int ldt,
                             double* X, int ldx);
+hipError_t launch_tri_solve(hipStream_t s, bool lower, bool trans, bool unit,
+                            int k, int nrhs, const float* T, int ldt, float* X,
+                            int ldx);
 hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const double* L,
                                 int ldl, double* B, int ldb);
+hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const float* L,
+                                int ldl, float* B, int ldb);
 
 // C = beta*C + alpha*op(A)*op(B); alpha in {+1,-1}, beta in {0,1}.
 // lower_a: A is lower triangular (only k < row-tile end contributes).  kind 1
@@ -108,6 +123,10 @@ hipError_t launch_dgemm(hipStream_t s, int m, int n, int k, double alpha,
                         double beta, double* C, int ldc, bool lower_a,
                         int kind = 0, bool trans_a = false,
                         bool trans_b = false);
+// The FP32 MFMA instance (no transposes, no trailing-update kind).
+hipError_t launch_sgemm(hipStream_t s, int m, int n, int k, double alpha,
+                        const float* A, int lda, const float* B, int ldb,
+                        double beta, float* C, int ldc, bool lower_a);
 
 hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                        double* x);

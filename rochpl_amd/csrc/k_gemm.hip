@@ -47,14 +47,31 @@ constexpr int LDS_LD  = BM + 2;  // even (16-byte rows), staggers LDS banks
 constexpr int kGroupM = 8;
 constexpr int kXcds   = 8;
 
-typedef double double4_t __attribute__((ext_vector_type(4)));
-typedef double double2_t __attribute__((ext_vector_type(2)));
+template <typename V>
+using vec4_t = V __attribute__((ext_vector_type(4)));
+template <typename V>
+using vec2_t = V __attribute__((ext_vector_type(2)));
 
-// V_MFMA_F64_16X16X4_F64 operand layout (lane l of 64):
+// V_MFMA_F64_16X16X4_F64 / V_MFMA_F32_16X16X4_F32 operand layout (lane l):
 //   a-operand: X[l % 16][l / 16]   (16 x 4)
 //   b-operand: Y[l / 16][l % 16]   (4 x 16)
-//   result:    D[4*r + l / 16][l % 16], r = 0..3
-__device__ inline int dl_row(int r, int lane) { return 4 * r + (lane >> 4); }
+//   result:    D[4*r + l / 16][l % 16], r = 0..3         (f64)
+//              D[4*(l / 16) + r][l % 16], r = 0..3       (f32)
+template <typename V>
+__device__ inline int dl_row(int r, int lane) {
+  if constexpr(sizeof(V) == 8)
+    return 4 * r + (lane >> 4);
+  else
+    return 4 * (lane >> 4) + r;
+}
+
+template <typename V>
+__device__ inline vec4_t<V> mfma16x16x4(V a, V b, vec4_t<V> c) {
+  if constexpr(sizeof(V) == 8)
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
 // Stages one 128 x 16 tile of an operand (outer index o = row of A or
 // column of B, inner index k) from global memory into LDS as S[k][o].
@@ -62,13 +79,13 @@ __device__ inline int dl_row(int r, int lane) { return 4 * r + (lane >> 4); }
 //   KFAST = true:  k is the contiguous index (B, or A^T when A is transposed)
 // Loads are unconditional: out-of-range elements are read from a clamped
 // address and replaced by zero at the LDS store.
-template <bool KFAST, bool VEC2>
+template <typename V, bool KFAST, bool VEC2>
 struct TileLoader {
   static constexpr int NQ = VEC2 ? 4 : 8;
-  double r[8];
-  bool   v[NQ];
+  V    r[8];
+  bool v[NQ];
 
-  __device__ void load(const double* __restrict__ P, int ld, int o0, int O,
+  __device__ void load(const V* __restrict__ P, int ld, int o0, int O,
                        int k0, int kend) {
     const int t = threadIdx.x;
     if(!KFAST && VEC2) {  // pairs of o at k = (t>>6) + 4q
@@ -79,7 +96,7 @@ struct TileLoader {
         const int k  = k0 + (t >> 6) + 4 * q;
         v[q]         = op < O && k < kend;
         const int kc = min(k, kend - 1);
-        const double2_t x = *reinterpret_cast<const double2_t*>(
+        const vec2_t<V> x = *reinterpret_cast<const vec2_t<V>*>(
             P + oc + static_cast<size_t>(kc) * ld);
         r[2 * q]     = x.x;
         r[2 * q + 1] = x.y;
@@ -102,7 +119,7 @@ struct TileLoader {
         const int o  = o0 + (t >> 3) + 32 * q;
         v[q]         = k < kend && o < O;
         const int oc = min(o, O - 1);
-        const double2_t x = *reinterpret_cast<const double2_t*>(
+        const vec2_t<V> x = *reinterpret_cast<const vec2_t<V>*>(
             P + kc + static_cast<size_t>(oc) * ld);
         r[2 * q]     = x.x;
         r[2 * q + 1] = x.y;
@@ -120,37 +137,37 @@ struct TileLoader {
     }
   }
 
-  __device__ void store(double* S, bool neg) const {
+  __device__ void store(V* S, bool neg) const {
     const int t = threadIdx.x;
     if(!KFAST && VEC2) {
 #pragma unroll
       for(int q = 0; q < 4; ++q) {
         const int k = (t >> 6) + 4 * q;
-        double2_t x;
-        x.x = v[q] ? (neg ? -r[2 * q] : r[2 * q]) : 0.0;
-        x.y = v[q] ? (neg ? -r[2 * q + 1] : r[2 * q + 1]) : 0.0;
-        *reinterpret_cast<double2_t*>(S + k * LDS_LD + 2 * (t & 63)) = x;
+        vec2_t<V> x;
+        x.x = v[q] ? (neg ? -r[2 * q] : r[2 * q]) : V(0);
+        x.y = v[q] ? (neg ? -r[2 * q + 1] : r[2 * q + 1]) : V(0);
+        *reinterpret_cast<vec2_t<V>*>(S + k * LDS_LD + 2 * (t & 63)) = x;
       }
     } else if(!KFAST) {
 #pragma unroll
       for(int q = 0; q < 8; ++q) {
         const int k = (t >> 7) + 2 * q;
-        S[k * LDS_LD + (t & 127)] = v[q] ? (neg ? -r[q] : r[q]) : 0.0;
+        S[k * LDS_LD + (t & 127)] = v[q] ? (neg ? -r[q] : r[q]) : V(0);
       }
     } else if(VEC2) {
 #pragma unroll
       for(int q = 0; q < 4; ++q) {
         const int o = (t >> 3) + 32 * q;
         const int k = 2 * (t & 7);
-        S[k * LDS_LD + o]       = v[q] ? (neg ? -r[2 * q] : r[2 * q]) : 0.0;
+        S[k * LDS_LD + o]       = v[q] ? (neg ? -r[2 * q] : r[2 * q]) : V(0);
         S[(k + 1) * LDS_LD + o] = v[q] ? (neg ? -r[2 * q + 1] : r[2 * q + 1])
-                                       : 0.0;
+                                       : V(0);
       }
     } else {
 #pragma unroll
       for(int q = 0; q < 8; ++q) {
         const int o = (t >> 4) + 16 * q;
-        S[(t & 15) * LDS_LD + o] = v[q] ? (neg ? -r[q] : r[q]) : 0.0;
+        S[(t & 15) * LDS_LD + o] = v[q] ? (neg ? -r[q] : r[q]) : V(0);
       }
     }
   }
@@ -159,14 +176,16 @@ struct TileLoader {
 // KIND only names the instance: 1 = the trailing update A22 -= L21*U12, so
 // profiles report it apart from the panel-internal products.
 // TA / TB: the operand is stored transposed (op(A) = A^T, op(B) = B^T).
-template <bool BETA0, bool NEG, bool LOWER_A, bool TA, bool TB, bool VA,
-          bool VB, int KIND>
+// V: double (FP64 MFMA) or float (FP32 MFMA, the FP32 panel of the
+// mixed-precision factorisation).
+template <typename V, bool BETA0, bool NEG, bool LOWER_A, bool TA, bool TB,
+          bool VA, bool VB, int KIND>
 __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
-    int M, int N, int K, const double* __restrict__ A, int lda,
-    const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc,
+    int M, int N, int K, const V* __restrict__ A, int lda,
+    const V* __restrict__ B, int ldb, V* __restrict__ C, int ldc,
     int tiles_m, int tiles_n) {
-  __shared__ __attribute__((aligned(16))) double sA[2][BK * LDS_LD];
-  __shared__ __attribute__((aligned(16))) double sB[2][BK * LDS_LD];
+  __shared__ __attribute__((aligned(16))) V sA[2][BK * LDS_LD];
+  __shared__ __attribute__((aligned(16))) V sB[2][BK * LDS_LD];
 
   // ---- XCD-aware tile mapping
   const int T    = tiles_m * tiles_n;
@@ -193,8 +212,8 @@ __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
   const int l16  = lane & 15;
   const int lk   = lane >> 4;
 
-  TileLoader<TA, VA>  la;   // A: o = row of C; k-contiguous when TA
-  TileLoader<!TB, VB> lb;   // B: o = column of C; k-contiguous unless TB
+  TileLoader<V, TA, VA>  la;   // A: o = row of C; k-contiguous when TA
+  TileLoader<V, !TB, VB> lb;   // B: o = column of C; k-contiguous unless TB
   auto load_stage = [&](int k0) {
     la.load(A, lda, m0, M, k0, kend);
     lb.load(B, ldb, n0, N, k0, kend);
@@ -208,11 +227,11 @@ __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
 
   // ---- accumulators start at zero; C is added in the epilogue, so no load
   // of C is outstanding while the main loop runs
-  double4_t acc[TM][TN];
+  vec4_t<V> acc[TM][TN];
 #pragma unroll
   for(int i = 0; i < TM; ++i)
 #pragma unroll
-    for(int j = 0; j < TN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+    for(int j = 0; j < TN; ++j) acc[i][j] = vec4_t<V>{0, 0, 0, 0};
 
   if(kend > 0) store_stage(0);
   __syncthreads();
@@ -221,9 +240,9 @@ __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
   for(int k0 = 0; k0 < kend; k0 += BK) {
     const bool more = (k0 + BK) < kend;
     if(more) load_stage(k0 + BK);
-    const double* a_s = sA[buf] + lk * LDS_LD + wm0 + l16;
-    const double* b_s = sB[buf] + lk * LDS_LD + wn0 + l16;
-    double        fa[2][TM], fb[2][TN];
+    const V* a_s = sA[buf] + lk * LDS_LD + wm0 + l16;
+    const V* b_s = sB[buf] + lk * LDS_LD + wn0 + l16;
+    V        fa[2][TM], fb[2][TN];
 #pragma unroll
     for(int i = 0; i < TM; ++i) fa[0][i] = a_s[i * 16];
 #pragma unroll
@@ -243,8 +262,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
 #pragma unroll
         for(int j = 0; j < TN; ++j)
           // D = B^T-block x A^T-block: D[n][m] accumulates C[m][n]
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-              fb[cur][j], fa[cur][i], acc[i][j], 0, 0, 0);
+          acc[i][j] = mfma16x16x4<V>(fb[cur][j], fa[cur][i], acc[i][j]);
     }
     if(more) store_stage(buf ^ 1);
     __syncthreads();
@@ -260,22 +278,25 @@ __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
     for(int j = 0; j < TN; ++j)
 #pragma unroll
       for(int r = 0; r < 4; ++r) {
-        const int col = n0 + wn0 + j * 16 + dl_row(r, lane);
+        const int col = n0 + wn0 + j * 16 + dl_row<V>(r, lane);
         if(col < N) {
-          double* c = C + row + static_cast<size_t>(col) * ldc;
+          V* c = C + row + static_cast<size_t>(col) * ldc;
           *c        = BETA0 ? acc[i][j][r] : *c + acc[i][j][r];
         }
       }
   }
 }
 
-bool aligned16(const double* p) {
-  return reinterpret_cast<uintptr_t>(p) % 16 == 0;
+// a pair of elements (the VEC2 load) is naturally aligned
+template <typename V>
+bool pair_aligned(const V* p) {
+  return reinterpret_cast<uintptr_t>(p) % (2 * sizeof(V)) == 0;
 }
 
-template <bool BETA0, bool NEG, bool LOWER_A, bool TA, bool TB, int KIND>
-hipError_t run(hipStream_t s, int m, int n, int k, const double* A, int lda,
-               const double* B, int ldb, double* C, int ldc) {
+template <bool BETA0, bool NEG, bool LOWER_A, bool TA, bool TB, int KIND,
+          typename V>
+hipError_t run(hipStream_t s, int m, int n, int k, const V* A, int lda,
+               const V* B, int ldb, V* C, int ldc) {
   const int  tiles_m = (m + BM - 1) / BM;
   const int  tiles_n = (n + BN - 1) / BN;
   const long T       = static_cast<long>(tiles_m) * tiles_n;
@@ -284,30 +305,28 @@ hipError_t run(hipStream_t s, int m, int n, int k, const double* A, int lda,
   // 16-byte staging: aligned operand, even leading dimension and an even
   // extent along the pair direction (o for o-contiguous tiles, K for
   // k-contiguous ones; a LOWER_A row tile's K bound m0 + BM is even too)
-  const bool va = aligned16(A) && lda % 2 == 0 && (TA ? k % 2 == 0 : m % 2 == 0);
-  const bool vb = aligned16(B) && ldb % 2 == 0 && (TB ? n % 2 == 0 : k % 2 == 0);
+  const bool va = pair_aligned(A) && lda % 2 == 0 && (TA ? k % 2 == 0 : m % 2 == 0);
+  const bool vb = pair_aligned(B) && ldb % 2 == 0 && (TB ? n % 2 == 0 : k % 2 == 0);
   const dim3 g(static_cast<unsigned>(grid)), b(NT);
   if(va && vb)
-    hipLaunchKernelGGL((dgemm_kernel<BETA0, NEG, LOWER_A, TA, TB, true, true, KIND>),
+    hipLaunchKernelGGL((dgemm_kernel<V, BETA0, NEG, LOWER_A, TA, TB, true, true, KIND>),
                        g, b, 0, s, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
   else if(va)
-    hipLaunchKernelGGL((dgemm_kernel<BETA0, NEG, LOWER_A, TA, TB, true, false, KIND>),
+    hipLaunchKernelGGL((dgemm_kernel<V, BETA0, NEG, LOWER_A, TA, TB, true, false, KIND>),
                        g, b, 0, s, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
   else if(vb)
-    hipLaunchKernelGGL((dgemm_kernel<BETA0, NEG, LOWER_A, TA, TB, false, true, KIND>),
+    hipLaunchKernelGGL((dgemm_kernel<V, BETA0, NEG, LOWER_A, TA, TB, false, true, KIND>),
                        g, b, 0, s, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
   else
-    hipLaunchKernelGGL((dgemm_kernel<BETA0, NEG, LOWER_A, TA, TB, false, false, KIND>),
+    hipLaunchKernelGGL((dgemm_kernel<V, BETA0, NEG, LOWER_A, TA, TB, false, false, KIND>),
                        g, b, 0, s, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
   return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t launch_dgemm(hipStream_t s, int m, int n, int k, double alpha,
-                        const double* A, int lda, const double* B, int ldb,
-                        double beta, double* C, int ldc, bool lower_a,
-                        int kind, bool trans_a, bool trans_b) {
+template <typename V>
+hipError_t gemm_t(hipStream_t s, int m, int n, int k, double alpha,
+                  const V* A, int lda, const V* B, int ldb, double beta, V* C,
+                  int ldc, bool lower_a, int kind, bool trans_a, bool trans_b) {
   if(m <= 0 || n <= 0) return hipSuccess;
   if(!(alpha == 1.0 || alpha == -1.0) || !(beta == 0.0 || beta == 1.0))
     return hipErrorInvalidValue;
@@ -317,13 +336,15 @@ hipError_t launch_dgemm(hipStream_t s, int m, int n, int k, double alpha,
   if(k == 0 && !beta0) return hipSuccess;  // C unchanged
   if(trans_a || trans_b) {
     // products of the adjoint solves: C -= op(A) op(B) or C = -op(A) op(B)
-    if(lower_a || !neg) return hipErrorInvalidValue;
-    if(trans_a && !trans_b)
-      return beta0 ? run<true, true, false, true, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
-                   : run<false, true, false, true, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
-    if(!trans_a && trans_b)
-      return beta0 ? run<true, true, false, false, true, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
-                   : run<false, true, false, false, true, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
+    if constexpr(sizeof(V) == 8) {
+      if(lower_a || !neg) return hipErrorInvalidValue;
+      if(trans_a && !trans_b)
+        return beta0 ? run<true, true, false, true, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
+                     : run<false, true, false, true, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
+      if(!trans_a && trans_b)
+        return beta0 ? run<true, true, false, false, true, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
+                     : run<false, true, false, false, true, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
+    }
     return hipErrorInvalidValue;
   }
   if(beta0) {
@@ -333,13 +354,31 @@ hipError_t launch_dgemm(hipStream_t s, int m, int n, int k, double alpha,
     return lower_a ? run<true, false, true, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
                    : run<true, false, false, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
   }
-  if(neg && !lower_a && kind == 1)
-    return run<false, true, false, false, false, 1>(s, m, n, k, A, lda, B, ldb, C, ldc);
+  if constexpr(sizeof(V) == 8)
+    if(neg && !lower_a && kind == 1)
+      return run<false, true, false, false, false, 1>(s, m, n, k, A, lda, B, ldb, C, ldc);
   if(neg)
     return lower_a ? run<false, true, true, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
                    : run<false, true, false, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
   return lower_a ? run<false, false, true, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc)
                  : run<false, false, false, false, false, 0>(s, m, n, k, A, lda, B, ldb, C, ldc);
+}
+
+}  // namespace
+
+hipError_t launch_dgemm(hipStream_t s, int m, int n, int k, double alpha,
+                        const double* A, int lda, const double* B, int ldb,
+                        double beta, double* C, int ldc, bool lower_a,
+                        int kind, bool trans_a, bool trans_b) {
+  return gemm_t(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, lower_a, kind,
+                trans_a, trans_b);
+}
+
+hipError_t launch_sgemm(hipStream_t s, int m, int n, int k, double alpha,
+                        const float* A, int lda, const float* B, int ldb,
+                        double beta, float* C, int ldc, bool lower_a) {
+  return gemm_t(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, lower_a, 0,
+                false, false);
 }
 
 }  // namespace hplx

@@ -12,8 +12,8 @@
 // r = b - A x, the row sums (||A||_oo) and the column sums (||A||_1) are
 // accumulated.  ||r||_oo is the true maximum over all rows; the reference's
 // |r_0| form (src/pauxil/HPL_pdlange_device.cpp:284) is reported beside it.
-#include "hplx_internal.hpp"
 #include "lcg.hpp"
+#include "mxp.hpp"
 
 namespace hplx {
 namespace {
@@ -199,6 +199,64 @@ __global__ __launch_bounds__(256) void check_reduce_kernel(
   }
 }
 
+// Refinement residual, per row: r_i = b_i - sum_c part_r[c][i] (chunks in
+// order), row sums of |A| likewise; block maxima of |r_i|, the row sums,
+// |x_i| and |b_i| to bp[4*block + 0..3].
+__global__ __launch_bounds__(256) void ir_finish_kernel(
+    int n, int nch_c, uint64_t seed, const double* __restrict__ X,
+    const double* __restrict__ part_r, const double* __restrict__ part_a,
+    double* __restrict__ R, double* __restrict__ bp) {
+  __shared__ double sh[4];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double r = 0.0, rs = 0.0, xa = 0.0, ba = 0.0;
+  if(i < n) {
+    const double b = lcg_value(lcg_state(
+        seed, static_cast<uint64_t>(i) +
+                  static_cast<uint64_t>(n) * static_cast<uint64_t>(n)));
+    double ax = 0.0;
+    for(int c = 0; c < nch_c; ++c) {
+      ax += part_r[static_cast<size_t>(c) * n + i];
+      rs += part_a[static_cast<size_t>(c) * n + i];
+    }
+    r    = b - ax;
+    R[i] = r;
+    xa   = fabs(X[i]);
+    ba   = fabs(b);
+  }
+  const double m_r  = block_max(fabs(r), sh);
+  const double m_rs = block_max(rs, sh);
+  const double m_x  = block_max(xa, sh);
+  const double m_b  = block_max(ba, sh);
+  if(threadIdx.x == 0) {
+    double* o = bp + static_cast<size_t>(blockIdx.x) * 4;
+    o[0]      = m_r;
+    o[1]      = m_rs;
+    o[2]      = m_x;
+    o[3]      = m_b;
+  }
+}
+
+__global__ __launch_bounds__(256) void ir_reduce_kernel(
+    int n, int nblk, const double* __restrict__ bp, double* __restrict__ out) {
+  __shared__ double sh[4];
+  double v[4] = {0, 0, 0, 0};
+  for(int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+    for(int q = 0; q < 4; ++q) v[q] = fmax(v[q], bp[static_cast<size_t>(b) * 4 + q]);
+  const double rnorm = block_max(v[0], sh);
+  const double anorm = block_max(v[1], sh);
+  const double xnorm = block_max(v[2], sh);
+  const double bnorm = block_max(v[3], sh);
+  if(threadIdx.x == 0) {
+    const double eps = 0x1p-53;
+    out[0] = rnorm / (eps * (anorm * xnorm + bnorm) * static_cast<double>(n));
+    out[1] = rnorm;
+    out[2] = anorm;
+    out[3] = xnorm;
+    out[4] = bnorm;
+  }
+}
+
 }  // namespace
 
 hipError_t launch_matgen(hipStream_t s, int m, int n, double* A, int lda,
@@ -247,6 +305,35 @@ hipError_t launch_check(hipStream_t s, int n, int seed, double diag_shift,
                      nch_c, nch_r, sd, X, part_r, part_a, part_c, bp);
   hipLaunchKernelGGL(check_reduce_kernel, dim3(1), dim3(256), 0, s, n, nblk,
                      bp, out_dev);
+  return hipGetLastError();
+}
+
+size_t ir_part_len(int n) {
+  if(n <= 0) return 16;
+  const size_t nch_c = (n + kChkCols - 1) / kChkCols;
+  const size_t nblk  = (n + 255) / 256;
+  return 2 * nch_c * static_cast<size_t>(n) + 4 * nblk + 16;
+}
+
+hipError_t launch_ir_residual(hipStream_t s, int n, int seed,
+                              double diag_shift, const double* x, double* r,
+                              double* part, double* out) {
+  if(n <= 0) return hipErrorInvalidValue;
+  const int nch_c = (n + kChkCols - 1) / kChkCols;
+  const int nblk  = (n + 255) / 256;
+  double* part_r = part;
+  double* part_a = part_r + static_cast<size_t>(nch_c) * n;
+  double* bp     = part_a + static_cast<size_t>(nch_c) * n;
+  const Affine   colj = lcg_jump(static_cast<uint64_t>(n));
+  const uint64_t sd   = static_cast<uint64_t>(seed);
+  hipLaunchKernelGGL(check_rows_kernel,
+                     dim3(static_cast<unsigned>(nblk) * nch_c), dim3(256), 0, s,
+                     n, sd, diag_shift, x, part_r, part_a, colj.a, colj.c,
+                     nblk);
+  hipLaunchKernelGGL(ir_finish_kernel, dim3(nblk), dim3(256), 0, s, n, nch_c,
+                     sd, x, part_r, part_a, r, bp);
+  hipLaunchKernelGGL(ir_reduce_kernel, dim3(1), dim3(256), 0, s, n, nblk, bp,
+                     out);
   return hipGetLastError();
 }
 

@@ -1,0 +1,645 @@
+// Low-precision trailing update of the mixed-precision factorisations:
+// C -= L21 * U12 with C in the factor storage (FP32 or FP16), L21 and U12
+// rounded to BF16 or to per-tile-scaled FP8 e4m3, products on MFMA with FP32
+// accumulation.  It stands in for rocblas_dgemm of HPL_pdupdateTT
+// (src/pgesv/HPL_pdupdateTT.cpp:131-144) in the BF16 / FP8 forms; the
+// reference itself has no low-precision path.
+//
+// Operands are packed once per panel into the order the GEMM consumes them:
+//   A (L21, m x K):  [term][K stage][row][k in stage]
+//   B (U12, K x n):  [term][K stage][column][k in stage]
+// so that one workgroup's stage of A (or B) is one contiguous block of
+// rows x stage bytes, whatever the tile height.  A K stage is 32 (BF16, one
+// V_MFMA_F32_16X16X32_BF16 deep) or 128 (FP8, one
+// V_MFMA_SCALE_F32_16X16X128_F8F6F4 deep); K and the row/column counts are
+// padded with zeros to whole stages and whole 256-row tiles.
+//
+// BF16 forms.  Each FP32 operand x is split into `terms` BF16 numbers,
+// x ~ h0 + h1 + h2 with h0 = bf16(x), h1 = bf16(x - h0), h2 = bf16(x - h0 - h1)
+// (each difference is exact in FP32), and the product keeps the pairs whose
+// term indices add up to less than `terms`: 1 product for terms = 1 (plain
+// BF16), 3 for terms = 2 (about 16 significant bits), 6 for terms = 3 (about
+// FP32).  All products accumulate into the same FP32 accumulators.
+//
+// FP8 form.  L21 and U12 are cut into 128 x 128 tiles (row tile x K stage,
+// K stage x column tile); every tile gets a power-of-two scale 2^e with the
+// largest e such that max|tile| * 2^e <= 448 (the e4m3 maximum), is rounded
+// to e4m3 (round to nearest even) and keeps e.  Each 128-deep MFMA product
+// of an A tile with a B tile is multiplied by 2^-(eA + eB) and added to the
+// FP32 accumulators.
+//
+// Tile: BM x BN = 64*WM x 64*WN with WM x WN waves, each wave 64 x 64 as
+// 4 x 4 blocks of 16 x 16.  The MFMA computes the transposed block
+// (D = B^T A^T) so that 16 lanes hold 16 consecutive rows of one C column.
+// The next stage's global loads are in flight while the current stage's
+// MFMAs run; LDS is single-buffered (two barriers per stage).  Workgroup ids
+// are remapped XCD-aware as in the FP64 GEMM.
+#include "hplx_internal.hpp"
+#include "mxp.hpp"
+
+namespace hplx {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float  f32x4 __attribute__((ext_vector_type(4)));
+typedef int    i32x8 __attribute__((ext_vector_type(8)));
+typedef int    i32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kXcds   = 8;
+constexpr int kGroupM = 8;
+
+// ---------------------------------------------------------------------------
+// Rounding (bit-level, so that the rounding does not depend on conversion
+// instructions and matches a software model exactly).
+// ---------------------------------------------------------------------------
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ inline uint16_t bf16_bits(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_val(uint16_t h) {
+  return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+
+// FP32 -> OCP FP8 e4m3 (bias 7, max 448, subnormals 2^-9 apart), round to
+// nearest even; |x| <= 448 is guaranteed by the tile scale.
+__device__ inline uint8_t e4m3_bits(float x) {
+  const uint32_t u    = __float_as_uint(x);
+  const uint32_t sign = (u >> 24) & 0x80u;
+  const float    a    = fabsf(x);
+  if(a < 0x1p-10f) return static_cast<uint8_t>(sign);  // rounds to zero
+  int e = static_cast<int>((__float_as_uint(a) >> 23) & 0xff) - 127;
+  uint32_t q;
+  if(e < -6) {
+    // subnormal: value = m * 2^-9, m = 0..7 (8 rounds up into the normals)
+    const float m = a * 0x1p9f;
+    q             = static_cast<uint32_t>(__builtin_rintf(m));  // RNE
+    return static_cast<uint8_t>(sign | q);
+  }
+  // normal: 3 mantissa bits; round the FP32 mantissa to 3 bits, RNE
+  const uint32_t mant  = __float_as_uint(a) & 0x7fffffu;
+  const uint32_t keep  = mant >> 20;
+  const uint32_t rest  = mant & 0xfffffu;
+  uint32_t       r     = keep;
+  const uint32_t half  = 0x80000u;
+  if(rest > half || (rest == half && (keep & 1u))) ++r;
+  int ee = e + 7;
+  if(r == 8u) {
+    r = 0;
+    ++ee;
+  }
+  q = (static_cast<uint32_t>(ee) << 3) | r;
+  return static_cast<uint8_t>(sign | q);
+}
+
+// Largest e with amax * 2^e <= 448 (0 for an all-zero tile), clamped to the
+// range the scale arithmetic uses.
+__device__ inline int fp8_tile_exp(float amax) {
+  if(!(amax > 0.0f)) return 0;
+  int e = static_cast<int>(floorf(log2f(448.0f / amax)));
+  // guard the log2 rounding at exact powers
+  while(ldexpf(amax, e) > 448.0f) --e;
+  while(ldexpf(amax, e + 1) <= 448.0f) ++e;
+  return e < -100 ? -100 : (e > 100 ? 100 : e);
+}
+
+// ---------------------------------------------------------------------------
+// Packing.
+// ---------------------------------------------------------------------------
+
+// A (m x K, column-major FP32) -> BF16 terms, [term][ks][row][32].  Block:
+// 64 rows x one 32-deep stage, 256 threads.
+template <int TERMS>
+__global__ __launch_bounds__(256) void pack_a_bf16_kernel(
+    int m, int K, const float* __restrict__ L, int ldl,
+    uint16_t* __restrict__ Ap, int m_pad, size_t plane) {
+  __shared__ float tile[32][65];
+  const int t  = threadIdx.x;
+  const int r0 = blockIdx.x * 64;
+  const int ks = blockIdx.y;
+  const int k0 = ks * 32;
+#pragma unroll
+  for(int j = 0; j < 8; ++j) {
+    const int kk = (t >> 6) + 4 * j;
+    const int r  = r0 + (t & 63);
+    const int k  = k0 + kk;
+    tile[kk][t & 63] =
+        (r < m && k < K) ? L[r + static_cast<size_t>(k) * ldl] : 0.0f;
+  }
+  __syncthreads();
+  const int row = t >> 2, seg = t & 3;
+  uint16_t  h[TERMS][8];
+#pragma unroll
+  for(int i = 0; i < 8; ++i) {
+    float x = tile[seg * 8 + i][row];
+#pragma unroll
+    for(int q = 0; q < TERMS; ++q) {
+      h[q][i] = bf16_bits(x);
+      x -= bf16_val(h[q][i]);
+    }
+  }
+  const size_t off =
+      (static_cast<size_t>(ks) * m_pad + r0 + row) * 32 + seg * 8;
+#pragma unroll
+  for(int q = 0; q < TERMS; ++q) {
+    i32x4 v;
+#pragma unroll
+    for(int i = 0; i < 4; ++i)
+      v[i] = static_cast<int>(h[q][2 * i] | (static_cast<uint32_t>(h[q][2 * i + 1]) << 16));
+    *reinterpret_cast<i32x4*>(Ap + q * plane + off) = v;
+  }
+}
+
+// B (K x n, column-major FP32) -> BF16 terms, [term][ks][col][32].  Thread =
+// (column, 8-deep segment of one stage).
+template <int TERMS>
+__global__ __launch_bounds__(256) void pack_b_bf16_kernel(
+    int K, int n, const float* __restrict__ U, int ldu,
+    uint16_t* __restrict__ Bp, int n_pad, size_t plane) {
+  const int t   = threadIdx.x;
+  const int col = blockIdx.x * 64 + (t >> 2);
+  const int seg = t & 3;
+  const int ks  = blockIdx.y;
+  const int k0  = ks * 32 + seg * 8;
+  if(col >= n_pad) return;
+  uint16_t h[TERMS][8];
+#pragma unroll
+  for(int i = 0; i < 8; ++i) {
+    const int k = k0 + i;
+    float     x = (col < n && k < K) ? U[k + static_cast<size_t>(col) * ldu] : 0.0f;
+#pragma unroll
+    for(int q = 0; q < TERMS; ++q) {
+      h[q][i] = bf16_bits(x);
+      x -= bf16_val(h[q][i]);
+    }
+  }
+  const size_t off = (static_cast<size_t>(ks) * n_pad + col) * 32 + seg * 8;
+#pragma unroll
+  for(int q = 0; q < TERMS; ++q) {
+    i32x4 v;
+#pragma unroll
+    for(int i = 0; i < 4; ++i)
+      v[i] = static_cast<int>(h[q][2 * i] | (static_cast<uint32_t>(h[q][2 * i + 1]) << 16));
+    *reinterpret_cast<i32x4*>(Bp + q * plane + off) = v;
+  }
+}
+
+__device__ inline float block_amax(float v, float* sh) {
+  for(int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  const int w = threadIdx.x >> 6;
+  if((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  float r = sh[0];
+  for(int k = 1; k < static_cast<int>(blockDim.x >> 6); ++k) r = fmaxf(r, sh[k]);
+  return r;
+}
+
+// A (m x K FP32) -> e4m3 [ks][row][128] + exponents ea[row tile][ks].
+// Block: one 128 x 128 tile, 256 threads; thread = row (t & 127) and every
+// other column.
+__global__ __launch_bounds__(256) void pack_a_fp8_kernel(
+    int m, int K, const float* __restrict__ L, int ldl,
+    uint8_t* __restrict__ Ap, int m_pad, int nks, int* __restrict__ ea) {
+  __shared__ float   sh[4];
+  __shared__ uint8_t q8[128][132];
+  const int t  = threadIdx.x;
+  const int mt = blockIdx.x, ks = blockIdx.y;
+  const int r  = mt * 128 + (t & 127);
+  float     v[64];
+  float     amax = 0.0f;
+#pragma unroll
+  for(int j = 0; j < 64; ++j) {
+    const int k = ks * 128 + 2 * j + (t >> 7);
+    v[j]        = (r < m && k < K) ? L[r + static_cast<size_t>(k) * ldl] : 0.0f;
+    amax        = fmaxf(amax, fabsf(v[j]));
+  }
+  const int   e = fp8_tile_exp(block_amax(amax, sh));
+  const float s = ldexpf(1.0f, e);
+#pragma unroll
+  for(int j = 0; j < 64; ++j) q8[t & 127][2 * j + (t >> 7)] = e4m3_bits(v[j] * s);
+  if(t == 0) ea[mt * nks + ks] = e;
+  __syncthreads();
+  // write rows: thread = (row t/2, half t%2), 64 bytes each
+  const int row = t >> 1, half = t & 1;
+  uint8_t*  dst = Ap + (static_cast<size_t>(ks) * m_pad + mt * 128 + row) * 128 + half * 64;
+#pragma unroll
+  for(int c = 0; c < 4; ++c) {
+    i32x4 w;
+#pragma unroll
+    for(int i = 0; i < 4; ++i) {
+      const int b = half * 64 + c * 16 + i * 4;
+      w[i]        = static_cast<int>(q8[row][b] | (q8[row][b + 1] << 8) |
+                                     (q8[row][b + 2] << 16) |
+                                     (static_cast<uint32_t>(q8[row][b + 3]) << 24));
+    }
+    *reinterpret_cast<i32x4*>(dst + c * 16) = w;
+  }
+}
+
+// B (K x n FP32) -> e4m3 [ks][col][128] + exponents eb[ks][col tile].
+// Block: one 128 x 128 tile; thread = (column t/2, half of the stage t%2).
+__global__ __launch_bounds__(256) void pack_b_fp8_kernel(
+    int K, int n, const float* __restrict__ U, int ldu,
+    uint8_t* __restrict__ Bp, int n_pad, int ntn, int* __restrict__ eb) {
+  __shared__ float sh[4];
+  const int t   = threadIdx.x;
+  const int nt  = blockIdx.x, ks = blockIdx.y;
+  const int col = nt * 128 + (t >> 1);
+  const int k0  = ks * 128 + (t & 1) * 64;
+  float     v[64];
+  float     amax = 0.0f;
+#pragma unroll
+  for(int j = 0; j < 64; ++j) {
+    const int k = k0 + j;
+    v[j]        = (col < n && k < K) ? U[k + static_cast<size_t>(col) * ldu] : 0.0f;
+    amax        = fmaxf(amax, fabsf(v[j]));
+  }
+  const int   e = fp8_tile_exp(block_amax(amax, sh));
+  const float s = ldexpf(1.0f, e);
+  if(t == 0) eb[ks * ntn + nt] = e;
+  uint8_t* dst = Bp + (static_cast<size_t>(ks) * n_pad + col) * 128 + (t & 1) * 64;
+#pragma unroll
+  for(int c = 0; c < 4; ++c) {
+    i32x4 w;
+#pragma unroll
+    for(int i = 0; i < 4; ++i) {
+      const int j = c * 16 + i * 4;
+      w[i]        = static_cast<int>(e4m3_bits(v[j] * s) |
+                                     (e4m3_bits(v[j + 1] * s) << 8) |
+                                     (e4m3_bits(v[j + 2] * s) << 16) |
+                                     (static_cast<uint32_t>(e4m3_bits(v[j + 3] * s)) << 24));
+    }
+    *reinterpret_cast<i32x4*>(dst + c * 16) = w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM
+// ---------------------------------------------------------------------------
+
+template <typename S>
+__device__ inline float ld_c(const S* p) {
+  return static_cast<float>(*p);
+}
+template <typename S>
+__device__ inline void st_c(S* p, float v) {
+  *p = static_cast<S>(v);
+}
+
+// XCD-aware tile id; false when the workgroup has no tile.
+__device__ inline bool map_tile(int tiles_m, int tiles_n, int& tm, int& tn) {
+  const int T    = tiles_m * tiles_n;
+  const int per  = gridDim.x / kXcds;
+  const int bid  = blockIdx.x;
+  const int tile = (bid % kXcds) * per + bid / kXcds;
+  if(tile >= T) return false;
+  const int width  = kGroupM * tiles_n;
+  const int group  = tile / width;
+  const int firstm = group * kGroupM;
+  const int gsz    = min(tiles_m - firstm, kGroupM);
+  const int within = tile - group * width;
+  tm               = firstm + within % gsz;
+  tn               = within / gsz;
+  return true;
+}
+
+// Pairs (a-term, b-term) of the split product, largest first.
+__device__ constexpr int kPairA[6] = {0, 0, 1, 0, 1, 2};
+__device__ constexpr int kPairB[6] = {0, 1, 0, 2, 1, 0};
+
+template <int TERMS, int WM, int WN, typename S>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
+    int M, int N, int nks, const uint16_t* __restrict__ Ap, int m_pad,
+    size_t aplane, const uint16_t* __restrict__ Bp, int n_pad, size_t bplane,
+    S* __restrict__ C, int ldc, int tiles_m, int tiles_n) {
+  constexpr int NT     = 64 * WM * WN;
+  constexpr int BM     = 64 * WM;
+  constexpr int BN     = 64 * WN;
+  constexpr int RS     = 40;  // LDS row stride in bf16 (80 bytes)
+  constexpr int NPROD  = TERMS * (TERMS + 1) / 2;
+  constexpr int ACH    = BM * 4 / NT;  // 16-byte chunks per thread, A, per term
+  constexpr int BCH    = BN * 4 / NT;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[TERMS][BM * RS];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[TERMS][BN * RS];
+
+  int tm, tn;
+  if(!map_tile(tiles_m, tiles_n, tm, tn)) return;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm0 = (wave % WM) * 64, wn0 = (wave / WM) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  i32x4 ra[TERMS][ACH], rb[TERMS][BCH];
+  auto load = [&](int ks) {
+#pragma unroll
+    for(int q = 0; q < TERMS; ++q) {
+      const uint16_t* a = Ap + q * aplane + (static_cast<size_t>(ks) * m_pad + m0) * 32;
+      const uint16_t* b = Bp + q * bplane + (static_cast<size_t>(ks) * n_pad + n0) * 32;
+#pragma unroll
+      for(int c = 0; c < ACH; ++c)
+        ra[q][c] = *reinterpret_cast<const i32x4*>(a + (t + c * NT) * 8);
+#pragma unroll
+      for(int c = 0; c < BCH; ++c)
+        rb[q][c] = *reinterpret_cast<const i32x4*>(b + (t + c * NT) * 8);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for(int q = 0; q < TERMS; ++q) {
+#pragma unroll
+      for(int c = 0; c < ACH; ++c) {
+        const int ch = t + c * NT;
+        *reinterpret_cast<i32x4*>(&sA[q][(ch >> 2) * RS + (ch & 3) * 8]) = ra[q][c];
+      }
+#pragma unroll
+      for(int c = 0; c < BCH; ++c) {
+        const int ch = t + c * NT;
+        *reinterpret_cast<i32x4*>(&sB[q][(ch >> 2) * RS + (ch & 3) * 8]) = rb[q][c];
+      }
+    }
+  };
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for(int i = 0; i < 4; ++i)
+#pragma unroll
+    for(int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  load(0);
+  for(int ks = 0; ks < nks; ++ks) {
+    store();
+    __syncthreads();
+    if(ks + 1 < nks) load(ks + 1);
+    bf16x8 fa[TERMS][4], fb[TERMS][4];
+#pragma unroll
+    for(int q = 0; q < TERMS; ++q) {
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+        fa[q][i] = *reinterpret_cast<const bf16x8*>(&sA[q][(wm0 + i * 16 + l16) * RS + lk * 8]);
+#pragma unroll
+      for(int j = 0; j < 4; ++j)
+        fb[q][j] = *reinterpret_cast<const bf16x8*>(&sB[q][(wn0 + j * 16 + l16) * RS + lk * 8]);
+    }
+#pragma unroll
+    for(int p = NPROD - 1; p >= 0; --p)  // smallest products first
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              fb[kPairB[p]][j], fa[kPairA[p]][i], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+
+  // epilogue: C -= acc; lane l16 is the row, 4*(l/16)+r the column
+#pragma unroll
+  for(int i = 0; i < 4; ++i) {
+    const int row = m0 + wm0 + i * 16 + l16;
+    if(row >= M) continue;
+#pragma unroll
+    for(int j = 0; j < 4; ++j)
+#pragma unroll
+      for(int r = 0; r < 4; ++r) {
+        const int col = n0 + wn0 + j * 16 + lk * 4 + r;
+        if(col < N) {
+          S* c = C + row + static_cast<size_t>(col) * ldc;
+          st_c(c, ld_c(c) - acc[i][j][r]);
+        }
+      }
+  }
+}
+
+template <int WM, int WN, typename S>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
+    int M, int N, int nks, const uint8_t* __restrict__ Ap, int m_pad,
+    const int* __restrict__ ea, const uint8_t* __restrict__ Bp, int n_pad,
+    const int* __restrict__ eb, int ntn128, S* __restrict__ C, int ldc,
+    int tiles_m, int tiles_n) {
+  constexpr int NT  = 64 * WM * WN;
+  constexpr int BM  = 64 * WM;
+  constexpr int BN  = 64 * WN;
+  constexpr int RS  = 144;  // LDS row stride in bytes
+  constexpr int ACH = BM * 8 / NT;  // 16-byte chunks per thread (128 B rows)
+  constexpr int BCH = BN * 8 / NT;
+  __shared__ __attribute__((aligned(16))) uint8_t sA[BM * RS];
+  __shared__ __attribute__((aligned(16))) uint8_t sB[BN * RS];
+
+  int tm, tn;
+  if(!map_tile(tiles_m, tiles_n, tm, tn)) return;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm0 = (wave % WM) * 64, wn0 = (wave / WM) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int atile = (m0 + wm0) / 128;  // this wave's scale tiles
+  const int btile = (n0 + wn0) / 128;
+
+  i32x4 ra[ACH], rb[BCH];
+  auto load = [&](int ks) {
+    const uint8_t* a = Ap + (static_cast<size_t>(ks) * m_pad + m0) * 128;
+    const uint8_t* b = Bp + (static_cast<size_t>(ks) * n_pad + n0) * 128;
+#pragma unroll
+    for(int c = 0; c < ACH; ++c)
+      ra[c] = *reinterpret_cast<const i32x4*>(a + (t + c * NT) * 16);
+#pragma unroll
+    for(int c = 0; c < BCH; ++c)
+      rb[c] = *reinterpret_cast<const i32x4*>(b + (t + c * NT) * 16);
+  };
+  auto store = [&]() {
+#pragma unroll
+    for(int c = 0; c < ACH; ++c) {
+      const int ch = t + c * NT;
+      *reinterpret_cast<i32x4*>(&sA[(ch >> 3) * RS + (ch & 7) * 16]) = ra[c];
+    }
+#pragma unroll
+    for(int c = 0; c < BCH; ++c) {
+      const int ch = t + c * NT;
+      *reinterpret_cast<i32x4*>(&sB[(ch >> 3) * RS + (ch & 7) * 16]) = rb[c];
+    }
+  };
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for(int i = 0; i < 4; ++i)
+#pragma unroll
+    for(int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+
+  load(0);
+  for(int ks = 0; ks < nks; ++ks) {
+    store();
+    __syncthreads();
+    if(ks + 1 < nks) load(ks + 1);
+    const float s = ldexpf(1.0f, -(ea[atile * nks + ks] + eb[ks * ntn128 + btile]));
+    i32x8 fa[4], fb[4];
+#pragma unroll
+    for(int i = 0; i < 4; ++i) {
+      const uint8_t* p = &sA[(wm0 + i * 16 + l16) * RS + lk * 32];
+      const i32x4    lo = *reinterpret_cast<const i32x4*>(p);
+      const i32x4    hi = *reinterpret_cast<const i32x4*>(p + 16);
+      fa[i] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    }
+#pragma unroll
+    for(int j = 0; j < 4; ++j) {
+      const uint8_t* p = &sB[(wn0 + j * 16 + l16) * RS + lk * 32];
+      const i32x4    lo = *reinterpret_cast<const i32x4*>(p);
+      const i32x4    hi = *reinterpret_cast<const i32x4*>(p + 16);
+      fb[j] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    }
+#pragma unroll
+    for(int i = 0; i < 4; ++i)
+#pragma unroll
+      for(int j = 0; j < 4; ++j) {
+        // format 0 = e4m3 for both operands, E8M0 scales 127 = 2^0
+        const f32x4 d = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+            fb[j], fa[i], zero, 0, 0, 0, 127, 0, 127);
+#pragma unroll
+        for(int r = 0; r < 4; ++r) acc[i][j][r] = fmaf(d[r], s, acc[i][j][r]);
+      }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for(int i = 0; i < 4; ++i) {
+    const int row = m0 + wm0 + i * 16 + l16;
+    if(row >= M) continue;
+#pragma unroll
+    for(int j = 0; j < 4; ++j)
+#pragma unroll
+      for(int r = 0; r < 4; ++r) {
+        const int col = n0 + wn0 + j * 16 + lk * 4 + r;
+        if(col < N) {
+          S* c = C + row + static_cast<size_t>(col) * ldc;
+          st_c(c, ld_c(c) - acc[i][j][r]);
+        }
+      }
+  }
+}
+
+constexpr int kWM = 2, kWN = 2;  // 128 x 128 workgroup tiles
+
+inline long tile_grid(int tiles_m, int tiles_n) {
+  const long T = static_cast<long>(tiles_m) * tiles_n;
+  return ((T + kXcds - 1) / kXcds) * kXcds;
+}
+
+template <typename S>
+hipError_t gemm_bf16_t(hipStream_t s, int terms, int m, int n, int K,
+                       const MxpPacked& P, S* C, int ldc) {
+  if(m <= 0 || n <= 0 || K <= 0) return hipSuccess;
+  constexpr int BM = 64 * kWM, BN = 64 * kWN;
+  const int  tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+  const long grid    = tile_grid(tiles_m, tiles_n);
+  if(grid > 0x7fffffffL) return hipErrorInvalidValue;
+  const int nks = (K + 31) / 32;
+  const dim3 g(static_cast<unsigned>(grid)), b(64 * kWM * kWN);
+  const uint16_t* A = static_cast<const uint16_t*>(P.a);
+  const uint16_t* B = static_cast<const uint16_t*>(P.b);
+  switch(terms) {
+    case 1:
+      hipLaunchKernelGGL((gemm_bf16_kernel<1, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
+                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n);
+      break;
+    case 2:
+      hipLaunchKernelGGL((gemm_bf16_kernel<2, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
+                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n);
+      break;
+    case 3:
+      hipLaunchKernelGGL((gemm_bf16_kernel<3, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
+                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename S>
+hipError_t gemm_fp8_t(hipStream_t s, int m, int n, int K, const MxpPacked& P,
+                      S* C, int ldc) {
+  if(m <= 0 || n <= 0 || K <= 0) return hipSuccess;
+  constexpr int BM = 64 * kWM, BN = 64 * kWN;
+  const int  tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+  const long grid    = tile_grid(tiles_m, tiles_n);
+  if(grid > 0x7fffffffL) return hipErrorInvalidValue;
+  const int nks = (K + 127) / 128;
+  const dim3 g(static_cast<unsigned>(grid)), b(64 * kWM * kWN);
+  hipLaunchKernelGGL((gemm_fp8_kernel<kWM, kWN, S>), g, b, 0, s, m, n, nks,
+                     static_cast<const uint8_t*>(P.a), P.m_pad, P.ea,
+                     static_cast<const uint8_t*>(P.b), P.n_pad, P.eb,
+                     P.n_pad / 128, C, ldc, tiles_m, tiles_n);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t mxp_pack_bytes(int update, int terms, int rows, int K) {
+  const size_t rp = static_cast<size_t>((rows + kMxpRowPad - 1) / kMxpRowPad) * kMxpRowPad;
+  if(update == HPLX_UPDATE_FP8) {
+    const size_t kp = static_cast<size_t>((K + 127) / 128) * 128;
+    return rp * kp;  // one byte per element
+  }
+  const size_t kp = static_cast<size_t>((K + 31) / 32) * 32;
+  return rp * kp * 2 * static_cast<size_t>(terms);
+}
+
+size_t mxp_scale_count(int rows, int K) {
+  return static_cast<size_t>((rows + kMxpRowPad - 1) / kMxpRowPad) *
+         (kMxpRowPad / 128) * static_cast<size_t>((K + 127) / 128);
+}
+
+hipError_t launch_mxp_pack(hipStream_t s, int update, int terms, int m, int K,
+                           const float* L, int ldl, int n, const float* U,
+                           int ldu, MxpPacked& P) {
+  P.m_pad = (m + kMxpRowPad - 1) / kMxpRowPad * kMxpRowPad;
+  P.n_pad = (n + kMxpRowPad - 1) / kMxpRowPad * kMxpRowPad;
+  if(m <= 0 || n <= 0 || K <= 0) return hipSuccess;
+  if(update == HPLX_UPDATE_FP8) {
+    const int nks = (K + 127) / 128;
+    P.aplane = P.bplane = 0;
+    hipLaunchKernelGGL(pack_a_fp8_kernel, dim3(P.m_pad / 128, nks), dim3(256), 0, s,
+                       m, K, L, ldl, static_cast<uint8_t*>(P.a), P.m_pad, nks, P.ea);
+    hipLaunchKernelGGL(pack_b_fp8_kernel, dim3(P.n_pad / 128, nks), dim3(256), 0, s,
+                       K, n, U, ldu, static_cast<uint8_t*>(P.b), P.n_pad, P.n_pad / 128,
+                       P.eb);
+    return hipGetLastError();
+  }
+  if(update != HPLX_UPDATE_BF16) return hipErrorInvalidValue;
+  const int nks = (K + 31) / 32;
+  P.aplane      = static_cast<size_t>(nks) * P.m_pad * 32;
+  P.bplane      = static_cast<size_t>(nks) * P.n_pad * 32;
+  uint16_t* A   = static_cast<uint16_t*>(P.a);
+  uint16_t* B   = static_cast<uint16_t*>(P.b);
+  const dim3 ga(P.m_pad / 64, nks), gb(P.n_pad / 64, nks), b(256);
+  switch(terms) {
+    case 1:
+      hipLaunchKernelGGL(pack_a_bf16_kernel<1>, ga, b, 0, s, m, K, L, ldl, A, P.m_pad, P.aplane);
+      hipLaunchKernelGGL(pack_b_bf16_kernel<1>, gb, b, 0, s, K, n, U, ldu, B, P.n_pad, P.bplane);
+      break;
+    case 2:
+      hipLaunchKernelGGL(pack_a_bf16_kernel<2>, ga, b, 0, s, m, K, L, ldl, A, P.m_pad, P.aplane);
+      hipLaunchKernelGGL(pack_b_bf16_kernel<2>, gb, b, 0, s, K, n, U, ldu, B, P.n_pad, P.bplane);
+      break;
+    case 3:
+      hipLaunchKernelGGL(pack_a_bf16_kernel<3>, ga, b, 0, s, m, K, L, ldl, A, P.m_pad, P.aplane);
+      hipLaunchKernelGGL(pack_b_bf16_kernel<3>, gb, b, 0, s, K, n, U, ldu, B, P.n_pad, P.bplane);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
+                           int K, const MxpPacked& P, float* C, int ldc) {
+  return update == HPLX_UPDATE_FP8 ? gemm_fp8_t(s, m, n, K, P, C, ldc)
+                                   : gemm_bf16_t(s, terms, m, n, K, P, C, ldc);
+}
+
+hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
+                           int K, const MxpPacked& P, __half* C, int ldc) {
+  return update == HPLX_UPDATE_FP8 ? gemm_fp8_t(s, m, n, K, P, C, ldc)
+                                   : gemm_bf16_t(s, terms, m, n, K, P, C, ldc);
+}
+
+}  // namespace hplx

@@ -1,0 +1,114 @@
+// Small kernels of the mixed-precision path: storage <-> FP32 block copies,
+// FP64 vector row interchanges, x += d, and the right-hand side b generated
+// in FP64 from the matrix generator (column n of [A|b], state i + n*n).
+#include "lcg.hpp"
+#include "mxp.hpp"
+
+namespace hplx {
+namespace {
+
+template <typename D, typename S>
+__device__ inline D cvt(S x) {
+  if constexpr(sizeof(D) == 2)
+    return __float2half(static_cast<float>(x));
+  else if constexpr(sizeof(S) == 2)
+    return static_cast<D>(__half2float(x));
+  else
+    return static_cast<D>(x);
+}
+
+// B (m x n) = A (m x n), converted.  Block: 256 rows of one column; gx
+// blocks per column.
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void copy_block_kernel(
+    int m, int gx, const S* __restrict__ A, int lda, D* __restrict__ B,
+    int ldb) {
+  const int j = blockIdx.x / gx;
+  const int i = (blockIdx.x % gx) * 256 + threadIdx.x;
+  if(i < m)
+    B[i + static_cast<size_t>(j) * ldb] =
+        cvt<D>(A[i + static_cast<size_t>(j) * lda]);
+}
+
+// Sequential interchanges y[i] <-> y[ipiv[i]], i = k0 .. k0+jb-1.
+__global__ void vec_swaps_kernel(int k0, int jb, const int* __restrict__ ipiv,
+                                 double* __restrict__ y) {
+  for(int i = k0; i < k0 + jb; ++i) {
+    const int p = ipiv[i];
+    if(p != i) {
+      const double t = y[i];
+      y[i]           = y[p];
+      y[p]           = t;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void axpy_kernel(int n,
+                                                   const double* __restrict__ d,
+                                                   double* __restrict__ x) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i < n) x[i] += d[i];
+}
+
+__global__ __launch_bounds__(256) void gen_b_kernel(int n, uint64_t seed,
+                                                    double* __restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i < n)
+    y[i] = lcg_value(lcg_state(seed, static_cast<uint64_t>(i) +
+                                         static_cast<uint64_t>(n) *
+                                             static_cast<uint64_t>(n)));
+}
+
+template <typename S, typename D>
+hipError_t copy_t(hipStream_t s, int m, int n, const S* A, int lda, D* B,
+                  int ldb) {
+  if(m <= 0 || n <= 0) return hipSuccess;
+  const long gx = (m + 255) / 256;
+  if(gx * n > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((copy_block_kernel<S, D>),
+                     dim3(static_cast<unsigned>(gx * n)), dim3(256), 0, s, m,
+                     static_cast<int>(gx), A, lda, B, ldb);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_copy_block(hipStream_t s, int m, int n, const __half* A,
+                             int lda, float* B, int ldb) {
+  return copy_t(s, m, n, A, lda, B, ldb);
+}
+hipError_t launch_copy_block(hipStream_t s, int m, int n, const float* A,
+                             int lda, __half* B, int ldb) {
+  return copy_t(s, m, n, A, lda, B, ldb);
+}
+hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
+                             int lda, float* B, int ldb) {
+  return copy_t(s, m, n, A, lda, B, ldb);
+}
+hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
+                             int lda, __half* B, int ldb) {
+  return copy_t(s, m, n, A, lda, B, ldb);
+}
+
+hipError_t launch_vec_swaps(hipStream_t s, int k0, int jb, const int* ipiv,
+                            double* y) {
+  if(jb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(vec_swaps_kernel, dim3(1), dim3(1), 0, s, k0, jb, ipiv, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy(hipStream_t s, int n, const double* d, double* x) {
+  if(n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d,
+                     x);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_b(hipStream_t s, int n, int seed, double* y) {
+  if(n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gen_b_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                     static_cast<uint64_t>(seed), y);
+  return hipGetLastError();
+}
+
+}  // namespace hplx

@@ -120,22 +120,25 @@ __global__ __launch_bounds__(1024) void pipid_kernel(
 }
 
 // New top rows -> W (ld ldw); displaced top rows -> their destinations.
+// S: storage type of A; WT: type of W (the FP16-stored factorisation gathers
+// into FP32).
+template <typename S, typename WT>
 __global__ __launch_bounds__(256) void swap_gather_kernel(
-    int jb, double* __restrict__ A, int lda, const int* __restrict__ permU,
-    double* __restrict__ W, int ldw) {
+    int jb, S* __restrict__ A, int lda, const int* __restrict__ permU,
+    WT* __restrict__ W, int ldw) {
   constexpr int Q   = kMaxNB / 256;
   const int     col = blockIdx.x;
-  double*       ac  = A + static_cast<size_t>(col) * lda;
-  double*       wc  = W + static_cast<size_t>(col) * ldw;
-  double        keep[Q];
+  S*            ac  = A + static_cast<size_t>(col) * lda;
+  WT*           wc  = W + static_cast<size_t>(col) * ldw;
+  S             keep[Q];
   int           dst[Q];
 #pragma unroll
   for(int q = 0; q < Q; ++q) {
     const int i = threadIdx.x + 256 * q;
     dst[q]      = -1;
-    keep[q]     = 0.0;
+    keep[q]     = S(0);
     if(i < jb) {
-      wc[i]  = ac[permU[i]];  // the store waits for the gathered value
+      wc[i]  = static_cast<WT>(ac[permU[i]]);  // the store waits for the load
       dst[q] = permU[jb + i];
       if(dst[q] >= 0) keep[q] = ac[i];
     }
@@ -210,8 +213,17 @@ __global__ __launch_bounds__(256) void swap_inverse_kernel(
 // host side
 // ---------------------------------------------------------------------------
 
-hipError_t launch_leaf(Ctx& c, double* P, int lda, int m, int JB, int c0,
-                       int w, int* ipiv, int row_offset, int info_base) {
+template <typename V>
+const void* leaf_entry() {
+  if constexpr(sizeof(V) == 8)
+    return leaf_kernel_ptr<LeafKind::kF64>();
+  else
+    return leaf_kernel_ptr<LeafKind::kF32>();
+}
+
+template <typename V>
+hipError_t launch_leaf(Ctx& c, V* P, int lda, int m, int JB, int c0, int w,
+                       int* ipiv, int row_offset, int info_base) {
   constexpr int CH = kLeafThreads * kLeafRPT;
   const int     R  = m - c0;
   const int     G  = (R + CH - 1) / CH;
@@ -250,14 +262,25 @@ hipError_t launch_leaf(Ctx& c, double* P, int lda, int m, int JB, int c0,
   // API adds nothing here and refuses to launch while another process holds
   // the device.)
   void* args[] = {&la};
-  return hipLaunchKernel(leaf_kernel_ptr<LeafKind::kF64>(), dim3(G),
-                         dim3(kLeafThreads), args, 0, c.stream);
+  return hipLaunchKernel(leaf_entry<V>(), dim3(G), dim3(kLeafThreads), args,
+                         0, c.stream);
+}
+
+inline hipError_t panel_gemm(hipStream_t s, int m, int n, int k,
+                             const double* A, int lda, const double* B,
+                             int ldb, double* C, int ldc) {
+  return launch_dgemm(s, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc, false);
+}
+inline hipError_t panel_gemm(hipStream_t s, int m, int n, int k,
+                             const float* A, int lda, const float* B, int ldb,
+                             float* C, int ldc) {
+  return launch_sgemm(s, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc, false);
 }
 
 // Columns [c0, c0+n) of the panel, rows [c0, m).
-hipError_t leaf_or_chunks(Ctx& c, double* P, int lda, int m,
-                          int JB, int c0, int n, int* ipiv, int row_offset,
-                          int info_base) {
+template <typename V>
+hipError_t leaf_or_chunks(Ctx& c, V* P, int lda, int m, int JB, int c0, int n,
+                          int* ipiv, int row_offset, int info_base) {
   if(n <= kLeafMax)
     return launch_leaf(c, P, lda, m, JB, c0, n, ipiv, row_offset,
                        info_base);
@@ -270,21 +293,21 @@ hipError_t leaf_or_chunks(Ctx& c, double* P, int lda, int m,
     const int top  = c0 + k;
     const int rest = n - k - w;
     if(rest > 0) {
-      double* L11 = P + top + static_cast<size_t>(top) * lda;
-      double* A12 = P + top + static_cast<size_t>(top + w) * lda;
+      V* L11 = P + top + static_cast<size_t>(top) * lda;
+      V* A12 = P + top + static_cast<size_t>(top + w) * lda;
       e = launch_trsm_colsweep(c.stream, w, rest, L11, lda, A12, lda);
       if(e != hipSuccess) return e;
-      e = launch_dgemm(c.stream, m - top - w, rest, w, -1.0, L11 + w, lda, A12,
-                       lda, 1.0, A12 + w, lda, false);
+      e = panel_gemm(c.stream, m - top - w, rest, w, L11 + w, lda, A12, lda,
+                     A12 + w, lda);
       if(e != hipSuccess) return e;
     }
   }
   return hipSuccess;
 }
 
-hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, double* P,
-                  int lda, int m, int JB, int c0, int n, int* ipiv,
-                  int row_offset, int info_base) {
+template <typename V>
+hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, V* P, int lda, int m, int JB,
+                  int c0, int n, int* ipiv, int row_offset, int info_base) {
   if(n <= nbmin)
     return leaf_or_chunks(c, P, lda, m, JB, c0, n, ipiv, row_offset,
                           info_base);
@@ -297,12 +320,12 @@ hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, double* P,
     if(e != hipSuccess) return e;
     const int top = c0 + jj;
     if(nrem > 0) {
-      double* L11 = P + top + static_cast<size_t>(top) * lda;
-      double* A12 = P + top + static_cast<size_t>(top + jb) * lda;
+      V* L11 = P + top + static_cast<size_t>(top) * lda;
+      V* A12 = P + top + static_cast<size_t>(top + jb) * lda;
       e = launch_trsm_colsweep(c.stream, jb, nrem, L11, lda, A12, lda);
       if(e != hipSuccess) return e;
-      e = launch_dgemm(c.stream, m - top - jb, nrem, jb, -1.0, L11 + jb, lda,
-                       A12, lda, 1.0, A12 + jb, lda, false);
+      e = panel_gemm(c.stream, m - top - jb, nrem, jb, L11 + jb, lda, A12, lda,
+                     A12 + jb, lda);
       if(e != hipSuccess) return e;
     }
     jj += jb;
@@ -311,11 +334,30 @@ hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, double* P,
   return hipSuccess;
 }
 
+template <typename V>
+hipError_t panel_factor_t(Ctx& c, int nbmin, int nbdiv, V* P, int lda, int m,
+                          int jb, int* ipiv, int row_offset) {
+  if(nbmin < 1) nbmin = 1;
+  if(nbdiv < 2) nbdiv = 2;
+  if(jb > kMaxNB || jb > m) return hipErrorInvalidValue;
+  if(m > max_leaf_rows(c)) return hipErrorInvalidValue;
+  return rpanel(c, nbmin, nbdiv, P, lda, m, jb, 0, jb, ipiv, row_offset,
+                row_offset);
+}
+
 }  // namespace
 
 hipError_t leaf_occupancy(int* blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, leaf_kernel_ptr<LeafKind::kF64>(), kLeafThreads, 0);
+  // the smaller of the FP64 and FP32 leaves' occupancies
+  int        b64 = 0, b32 = 0;
+  hipError_t e   = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &b64, leaf_kernel_ptr<LeafKind::kF64>(), kLeafThreads, 0);
+  if(e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &b32, leaf_kernel_ptr<LeafKind::kF32>(), kLeafThreads, 0);
+  if(e != hipSuccess) return e;
+  *blocks_per_cu = b64 < b32 ? b64 : b32;
+  return hipSuccess;
 }
 
 hipError_t reset_panel_state(Ctx& c) {
@@ -326,12 +368,12 @@ hipError_t reset_panel_state(Ctx& c) {
 
 hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, double* P, int lda,
                         int m, int jb, int* ipiv, int row_offset) {
-  if(nbmin < 1) nbmin = 1;
-  if(nbdiv < 2) nbdiv = 2;
-  if(jb > kMaxNB || jb > m) return hipErrorInvalidValue;
-  if(m > max_leaf_rows(c)) return hipErrorInvalidValue;
-  return rpanel(c, nbmin, nbdiv, P, lda, m, jb, 0, jb, ipiv, row_offset,
-                row_offset);
+  return panel_factor_t(c, nbmin, nbdiv, P, lda, m, jb, ipiv, row_offset);
+}
+
+hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, float* P, int lda,
+                        int m, int jb, int* ipiv, int row_offset) {
+  return panel_factor_t(c, nbmin, nbdiv, P, lda, m, jb, ipiv, row_offset);
 }
 
 hipError_t launch_pipid(hipStream_t s, int jb, const int* ipiv, int ipiv_base,
@@ -343,13 +385,29 @@ hipError_t launch_pipid(hipStream_t s, int jb, const int* ipiv, int ipiv_base,
   return hipGetLastError();
 }
 
-hipError_t launch_swap_gather(hipStream_t s, int jb, int n, double* A, int lda,
-                              const int* permU, double* W, int ldw) {
+template <typename S, typename WT>
+hipError_t swap_gather_t(hipStream_t s, int jb, int n, S* A, int lda,
+                         const int* permU, WT* W, int ldw) {
   if(jb <= 0 || n <= 0) return hipSuccess;
   if(jb > kMaxNB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(swap_gather_kernel, dim3(n), dim3(256), 0, s, jb, A, lda,
-                     permU, W, ldw);
+  hipLaunchKernelGGL((swap_gather_kernel<S, WT>), dim3(n), dim3(256), 0, s, jb,
+                     A, lda, permU, W, ldw);
   return hipGetLastError();
+}
+
+hipError_t launch_swap_gather(hipStream_t s, int jb, int n, double* A, int lda,
+                              const int* permU, double* W, int ldw) {
+  return swap_gather_t(s, jb, n, A, lda, permU, W, ldw);
+}
+
+hipError_t launch_swap_gather(hipStream_t s, int jb, int n, float* A, int lda,
+                              const int* permU, float* W, int ldw) {
+  return swap_gather_t(s, jb, n, A, lda, permU, W, ldw);
+}
+
+hipError_t launch_swap_gather(hipStream_t s, int jb, int n, __half* A,
+                              int lda, const int* permU, float* W, int ldw) {
+  return swap_gather_t(s, jb, n, A, lda, permU, W, ldw);
 }
 
 hipError_t launch_swap_inverse(hipStream_t s, int jb, int n, double* A,

@@ -13,30 +13,41 @@
 // of op(T) in registers (one row per lane) and x broadcast by lane shuffles;
 // then every thread applies the chunk's 64 columns to the rows still to be
 // solved.  One workgroup barrier per 64 rows.
-#include "hplx_internal.hpp"
+#include "mxp.hpp"
 
 namespace hplx {
 namespace {
 
-template <bool TRANS>
-__device__ inline double op_at(const double* T, int ldt, int i, int j) {
-  return TRANS ? T[j + static_cast<size_t>(i) * ldt]
-               : T[i + static_cast<size_t>(j) * ldt];
+template <typename V, typename MT>
+__device__ inline V to_v(MT x) {
+  if constexpr(sizeof(MT) == 2)
+    return static_cast<V>(__half2float(x));
+  else
+    return static_cast<V>(x);
+}
+
+template <bool TRANS, typename V, typename MT>
+__device__ inline V op_at(const MT* T, int ldt, int i, int j) {
+  return to_v<V>(TRANS ? T[j + static_cast<size_t>(i) * ldt]
+                       : T[i + static_cast<size_t>(j) * ldt]);
 }
 
 // LOWER: op(T) is lower triangular (forward sweep), else upper (backward).
 // UNIT: unit diagonal (not read).  IDENTITY: b is column `col` of I.
-template <bool LOWER, bool TRANS, bool UNIT, bool IDENTITY>
+// V: element type of X and of the arithmetic (double, or float for the FP32
+// panel); MT: element type of T (V, or FP32 / FP16 factors applied to FP64
+// vectors in the mixed-precision solves).
+template <typename MT, typename V, bool LOWER, bool TRANS, bool UNIT,
+          bool IDENTITY>
 __global__ __launch_bounds__(256) void tri_solve_kernel(
-    int k, const double* __restrict__ T, int ldt, double* __restrict__ X,
-    int ldx) {
-  __shared__ double sx[kMaxNB];
+    int k, const MT* __restrict__ T, int ldt, V* __restrict__ X, int ldx) {
+  __shared__ V sx[kMaxNB];
   const int col  = blockIdx.x;
   const int t    = threadIdx.x;
   const int lane = t & 63;
-  double*   xc   = X + static_cast<size_t>(col) * ldx;
+  V*        xc   = X + static_cast<size_t>(col) * ldx;
   for(int i = t; i < k; i += blockDim.x)
-    sx[i] = IDENTITY ? (i == col ? 1.0 : 0.0) : xc[i];
+    sx[i] = IDENTITY ? (i == col ? V(1) : V(0)) : xc[i];
   __syncthreads();
   const int nchunk = (k + 63) / 64;
   // with the identity and a forward sweep, rows above `col` stay zero
@@ -46,20 +57,20 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(
     const int w  = min(64, k - c0);
     if(t < 64) {
       const int i  = c0 + lane;
-      double    xi = lane < w ? sx[i] : 0.0;
-      const double d = (UNIT || lane >= w) ? 1.0 : op_at<TRANS>(T, ldt, i, i);
-      double    e[64];
+      V         xi = lane < w ? sx[i] : V(0);
+      const V   d  = (UNIT || lane >= w) ? V(1) : op_at<TRANS, V>(T, ldt, i, i);
+      V         e[64];
 #pragma unroll
       for(int j = 0; j < 64; ++j) {
         const bool in = j < w && lane < w && (LOWER ? j < lane : j > lane);
-        e[j]          = in ? op_at<TRANS>(T, ldt, i, c0 + j) : 0.0;
+        e[j]          = in ? op_at<TRANS, V>(T, ldt, i, c0 + j) : V(0);
       }
       if(LOWER) {
 #pragma unroll
         for(int j = 0; j < 64; ++j) {
           if(j < w) {
             if(!UNIT && lane == j) xi = xi / d;
-            const double xj = __shfl(xi, j);
+            const V xj = __shfl(xi, j);
             if(lane > j) xi = fma(-e[j], xj, xi);
           }
         }
@@ -68,7 +79,7 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(
         for(int j = 63; j >= 0; --j) {
           if(j < w) {
             if(!UNIT && lane == j) xi = xi / d;
-            const double xj = __shfl(xi, j);
+            const V xj = __shfl(xi, j);
             if(lane < j) xi = fma(-e[j], xj, xi);
           }
         }
@@ -79,9 +90,9 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(
     const int r0 = LOWER ? c0 + w : 0;
     const int r1 = LOWER ? k : c0;
     for(int r = r0 + t; r < r1; r += blockDim.x) {
-      double acc = 0.0;
+      V acc = V(0);
       for(int j = 0; j < w; ++j)
-        acc = fma(op_at<TRANS>(T, ldt, r, c0 + j), sx[c0 + j], acc);
+        acc = fma(op_at<TRANS, V>(T, ldt, r, c0 + j), sx[c0 + j], acc);
       sx[r] -= acc;
     }
     __syncthreads();
@@ -89,38 +100,90 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(
   for(int i = t; i < k; i += blockDim.x) xc[i] = sx[i];
 }
 
-}  // namespace
-
-hipError_t launch_tri_solve(hipStream_t s, bool lower, bool trans, bool unit,
-                            int k, int nrhs, const double* T, int ldt,
-                            double* X, int ldx) {
+template <typename V>
+hipError_t tri_solve_t(hipStream_t s, bool lower, bool trans, bool unit, int k,
+                       int nrhs, const V* T, int ldt, V* X, int ldx) {
   if(k <= 0 || nrhs <= 0) return hipSuccess;
   if(k > kMaxNB) return hipErrorInvalidValue;
   const dim3 g(nrhs), b(256);
   if(lower && !trans && unit)
-    hipLaunchKernelGGL((tri_solve_kernel<true, false, true, false>), g, b, 0, s, k, T, ldt, X, ldx);
+    hipLaunchKernelGGL((tri_solve_kernel<V, V, true, false, true, false>), g, b, 0, s, k, T, ldt, X, ldx);
   else if(!lower && !trans && !unit)
-    hipLaunchKernelGGL((tri_solve_kernel<false, false, false, false>), g, b, 0, s, k, T, ldt, X, ldx);
+    hipLaunchKernelGGL((tri_solve_kernel<V, V, false, false, false, false>), g, b, 0, s, k, T, ldt, X, ldx);
   else if(lower && trans && !unit)  // U^T: U stored upper
-    hipLaunchKernelGGL((tri_solve_kernel<true, true, false, false>), g, b, 0, s, k, T, ldt, X, ldx);
+    hipLaunchKernelGGL((tri_solve_kernel<V, V, true, true, false, false>), g, b, 0, s, k, T, ldt, X, ldx);
   else if(!lower && trans && unit)  // L^T: L stored unit lower
-    hipLaunchKernelGGL((tri_solve_kernel<false, true, true, false>), g, b, 0, s, k, T, ldt, X, ldx);
+    hipLaunchKernelGGL((tri_solve_kernel<V, V, false, true, true, false>), g, b, 0, s, k, T, ldt, X, ldx);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_linv(hipStream_t s, int jb, const double* L, int ldl,
-                       double* Linv, int ldi) {
+template <typename V>
+hipError_t linv_t(hipStream_t s, int jb, const V* L, int ldl, V* Linv,
+                  int ldi) {
   if(jb <= 0) return hipSuccess;
   if(jb > kMaxNB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((tri_solve_kernel<true, false, true, true>), dim3(jb),
+  hipLaunchKernelGGL((tri_solve_kernel<V, V, true, false, true, true>), dim3(jb),
                      dim3(256), 0, s, jb, L, ldl, Linv, ldi);
   return hipGetLastError();
 }
 
+}  // namespace
+
+hipError_t launch_tri_solve(hipStream_t s, bool lower, bool trans, bool unit,
+                            int k, int nrhs, const double* T, int ldt,
+                            double* X, int ldx) {
+  return tri_solve_t(s, lower, trans, unit, k, nrhs, T, ldt, X, ldx);
+}
+
+hipError_t launch_tri_solve(hipStream_t s, bool lower, bool trans, bool unit,
+                            int k, int nrhs, const float* T, int ldt, float* X,
+                            int ldx) {
+  return tri_solve_t(s, lower, trans, unit, k, nrhs, T, ldt, X, ldx);
+}
+
+hipError_t launch_linv(hipStream_t s, int jb, const double* L, int ldl,
+                       double* Linv, int ldi) {
+  return linv_t(s, jb, L, ldl, Linv, ldi);
+}
+
+hipError_t launch_linv(hipStream_t s, int jb, const float* L, int ldl,
+                       float* Linv, int ldi) {
+  return linv_t(s, jb, L, ldl, Linv, ldi);
+}
+
+template <typename MT>
+hipError_t tri_solve_vec_t(hipStream_t s, bool lower, int k, const MT* T,
+                           int ldt, double* x) {
+  if(k <= 0) return hipSuccess;
+  if(k > kMaxNB) return hipErrorInvalidValue;
+  if(lower)
+    hipLaunchKernelGGL((tri_solve_kernel<MT, double, true, false, true, false>),
+                       dim3(1), dim3(256), 0, s, k, T, ldt, x, k);
+  else
+    hipLaunchKernelGGL((tri_solve_kernel<MT, double, false, false, false, false>),
+                       dim3(1), dim3(256), 0, s, k, T, ldt, x, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
+                                const float* T, int ldt, double* x) {
+  return tri_solve_vec_t(s, lower, k, T, ldt, x);
+}
+
+hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
+                                const __half* T, int ldt, double* x) {
+  return tri_solve_vec_t(s, lower, k, T, ldt, x);
+}
+
 hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const double* L,
                                 int ldl, double* B, int ldb) {
+  return launch_tri_solve(s, true, false, true, k, n, L, ldl, B, ldb);
+}
+
+hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const float* L,
+                                int ldl, float* B, int ldb) {
   return launch_tri_solve(s, true, false, true, k, n, L, ldl, B, ldb);
 }
 

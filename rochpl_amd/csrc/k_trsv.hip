@@ -8,15 +8,25 @@
 // with the block's columns split across the 4 waves of a workgroup so that
 // enough loads are in flight.  Every element of the upper triangle is read
 // exactly once.
-#include "hplx_internal.hpp"
+#include "mxp.hpp"
 
 namespace hplx {
 namespace {
 
-// y[0:m] -= U[0:m, 0:kb] * xs[0:kb].  Workgroup = 64 rows; wave w takes a
-// quarter of the columns; partial sums meet in LDS.
+template <typename MT>
+__device__ inline double to_d(MT x) {
+  if constexpr(sizeof(MT) == 2)
+    return static_cast<double>(__half2float(x));
+  else
+    return static_cast<double>(x);
+}
+
+// y[0:m] -= U[0:m, 0:kb] * xs[0:kb] in FP64, U of element type MT (double, or
+// the FP32 / FP16 factors of the mixed-precision solves).  Workgroup = 64
+// rows; wave w takes a quarter of the columns; partial sums meet in LDS.
+template <typename MT>
 __global__ __launch_bounds__(256) void gemv_sub_kernel(
-    int m, int kb, const double* __restrict__ U, int ldu,
+    int m, int kb, const MT* __restrict__ U, int ldu,
     const double* __restrict__ xs, double* __restrict__ y) {
   __shared__ double sxs[kMaxNB];
   __shared__ double part[4][64];
@@ -29,13 +39,13 @@ __global__ __launch_bounds__(256) void gemv_sub_kernel(
   const int c1  = min(kb, c0 + ck);
   double    acc = 0.0;
   if(row < m) {
-    const double* u = U + row + static_cast<size_t>(c0) * ldu;
+    const MT* u = U + row + static_cast<size_t>(c0) * ldu;
     int c = c0;
     for(; c + 4 <= c1; c += 4) {
-      const double u0 = u[0];
-      const double u1 = u[static_cast<size_t>(ldu)];
-      const double u2 = u[2 * static_cast<size_t>(ldu)];
-      const double u3 = u[3 * static_cast<size_t>(ldu)];
+      const double u0 = to_d(u[0]);
+      const double u1 = to_d(u[static_cast<size_t>(ldu)]);
+      const double u2 = to_d(u[2 * static_cast<size_t>(ldu)]);
+      const double u3 = to_d(u[3 * static_cast<size_t>(ldu)]);
       acc = fma(u0, sxs[c], acc);
       acc = fma(u1, sxs[c + 1], acc);
       acc = fma(u2, sxs[c + 2], acc);
@@ -43,7 +53,7 @@ __global__ __launch_bounds__(256) void gemv_sub_kernel(
       u += 4 * static_cast<size_t>(ldu);
     }
     for(; c < c1; ++c) {
-      acc = fma(u[0], sxs[c], acc);
+      acc = fma(to_d(u[0]), sxs[c], acc);
       u += ldu;
     }
   }
@@ -53,7 +63,32 @@ __global__ __launch_bounds__(256) void gemv_sub_kernel(
     y[row] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
+template <typename MT>
+hipError_t gemv_sub_t(hipStream_t s, int m, int k, const MT* M, int ldm,
+                      const double* x, double* y) {
+  if(m <= 0 || k <= 0) return hipSuccess;
+  if(k > kMaxNB) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((gemv_sub_kernel<MT>), dim3((m + 63) / 64), dim3(256), 0,
+                     s, m, k, M, ldm, x, y);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const float* M,
+                           int ldm, const double* x, double* y) {
+  return gemv_sub_t(s, m, k, M, ldm, x, y);
+}
+
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const __half* M,
+                           int ldm, const double* x, double* y) {
+  return gemv_sub_t(s, m, k, M, ldm, x, y);
+}
+
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const double* M,
+                           int ldm, const double* x, double* y) {
+  return gemv_sub_t(s, m, k, M, ldm, x, y);
+}
 
 hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                        double* x) {
@@ -68,9 +103,9 @@ hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                                     lda, x + s0, n);
     if(e != hipSuccess) return e;
     if(s0 > 0)
-      hipLaunchKernelGGL(gemv_sub_kernel, dim3((s0 + 63) / 64), dim3(256), 0,
-                         s, s0, kb, A + static_cast<size_t>(s0) * lda, lda,
-                         x + s0, x);
+      hipLaunchKernelGGL(gemv_sub_kernel<double>, dim3((s0 + 63) / 64),
+                         dim3(256), 0, s, s0, kb,
+                         A + static_cast<size_t>(s0) * lda, lda, x + s0, x);
   }
   return hipGetLastError();
 }

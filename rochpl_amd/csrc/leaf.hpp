@@ -1,8 +1,8 @@
 // Register-resident cooperative leaf of the panel factorisation
 // (HPL_pdpanrlT, src/pfact/HPL_pdpanrlT_device.cpp:23-468): one launch per
 // leaf of <= 32 columns.  Instantiated in k_leaf*.hip for each (register
-// type T, storage type S) pair: FP64 for the HPL solve, FP32 registers over
-// FP32 or FP16 storage for the mixed-precision factorisations.
+// type T, storage type S) pair: FP64 for the HPL solve, FP32 for the panels
+// of the mixed-precision factorisations.
 //
 // Every workgroup keeps its slice of the leaf (up to 512*RPT rows x 32
 // columns) in registers for the whole leaf.  Per column:
@@ -28,8 +28,6 @@
 // all workgroups leave the kernel, so a grid that is not co-resident cannot
 // hang the device.
 #pragma once
-
-#include <hip/hip_fp16.h>
 
 #include <climits>
 
@@ -96,23 +94,14 @@ struct LeafShared {
   int    p;
 };
 
-template <typename S>
-__device__ inline float to_f(S x);
-template <>
-__device__ inline float to_f<float>(float x) { return x; }
-template <>
-__device__ inline float to_f<__half>(__half x) { return __half2float(x); }
-
 template <typename T, typename S>
 __device__ inline T load_s(const S* p) {
-  if constexpr(sizeof(T) == sizeof(S)) return *p;
-  else return static_cast<T>(to_f<S>(*p));
+  return static_cast<T>(*p);
 }
 
 template <typename T, typename S>
 __device__ inline void store_s(S* p, T v) {
-  if constexpr(sizeof(T) == sizeof(S)) *p = v;
-  else *p = __float2half(static_cast<float>(v));
+  *p = static_cast<S>(v);
 }
 
 // Workgroup max-loc of (v, r); every thread gets the result.
@@ -430,9 +419,8 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
 
 }  // namespace leaf
 
-// Kernel entry of one variant (defined in k_leaf*.hip): FP64, or FP32
-// registers over FP32 / FP16 storage.
-enum class LeafKind { kF64, kF32, kF32H };
+// Kernel entry of one variant (defined in k_leaf*.hip): FP64, or FP32.
+enum class LeafKind { kF64, kF32 };
 template <LeafKind K>
 const void* leaf_kernel_ptr();
 

@@ -1,0 +1,84 @@
+// Internal declarations of the mixed-precision factorisation and solve
+// (FP32 or FP16 factor storage, BF16 / FP8 trailing update, FP64 iterative
+// refinement).
+#pragma once
+
+#include "hplx_internal.hpp"
+
+namespace hplx {
+
+// Packed operands are padded to whole multiples of this many rows/columns
+// (the largest GEMM tile edge).
+constexpr int kMxpRowPad = 256;
+
+// One panel's packed update operands (see k_mxp_gemm.hip for the layout).
+struct MxpPacked {
+  void*  a      = nullptr;  // L21, m_pad rows
+  void*  b      = nullptr;  // U12, n_pad columns
+  int*   ea     = nullptr;  // FP8: exponent per (128-row tile, 128-deep stage)
+  int*   eb     = nullptr;  // FP8: exponent per (stage, 128-column tile)
+  int    m_pad  = 0;
+  int    n_pad  = 0;
+  size_t aplane = 0;  // BF16: elements per split term
+  size_t bplane = 0;
+};
+
+// Bytes of one packed operand of `rows` rows (or columns) and depth K.
+size_t mxp_pack_bytes(int update, int terms, int rows, int K);
+// FP8 scale exponents of one packed operand.
+size_t mxp_scale_count(int rows, int K);
+
+// Pack L (m x K, column-major FP32) and U (K x n, column-major FP32); sets
+// P.m_pad / n_pad / planes.  P.a, P.b (and P.ea, P.eb for FP8) must be sized
+// by mxp_pack_bytes / mxp_scale_count.
+hipError_t launch_mxp_pack(hipStream_t s, int update, int terms, int m, int K,
+                           const float* L, int ldl, int n, const float* U,
+                           int ldu, MxpPacked& P);
+// C (m x n, storage type) -= L * U from the packed operands.
+hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
+                           int K, const MxpPacked& P, float* C, int ldc);
+hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
+                           int K, const MxpPacked& P, __half* C, int ldc);
+
+// Storage <-> FP32 copies of an m x n block (panel staging of FP16 storage,
+// U rows back into storage).
+hipError_t launch_copy_block(hipStream_t s, int m, int n, const __half* A,
+                             int lda, float* B, int ldb);
+hipError_t launch_copy_block(hipStream_t s, int m, int n, const float* A,
+                             int lda, __half* B, int ldb);
+hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
+                             int lda, float* B, int ldb);
+hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
+                             int lda, __half* B, int ldb);
+
+// ---- solves with low-precision factors and FP64 vectors
+// y[k0 : k0+jb] <-> y[ipiv[i]] for i = k0 .. k0+jb-1 in order.
+hipError_t launch_vec_swaps(hipStream_t s, int k0, int jb, const int* ipiv,
+                            double* y);
+// op(T) x = x for one FP64 vector, T in storage type (k <= kMaxNB):
+// lower unit (forward) or upper non-unit (backward).
+hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
+                                const float* T, int ldt, double* x);
+hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
+                                const __half* T, int ldt, double* x);
+// y[0:m] -= M[0:m, 0:k] * x[0:k], M in storage type, FP64 accumulation.
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const float* M,
+                           int ldm, const double* x, double* y);
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const __half* M,
+                           int ldm, const double* x, double* y);
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const double* M,
+                           int ldm, const double* x, double* y);
+
+// ---- FP64 refinement
+// b (the generator's column n) into y.
+hipError_t launch_gen_b(hipStream_t s, int n, int seed, double* y);
+// r = b - A x with A regenerated in FP64; out[0] = scaled residual,
+// out[1] = ||r||_oo, out[2] = ||A||_oo, out[3] = ||x||_oo, out[4] = ||b||_oo.
+size_t     ir_part_len(int n);
+hipError_t launch_ir_residual(hipStream_t s, int n, int seed,
+                              double diag_shift, const double* x, double* r,
+                              double* part, double* out);
+// x += d
+hipError_t launch_axpy(hipStream_t s, int n, const double* d, double* x);
+
+}  // namespace hplx
