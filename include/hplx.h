@@ -219,6 +219,113 @@ int hplx_pdtrsv(hplx_ctx* ctx, int n, int nb, const double* A, int lda,
 int hplx_update_stats(hplx_ctx* ctx, double* gemm_ms, int* launches,
                       double* gemm_flops);
 
+/* ======================================================================
+ * Mixed precision (no counterpart in the reference, which is FP64 only).
+ *
+ * The same right-looking recursive LU with partial pivoting as hplx_pdgesv
+ * (same panel recursion, leaves, pivot rule and pivot composition), with
+ *   - the factors stored in FP32 or FP16 (storage),
+ *   - every panel factorised in FP32 (FP16-stored panels are staged in an
+ *     FP32 copy),
+ *   - U12 = L11^{-1} P A12 in FP32,
+ *   - the trailing update A22 -= L21 U12 in BF16 (optionally split into 2 or
+ *     3 BF16 terms per operand) or in per-128x128-tile power-of-two-scaled
+ *     FP8 e4m3, on MFMA with FP32 accumulation,
+ * followed by FP64 iterative refinement: r = b - A x with A and b
+ * regenerated tile by tile in FP64 from the generator's seed (never stored),
+ * the correction solved with the low-precision factors on FP64 vectors, and
+ * x += d, until the HPL scaled residual is below the threshold.
+ * ==================================================================== */
+
+/* storage of the factors */
+#define HPLX_STORE_FP32 32
+#define HPLX_STORE_FP16 16
+/* arithmetic of the trailing update */
+#define HPLX_UPDATE_FP32 0 /* FP32 MFMA (reference form for tests)          */
+#define HPLX_UPDATE_BF16 1 /* BF16 MFMA, `terms` BF16 numbers per operand     */
+#define HPLX_UPDATE_FP8 2  /* e4m3 MFMA, power-of-two scale per 128x128 tile */
+
+typedef struct hplx_mxp_opts {
+  int    update;    /* HPLX_UPDATE_*                                        */
+  int    terms;     /* BF16: 1 (plain), 2 (3 products) or 3 (6 products)  */
+  int    max_its;   /* refinement solves allowed after the first (<= 64)    */
+  int    reserved;  /* 0                                                    */
+  double threshold; /* target HPL scaled residual (16.0)                    */
+} hplx_mxp_opts;
+
+typedef struct hplx_mxp_pmat {
+  void*   A;          /* ld x n factors, storage type, column-major          */
+  double* X;          /* solution (n)                                        */
+  int*    ipiv;       /* pivots (n), as hplx_pmat                            */
+  int     n, nb, ld;
+  int     storage;    /* HPLX_STORE_*                                        */
+  int     seed;       /* generator of A and b (set by hplx_mxp_matgen)       */
+  int     info;       /* first exact zero pivot (1-based) or 0               */
+  double  diag_shift; /* A = generator + diag_shift * I                      */
+  void*   ws;         /* library workspace                                   */
+} hplx_mxp_pmat;
+
+typedef struct hplx_mxp_result {
+  int    its;          /* refinement corrections applied                    */
+  int    converged;    /* 1 when resid < threshold                          */
+  double resid;        /* final scaled residual (true ||r||_oo)             */
+  double factor_ms;    /* device time of the factorisation                   */
+  double solve_ms;     /* first solve + refinement                           */
+  double update_ms;    /* device time of the low-precision update GEMMs      */
+  double update_flops; /* their FP64-equivalent flops (2 m n k per launch)   */
+  int    update_launches;
+  int    pad;
+  double history[65];  /* scaled residual after each solve (its + 1 values) */
+} hplx_mxp_result;
+
+void hplx_mxp_opts_default(hplx_mxp_opts* opts);
+
+/* Allocate the n x n factor storage and every workspace of the mixed solve
+ * (nb <= 1024). */
+int hplx_mxp_alloc(hplx_ctx* ctx, int n, int nb, int storage,
+                   hplx_mxp_pmat* mat);
+int hplx_mxp_free(hplx_ctx* ctx, hplx_mxp_pmat* mat);
+
+/* A = generator(seed) + diag_shift * I, generated in FP64 (the
+ * HPL_pdrandmat values) and rounded to the storage type. */
+int hplx_mxp_matgen(hplx_ctx* ctx, hplx_mxp_pmat* mat, int seed,
+                    double diag_shift);
+
+/* Factor mat->A in place (no refinement).  Returns 0, an error, or a
+ * positive zero-pivot column. */
+int hplx_mxp_pdgetrf(hplx_ctx* ctx, const hplx_palg* palg,
+                     const hplx_mxp_opts* opts, hplx_mxp_pmat* mat);
+
+/* x := A^{-1} x with the stored low-precision factors, FP64 vector x
+ * (device, n). */
+int hplx_mxp_pdgetrs(hplx_ctx* ctx, const hplx_mxp_pmat* mat, double* x);
+
+/* The mixed-precision counterpart of hplx_pdgesv: factor, solve for the
+ * generator's b, refine in FP64 until the scaled residual is below
+ * opts->threshold or opts->max_its corrections were applied.  mat->X holds
+ * x; res (may be NULL) the refinement record.  Returns 0 when converged,
+ * HPLX_ERR_UNSUPPORTED-style errors on bad input, a positive zero-pivot
+ * column, or 1 + n when refinement did not converge. */
+int hplx_mxp_pdgesv(hplx_ctx* ctx, const hplx_palg* palg,
+                    const hplx_mxp_opts* opts, hplx_mxp_pmat* mat,
+                    hplx_mxp_result* res);
+
+/* The refinement residual of mat->X: out[5] = scaled residual,
+ * ||b - A x||_oo, ||A||_oo, ||x||_oo, ||b||_oo (A, b regenerated in FP64). */
+int hplx_mxp_residual(hplx_ctx* ctx, const hplx_mxp_pmat* mat,
+                      const double* x, double* out);
+
+/* Components (parity tests).
+ * The low-precision trailing update: C (m x n, storage type, ldc) -= L U with
+ * L (m x k) and U (k x n) FP32 column-major, rounded to the update
+ * arithmetic exactly as the factorisation rounds them. */
+int hplx_mxp_update(hplx_ctx* ctx, int update, int terms, int storage, int m,
+                    int n, int k, const float* L, int ldl, const float* U,
+                    int ldu, void* C, int ldc);
+/* The FP32 panel factorisation (hplx_pdfact in FP32). */
+int hplx_spdfact(hplx_ctx* ctx, int nbmin, int nbdiv, float* P, int lda,
+                 int m, int jb, int* ipiv, int row_offset);
+
 #ifdef __cplusplus
 }
 #endif

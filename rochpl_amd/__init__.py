@@ -30,6 +30,9 @@ HPLX_LEFT_LOOKING, HPLX_CROUT, HPLX_RIGHT_LOOKING = 301, 302, 303
 HPLX_1RING, HPLX_1RING_M, HPLX_2RING, HPLX_2RING_M, HPLX_BLONG, \
     HPLX_BLONG_M = 401, 402, 403, 404, 405, 406
 HPLX_SWAP00, HPLX_SWAP01, HPLX_SW_MIX, HPLX_NO_SWP = 451, 452, 453, 499
+# mixed precision (no counterpart in the reference)
+HPLX_STORE_FP32, HPLX_STORE_FP16 = 32, 16
+HPLX_UPDATE_FP32, HPLX_UPDATE_BF16, HPLX_UPDATE_FP8 = 0, 1, 2
 
 # Every symbol include/hplx.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -40,6 +43,9 @@ EXPORTS = [
     "hplx_pdtest_check", "hplx_pdfact", "hplx_pipid", "hplx_dlaswp00N",
     "hplx_pdupdate", "hplx_dgemm", "hplx_pdtrsv", "hplx_update_stats",
     "hplx_pdgetrf", "hplx_pdgetrs", "hplx_pdgesv_adjoint",
+    "hplx_mxp_opts_default", "hplx_mxp_alloc", "hplx_mxp_free",
+    "hplx_mxp_matgen", "hplx_mxp_pdgetrf", "hplx_mxp_pdgetrs",
+    "hplx_mxp_pdgesv", "hplx_mxp_residual", "hplx_mxp_update", "hplx_spdfact",
 ]
 
 
@@ -67,6 +73,39 @@ class PMat(ctypes.Structure):
                 ("nb", ctypes.c_int), ("ld", ctypes.c_int),
                 ("mp", ctypes.c_int), ("nq", ctypes.c_int),
                 ("info", ctypes.c_int), ("ws", ctypes.c_void_p)]
+
+
+class MxpOpts(ctypes.Structure):
+    """hplx_mxp_opts (mixed-precision options)."""
+    _fields_ = [("update", ctypes.c_int), ("terms", ctypes.c_int),
+                ("max_its", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("threshold", ctypes.c_double)]
+
+
+class MxpPMat(ctypes.Structure):
+    """hplx_mxp_pmat: low-precision factors, FP64 solution, pivots."""
+    _fields_ = [("A", ctypes.c_void_p), ("X", ctypes.c_void_p),
+                ("ipiv", ctypes.c_void_p), ("n", ctypes.c_int),
+                ("nb", ctypes.c_int), ("ld", ctypes.c_int),
+                ("storage", ctypes.c_int), ("seed", ctypes.c_int),
+                ("info", ctypes.c_int), ("diag_shift", ctypes.c_double),
+                ("ws", ctypes.c_void_p)]
+
+
+class MxpResult(ctypes.Structure):
+    """hplx_mxp_result: the refinement record."""
+    _fields_ = [("its", ctypes.c_int), ("converged", ctypes.c_int),
+                ("resid", ctypes.c_double), ("factor_ms", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("update_ms", ctypes.c_double),
+                ("update_flops", ctypes.c_double),
+                ("update_launches", ctypes.c_int), ("pad", ctypes.c_int),
+                ("history", ctypes.c_double * 65)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_
+             if k not in ("history", "pad")}
+        d["history"] = list(self.history[:self.its + 1])
+        return d
 
 
 _lib = None
@@ -123,6 +162,18 @@ def lib():
         "hplx_pdgetrs": (i32, [vp, P(PMat), i32, i32, vp, i32]),
         "hplx_pdgesv_adjoint": (i32, [vp, P(PMat), i32, vp, i32, vp, i32, vp,
                                       i32, vp, i32]),
+        "hplx_mxp_opts_default": (None, [P(MxpOpts)]),
+        "hplx_mxp_alloc": (i32, [vp, i32, i32, i32, P(MxpPMat)]),
+        "hplx_mxp_free": (i32, [vp, P(MxpPMat)]),
+        "hplx_mxp_matgen": (i32, [vp, P(MxpPMat), i32, f64]),
+        "hplx_mxp_pdgetrf": (i32, [vp, P(PAlg), P(MxpOpts), P(MxpPMat)]),
+        "hplx_mxp_pdgetrs": (i32, [vp, P(MxpPMat), vp]),
+        "hplx_mxp_pdgesv": (i32, [vp, P(PAlg), P(MxpOpts), P(MxpPMat),
+                                  P(MxpResult)]),
+        "hplx_mxp_residual": (i32, [vp, P(MxpPMat), vp, P(f64)]),
+        "hplx_mxp_update": (i32, [vp, i32, i32, i32, i32, i32, i32, vp, i32,
+                                  vp, i32, vp, i32]),
+        "hplx_spdfact": (i32, [vp, i32, i32, vp, i32, i32, i32, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -136,6 +187,17 @@ def palg_default() -> PAlg:
     p = PAlg()
     lib().hplx_palg_default(ctypes.byref(p))
     return p
+
+
+def mxp_opts_default(update: int | None = None,
+                     terms: int | None = None) -> MxpOpts:
+    o = MxpOpts()
+    lib().hplx_mxp_opts_default(ctypes.byref(o))
+    if update is not None:
+        o.update = update
+    if terms is not None:
+        o.terms = terms
+    return o
 
 
 def hpl_ld(n: int) -> int:
@@ -309,6 +371,58 @@ class Context:
         self._chk(lib().hplx_update_stats(self.h, ctypes.byref(ms),
                                           ctypes.byref(cnt), ctypes.byref(fl)))
         return ms.value, cnt.value, fl.value
+
+    # ---- mixed precision (FP32/FP16 factors, BF16/FP8 update, FP64 IR) --------
+    def mxp_alloc(self, n: int, nb: int,
+                  storage: int = HPLX_STORE_FP32) -> MxpPMat:
+        m = MxpPMat()
+        self._chk(lib().hplx_mxp_alloc(self.h, n, nb, storage, ctypes.byref(m)))
+        return m
+
+    def mxp_free(self, mat: MxpPMat) -> None:
+        self._chk(lib().hplx_mxp_free(self.h, ctypes.byref(mat)))
+
+    def mxp_matgen(self, mat: MxpPMat, seed: int = HPL_ISEED,
+                   diag_shift: float = 0.0) -> None:
+        self._chk(lib().hplx_mxp_matgen(self.h, ctypes.byref(mat), seed,
+                                        float(diag_shift)))
+
+    def mxp_pdgetrf(self, palg: PAlg, opts: MxpOpts, mat: MxpPMat) -> int:
+        return self._chk(lib().hplx_mxp_pdgetrf(
+            self.h, ctypes.byref(palg), ctypes.byref(opts), ctypes.byref(mat)))
+
+    def mxp_pdgetrs(self, mat: MxpPMat, x: int) -> None:
+        self._chk(lib().hplx_mxp_pdgetrs(self.h, ctypes.byref(mat),
+                                         ctypes.c_void_p(x)))
+
+    def mxp_pdgesv(self, palg: PAlg, opts: MxpOpts, mat: MxpPMat):
+        """Returns (rc, result dict): rc 0 when the refinement converged."""
+        res = MxpResult()
+        rc = self._chk(lib().hplx_mxp_pdgesv(
+            self.h, ctypes.byref(palg), ctypes.byref(opts), ctypes.byref(mat),
+            ctypes.byref(res)))
+        return rc, res.as_dict()
+
+    def mxp_residual(self, mat: MxpPMat, x: int) -> dict:
+        out = (ctypes.c_double * 5)()
+        self._chk(lib().hplx_mxp_residual(self.h, ctypes.byref(mat),
+                                          ctypes.c_void_p(x), out))
+        return dict(zip(["resid", "rnorm", "AnormI", "XnormI", "BnormI"],
+                        list(out)))
+
+    def mxp_update(self, update: int, terms: int, storage: int, m: int, n: int,
+                   k: int, L: int, ldl: int, U: int, ldu: int, C: int,
+                   ldc: int) -> None:
+        self._chk(lib().hplx_mxp_update(self.h, update, terms, storage, m, n,
+                                        k, ctypes.c_void_p(L), ldl,
+                                        ctypes.c_void_p(U), ldu,
+                                        ctypes.c_void_p(C), ldc))
+
+    def spdfact(self, P: int, lda: int, m: int, jb: int, ipiv: int,
+                nbmin: int = 32, nbdiv: int = 2, row_offset: int = 0) -> int:
+        return self._chk(lib().hplx_spdfact(self.h, nbmin, nbdiv,
+                                            ctypes.c_void_p(P), lda, m, jb,
+                                            ctypes.c_void_p(ipiv), row_offset))
 
     # ---- whole-matrix helpers -------------------------------------------------
     def download_matrix(self, mat: PMat, rows: int | None = None,

@@ -1,0 +1,110 @@
+// Pieces shared by the C-ABI translation units (hplx_api.hip, mxp_api.hip):
+// the context object, error reporting, palg validation and the panel-loop
+// progress lines.
+#pragma once
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+
+#include "hplx_internal.hpp"
+
+struct hplx_ctx {
+  hplx::Ctx c;
+};
+
+namespace hplx {
+
+inline int fail(Ctx& c, int code, const char* fmt, ...) {
+  char    buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  c.last_error = buf;
+  return code;
+}
+
+#define HIPCHK(C, expr)                                                    \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if(e_ != hipSuccess)                                                   \
+      return hplx::fail((C), HPLX_ERR_HIP, "%s failed: %s (%s:%d)", #expr, \
+                        hipGetErrorString(e_), __FILE__, __LINE__);        \
+  } while(0)
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+inline int use(hplx_ctx* ctx) {
+  if(!ctx) return HPLX_ERR_INVALID;
+  hipError_t e = hipSetDevice(ctx->c.device);
+  if(e != hipSuccess)
+    return fail(ctx->c, HPLX_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  return HPLX_SUCCESS;
+}
+
+// The P=Q=1 configurations this library builds (HPL_T_palg values).
+inline int check_palg(Ctx& c, const hplx_palg* p) {
+  if(!p) return fail(c, HPLX_ERR_INVALID, "null palg");
+  if(p->pfact != HPLX_RIGHT_LOOKING || p->rfact != HPLX_RIGHT_LOOKING)
+    return fail(c, HPLX_ERR_UNSUPPORTED,
+                "only right-looking panel/recursive factorisation is built "
+                "(pfact=%d rfact=%d)",
+                p->pfact, p->rfact);
+  if(p->L1notran != 0 || p->Unotran != 0)
+    return fail(c, HPLX_ERR_UNSUPPORTED, "L1 and U must be in transposed form");
+  if(p->equil != 0)
+    return fail(c, HPLX_ERR_UNSUPPORTED, "equilibration is not supported");
+  if(p->nbmin < 1 || p->nbdiv < 2)
+    return fail(c, HPLX_ERR_INVALID, "nbmin=%d nbdiv=%d", p->nbmin, p->nbdiv);
+  return HPLX_SUCCESS;
+}
+
+// Progress lines of a panel loop (HPLX_PROGRESS=<panels>, the counterpart of
+// the reference's HPL_PROGRESS_REPORT build option): reports panels the
+// device has finished, at most kLag panels behind what is queued, so the
+// host never runs further ahead than that while it reports.
+class Progress {
+ public:
+  static constexpr int kLag = 8;
+  Progress(Ctx& c, int npanels, int nb, int n)
+      : c_(c), npanels_(npanels), nb_(nb), n_(n),
+        t0_(std::chrono::steady_clock::now()) {}
+
+  hipError_t init() {
+    if(c_.progress_every <= 0) return hipSuccess;
+    while(static_cast<int>(c_.ev_prog.size()) < 2 * kLag) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+      if(r != hipSuccess) return r;
+      c_.ev_prog.push_back(e);
+    }
+    return hipSuccess;
+  }
+
+  hipError_t panel_queued(int k) {
+    if(c_.progress_every <= 0) return hipSuccess;
+    hipError_t r = hipEventRecord(c_.ev_prog[k % (2 * kLag)], c_.stream);
+    if(r != hipSuccess) return r;
+    const int done = k - kLag;
+    if(done < 0) return hipSuccess;
+    r = hipEventSynchronize(c_.ev_prog[done % (2 * kLag)]);
+    if(r != hipSuccess) return r;
+    if(done % c_.progress_every == 0) {
+      const double el =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_)
+              .count();
+      std::fprintf(stderr, "[hplx] panel %d/%d done (column %d of %d) %.2f s\n",
+                   done + 1, npanels_, (done + 1) * nb_, n_, el);
+      std::fflush(stderr);
+    }
+    return hipSuccess;
+  }
+
+ private:
+  Ctx&                                  c_;
+  int                                   npanels_, nb_, n_;
+  std::chrono::steady_clock::time_point t0_;
+};
+
+}  // namespace hplx

@@ -23,45 +23,16 @@
 #include <cstring>
 #include <new>
 
-#include "hplx_internal.hpp"
-
-struct hplx_ctx {
-  hplx::Ctx c;
-};
+#include "api_common.hpp"
 
 namespace {
 
 using hplx::Ctx;
-
-int fail(Ctx& c, int code, const char* fmt, ...) {
-  char    buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  c.last_error = buf;
-  return code;
-}
-
-#define HIPCHK(C, expr)                                                    \
-  do {                                                                     \
-    hipError_t e_ = (expr);                                                \
-    if(e_ != hipSuccess)                                                   \
-      return fail((C), HPLX_ERR_HIP, "%s failed: %s (%s:%d)", #expr,       \
-                  hipGetErrorString(e_), __FILE__, __LINE__);              \
-  } while(0)
-
-size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+using hplx::fail;
+using hplx::use;
+using hplx::align_up;
 
 int hpl_ld(int n) { return ((n > 0 ? n : 0) + 95) / 128 * 128 + 32; }
-
-int use(hplx_ctx* ctx) {
-  if(!ctx) return HPLX_ERR_INVALID;
-  hipError_t e = hipSetDevice(ctx->c.device);
-  if(e != hipSuccess)
-    return fail(ctx->c, HPLX_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
-  return HPLX_SUCCESS;
-}
 
 }  // namespace
 
@@ -289,29 +260,13 @@ int hplx_pdrandmat(hplx_ctx* ctx, int m, int n, int nb, double* A, int lda,
   return HPLX_SUCCESS;
 }
 
-static int check_palg(Ctx& c, const hplx_palg* p) {
-  if(!p) return fail(c, HPLX_ERR_INVALID, "null palg");
-  if(p->pfact != HPLX_RIGHT_LOOKING || p->rfact != HPLX_RIGHT_LOOKING)
-    return fail(c, HPLX_ERR_UNSUPPORTED,
-                "only right-looking panel/recursive factorisation is built "
-                "(pfact=%d rfact=%d)",
-                p->pfact, p->rfact);
-  if(p->L1notran != 0 || p->Unotran != 0)
-    return fail(c, HPLX_ERR_UNSUPPORTED, "L1 and U must be in transposed form");
-  if(p->equil != 0)
-    return fail(c, HPLX_ERR_UNSUPPORTED, "equilibration is not supported");
-  if(p->nbmin < 1 || p->nbdiv < 2)
-    return fail(c, HPLX_ERR_INVALID, "nbmin=%d nbdiv=%d", p->nbmin, p->nbdiv);
-  return HPLX_SUCCESS;
-}
-
 // The panel loop over the first `ncols` columns of mat->A (n+1 with b for
 // hplx_pdgesv, n for hplx_pdgetrf).  Returns 0, an error code, or the
 // 1-based column of the first exact zero pivot.
 static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
                        int ncols) {
   int r;
-  if((r = check_palg(c, palg))) return r;
+  if((r = hplx::check_palg(c, palg))) return r;
   if(!mat || !mat->A || !mat->ws) return fail(c, HPLX_ERR_INVALID, "bad pmat");
   const int N   = mat->n;
   const int nb  = mat->nb;
@@ -331,14 +286,8 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
   }
   c.ev_pairs   = 0;
   c.gemm_flops = 0.0;
-  constexpr int kProgLag = 8;  // panels queued ahead of the reported one
-  if(c.progress_every > 0)
-    while(static_cast<int>(c.ev_prog.size()) < 2 * kProgLag) {
-      hipEvent_t e;
-      HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      c.ev_prog.push_back(e);
-    }
-  const auto t_start = std::chrono::steady_clock::now();
+  hplx::Progress prog(c, npanels, nb, N);
+  HIPCHK(c, prog.init());
 
   HIPCHK(c, hplx::reset_panel_state(c));
   for(int j = 0; j < N; j += nb) {
@@ -365,25 +314,7 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
         c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
       }
     }
-    if(c.progress_every > 0) {
-      // report panels the device has finished, never more than kProgLag
-      // behind what is queued
-      const int k = j / nb;
-      HIPCHK(c, hipEventRecord(c.ev_prog[k % (2 * kProgLag)], s));
-      const int done = k - kProgLag;
-      if(done >= 0) {
-        HIPCHK(c, hipEventSynchronize(c.ev_prog[done % (2 * kProgLag)]));
-        if(done % c.progress_every == 0) {
-          const double el = std::chrono::duration<double>(
-                                std::chrono::steady_clock::now() - t_start)
-                                .count();
-          std::fprintf(stderr, "[hplx] panel %d/%d done (column %d of %d) "
-                               "%.2f s\n",
-                       done + 1, npanels, (done + 1) * nb, N, el);
-          std::fflush(stderr);
-        }
-      }
-    }
+    HIPCHK(c, prog.panel_queued(j / nb));
   }
   // zero pivot or barrier failure?  (one synchronisation)
   int flags[2] = {0, 0};

@@ -23,8 +23,19 @@ constexpr int kGenCols = 256;  // columns per block
 constexpr int kChkCols = 4096; // columns per partial-sum chunk (row pass)
 constexpr int kChkRows = 4096; // rows per partial-sum chunk (column pass)
 
+// T: stored type.  FP32 storage holds RN_fp32(v); FP16 storage holds
+// RN_fp16(RN_fp32(v)) of the FP64 generator value v.
+template <typename T>
+__device__ inline T store_as(double v) {
+  if constexpr(sizeof(T) == 2)
+    return __float2half(static_cast<float>(v));
+  else
+    return static_cast<T>(v);
+}
+
+template <typename T>
 __global__ __launch_bounds__(kGenRows) void matgen_kernel(
-    int m, int n, double* __restrict__ A, int lda, uint64_t seed, double shift,
+    int m, int n, T* __restrict__ A, int lda, uint64_t seed, double shift,
     uint64_t colA, uint64_t colC, int gx) {
   const int bx = blockIdx.x % gx;
   const int by = blockIdx.x / gx;
@@ -35,11 +46,11 @@ __global__ __launch_bounds__(kGenRows) void matgen_kernel(
   uint64_t s =
       lcg_state(seed, static_cast<uint64_t>(i) +
                           static_cast<uint64_t>(j0) * static_cast<uint64_t>(m));
-  double* col = A + static_cast<size_t>(j0) * lda + i;
+  T* col = A + static_cast<size_t>(j0) * lda + i;
   for(int j = j0; j < jend; ++j) {
     double v = lcg_value(s);
     if(i == j) v += shift;
-    *col = v;
+    *col = store_as<T>(v);
     col += lda;
     s = colA * s + colC;
   }
@@ -259,17 +270,31 @@ __global__ __launch_bounds__(256) void ir_reduce_kernel(
 
 }  // namespace
 
-hipError_t launch_matgen(hipStream_t s, int m, int n, double* A, int lda,
-                         int seed, double diag_shift) {
+template <typename T>
+hipError_t matgen_t(hipStream_t s, int m, int n, T* A, int lda, int seed,
+                    double diag_shift) {
   if(m <= 0 || n <= 0) return hipSuccess;
   const Affine colj = lcg_jump(static_cast<uint64_t>(m));
   const int    gx   = (m + kGenRows - 1) / kGenRows;
   const int    gy   = (n + kGenCols - 1) / kGenCols;
-  hipLaunchKernelGGL(matgen_kernel, dim3(static_cast<unsigned>(gx) * gy),
+  hipLaunchKernelGGL((matgen_kernel<T>), dim3(static_cast<unsigned>(gx) * gy),
                      dim3(kGenRows), 0, s, m, n, A, lda,
                      static_cast<uint64_t>(seed), diag_shift, colj.a, colj.c,
                      gx);
   return hipGetLastError();
+}
+
+hipError_t launch_matgen(hipStream_t s, int m, int n, double* A, int lda,
+                         int seed, double diag_shift) {
+  return matgen_t(s, m, n, A, lda, seed, diag_shift);
+}
+hipError_t launch_matgen(hipStream_t s, int m, int n, float* A, int lda,
+                         int seed, double diag_shift) {
+  return matgen_t(s, m, n, A, lda, seed, diag_shift);
+}
+hipError_t launch_matgen(hipStream_t s, int m, int n, __half* A, int lda,
+                         int seed, double diag_shift) {
+  return matgen_t(s, m, n, A, lda, seed, diag_shift);
 }
 
 size_t check_part_len(int n) {

@@ -69,6 +69,12 @@ hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const __half* M,
 hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const double* M,
                            int ldm, const double* x, double* y);
 
+// The generator's matrix rounded to FP32 / FP16 storage.
+hipError_t launch_matgen(hipStream_t s, int m, int n, float* A, int lda,
+                         int seed, double diag_shift);
+hipError_t launch_matgen(hipStream_t s, int m, int n, __half* A, int lda,
+                         int seed, double diag_shift);
+
 // ---- FP64 refinement
 // b (the generator's column n) into y.
 hipError_t launch_gen_b(hipStream_t s, int n, int seed, double* y);

@@ -1,0 +1,483 @@
+// C ABI of the mixed-precision factorisation with FP64 iterative refinement
+// (include/hplx.h, "Mixed precision").
+//
+// The panel loop is the FP64 loop of hplx_api.hip (src/pgesv/HPL_pdgesv.cpp
+// :168-245 at P=Q=1) with the factors in FP32 or FP16 storage S:
+//   1. panel factorisation in FP32 (in place for FP32 storage; FP16 panels
+//      are copied to an FP32 buffer, factored, and written back)
+//   2. pivot composition on the device                        -> permU
+//   3. L11^{-1} in FP32
+//   4. row swaps of the trailing columns; new top rows -> W (FP32)
+//   5. U12 = L11^{-1} W in FP32, stored back in S
+//   6. A22 -= L21 U12 in BF16 / FP8 (or FP32) on MFMA, FP32 accumulation
+// b is not carried through the loop.  The solve then runs in FP64 on the
+// device: x = U^{-1} L^{-1} P b with b generated in FP64 from the seed, and
+// refinement r = b - A x (A regenerated in FP64 tile by tile), d = LU \ r,
+// x += d until the HPL scaled residual is below the threshold.
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "api_common.hpp"
+#include "mxp.hpp"
+
+namespace {
+
+using hplx::Ctx;
+using hplx::fail;
+using hplx::use;
+
+struct MxpWs {
+  float*  P32  = nullptr;  // FP16 storage: FP32 panel copy, ldp x nb
+  int     ldp  = 0;
+  float*  W    = nullptr;  // gathered top rows, nb x n (ld nb)
+  float*  U    = nullptr;  // FP16 storage: U12 in FP32, nb x n (ld nb)
+  float*  linv = nullptr;  // nb x nb
+  void*   pa   = nullptr;  // packed L21
+  void*   pb   = nullptr;  // packed U12
+  int*    ea   = nullptr;
+  int*    eb   = nullptr;
+  double* r    = nullptr;  // refinement residual / correction, n
+  double* part = nullptr;  // ir_part_len(n)
+  double* out  = nullptr;  // 16 doubles
+};
+
+int hpl_ld(int n) { return ((n > 0 ? n : 0) + 95) / 128 * 128 + 32; }
+
+size_t elem_size(int storage) { return storage == HPLX_STORE_FP16 ? 2 : 4; }
+
+int check_opts(Ctx& c, const hplx_mxp_opts* o, int storage) {
+  if(!o) return fail(c, HPLX_ERR_INVALID, "null mxp opts");
+  if(o->update == HPLX_UPDATE_BF16) {
+    if(o->terms < 1 || o->terms > 3)
+      return fail(c, HPLX_ERR_INVALID, "BF16 terms must be 1, 2 or 3 (got %d)",
+                  o->terms);
+  } else if(o->update == HPLX_UPDATE_FP32) {
+    if(storage != HPLX_STORE_FP32)
+      return fail(c, HPLX_ERR_UNSUPPORTED, "the FP32 update needs FP32 storage");
+  } else if(o->update != HPLX_UPDATE_FP8) {
+    return fail(c, HPLX_ERR_INVALID, "unknown update %d", o->update);
+  }
+  if(o->max_its < 0 || o->max_its > 64)
+    return fail(c, HPLX_ERR_INVALID, "max_its %d outside 0..64", o->max_its);
+  if(!(o->threshold > 0.0)) return fail(c, HPLX_ERR_INVALID, "threshold <= 0");
+  return HPLX_SUCCESS;
+}
+
+int check_mat(Ctx& c, const hplx_mxp_pmat* m) {
+  if(!m || !m->A || !m->ws || m->n < 1 || m->nb < 1 ||
+     (m->storage != HPLX_STORE_FP32 && m->storage != HPLX_STORE_FP16))
+    return fail(c, HPLX_ERR_INVALID, "bad mxp pmat");
+  return HPLX_SUCCESS;
+}
+
+template <typename S>
+int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
+                hplx_mxp_pmat* mat) {
+  const int N = mat->n, nb = mat->nb, lda = mat->ld;
+  if(N > hplx::max_leaf_rows(c))
+    return fail(c, HPLX_ERR_UNSUPPORTED, "N=%d exceeds the panel kernel's %d rows",
+                N, hplx::max_leaf_rows(c));
+  MxpWs*      ws = static_cast<MxpWs*>(mat->ws);
+  S*          A  = static_cast<S*>(mat->A);
+  hipStream_t s  = c.stream;
+  constexpr bool kF16 = sizeof(S) == 2;
+
+  const int npanels = (N + nb - 1) / nb;
+  while(static_cast<int>(c.ev.size()) < 2 * npanels) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c.ev.push_back(e);
+  }
+  c.ev_pairs   = 0;
+  c.gemm_flops = 0.0;
+  hplx::Progress prog(c, npanels, nb, N);
+  HIPCHK(c, prog.init());
+
+  HIPCHK(c, hplx::reset_panel_state(c));
+  for(int j = 0; j < N; j += nb) {
+    const int jb = (N - j) < nb ? (N - j) : nb;
+    const int m  = N - j;
+    S*        Ap = A + j + static_cast<size_t>(j) * lda;
+    float*    P;
+    int       ldP;
+    if constexpr(kF16) {
+      HIPCHK(c, hplx::launch_copy_block(s, m, jb, Ap, lda, ws->P32, ws->ldp));
+      P   = ws->P32;
+      ldP = ws->ldp;
+    } else {
+      P   = Ap;
+      ldP = lda;
+    }
+    HIPCHK(c, hplx::panel_factor(c, palg->nbmin, palg->nbdiv, P, ldP, m, jb,
+                                 mat->ipiv + j, j));
+    if constexpr(kF16)
+      HIPCHK(c, hplx::launch_copy_block(s, m, jb, P, ldP, Ap, lda));
+    const int nt = N - j - jb;
+    if(nt > 0) {
+      S* A12 = A + j + static_cast<size_t>(j + jb) * lda;
+      HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
+      HIPCHK(c, hplx::launch_linv(s, jb, P, ldP, ws->linv, nb));
+      HIPCHK(c, hplx::launch_swap_gather(s, jb, nt, A12, lda, c.permU, ws->W, nb));
+      const float* U;
+      int          ldu;
+      if constexpr(kF16) {
+        HIPCHK(c, hplx::launch_sgemm(s, jb, nt, jb, 1.0, ws->linv, nb, ws->W, nb,
+                                     0.0, ws->U, nb, true));
+        HIPCHK(c, hplx::launch_copy_block(s, jb, nt, ws->U, nb, A12, lda));
+        U   = ws->U;
+        ldu = nb;
+      } else {
+        HIPCHK(c, hplx::launch_sgemm(s, jb, nt, jb, 1.0, ws->linv, nb, ws->W, nb,
+                                     0.0, A12, lda, true));
+        U   = A12;
+        ldu = lda;
+      }
+      if(m - jb > 0) {
+        S*        C22 = A12 + jb;
+        hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
+        if(o->update == HPLX_UPDATE_FP32) {
+          if constexpr(!kF16) {
+            HIPCHK(c, hipEventRecord(e0, s));
+            HIPCHK(c, hplx::launch_sgemm(s, m - jb, nt, jb, -1.0, P + jb, ldP, U,
+                                         ldu, 1.0, C22, lda, false));
+            HIPCHK(c, hipEventRecord(e1, s));
+          }
+        } else {
+          hplx::MxpPacked pk;
+          pk.a  = ws->pa;
+          pk.b  = ws->pb;
+          pk.ea = ws->ea;
+          pk.eb = ws->eb;
+          HIPCHK(c, hplx::launch_mxp_pack(s, o->update, o->terms, m - jb, jb,
+                                          P + jb, ldP, nt, U, ldu, pk));
+          HIPCHK(c, hipEventRecord(e0, s));
+          HIPCHK(c, hplx::launch_mxp_gemm(s, o->update, o->terms, m - jb, nt,
+                                          jb, pk, C22, lda));
+          HIPCHK(c, hipEventRecord(e1, s));
+        }
+        c.ev_pairs++;
+        c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
+      }
+    }
+    HIPCHK(c, prog.panel_queued(j / nb));
+  }
+  int flags[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
+                           s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if(flags[1])
+    return fail(c, HPLX_ERR_HIP,
+                "panel kernel wait timed out (workgroups not co-resident)");
+  mat->info = flags[0];
+  return flags[0];
+}
+
+template <typename S>
+int solve(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
+  const int   N = mat->n, nb = mat->nb, lda = mat->ld;
+  const S*    A = static_cast<const S*>(mat->A);
+  hipStream_t s = c.stream;
+  // L y = P b, panel by panel (each panel's interchanges, then its L11 and
+  // L21; L is left unpivoted by later panels)
+  for(int j = 0; j < N; j += nb) {
+    const int jb = (N - j) < nb ? (N - j) : nb;
+    HIPCHK(c, hplx::launch_vec_swaps(s, j, jb, mat->ipiv, x));
+    HIPCHK(c, hplx::launch_tri_solve_vec(s, true, jb,
+                                         A + j + static_cast<size_t>(j) * lda,
+                                         lda, x + j));
+    if(N - j - jb > 0)
+      HIPCHK(c, hplx::launch_gemv_sub(s, N - j - jb, jb,
+                                      A + j + jb + static_cast<size_t>(j) * lda,
+                                      lda, x + j, x + j + jb));
+  }
+  // U x = y, blocks from the bottom
+  const int nblk = (N + nb - 1) / nb;
+  for(int J = nblk - 1; J >= 0; --J) {
+    const int s0 = J * nb;
+    const int kb = (N - s0) < nb ? (N - s0) : nb;
+    HIPCHK(c, hplx::launch_tri_solve_vec(s, false, kb,
+                                         A + s0 + static_cast<size_t>(s0) * lda,
+                                         lda, x + s0));
+    if(s0 > 0)
+      HIPCHK(c, hplx::launch_gemv_sub(s, s0, kb, A + static_cast<size_t>(s0) * lda,
+                                      lda, x + s0, x));
+  }
+  return HPLX_SUCCESS;
+}
+
+int solve_any(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
+  return mat->storage == HPLX_STORE_FP16 ? solve<__half>(c, mat, x)
+                                         : solve<float>(c, mat, x);
+}
+
+int factor_any(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
+               hplx_mxp_pmat* mat) {
+  int r;
+  if((r = hplx::check_palg(c, palg))) return r;
+  if((r = check_mat(c, mat))) return r;
+  if((r = check_opts(c, o, mat->storage))) return r;
+  return mat->storage == HPLX_STORE_FP16 ? factor_loop<__half>(c, palg, o, mat)
+                                         : factor_loop<float>(c, palg, o, mat);
+}
+
+double gemm_ms(Ctx& c) {
+  double t = 0.0;
+  for(int i = 0; i < c.ev_pairs; ++i) {
+    float ms = 0.f;
+    if(hipEventElapsedTime(&ms, c.ev[2 * i], c.ev[2 * i + 1]) == hipSuccess) t += ms;
+  }
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hplx_mxp_opts_default(hplx_mxp_opts* o) {
+  if(!o) return;
+  o->update    = HPLX_UPDATE_BF16;
+  o->terms     = 2;
+  o->max_its   = 50;
+  o->reserved  = 0;
+  o->threshold = 16.0;
+}
+
+int hplx_mxp_alloc(hplx_ctx* ctx, int n, int nb, int storage,
+                   hplx_mxp_pmat* mat) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(!mat || n < 1 || nb < 1 || nb > hplx::kMaxNB ||
+     (storage != HPLX_STORE_FP32 && storage != HPLX_STORE_FP16))
+    return fail(c, HPLX_ERR_INVALID, "mxp_alloc: n=%d nb=%d storage=%d", n, nb,
+                storage);
+  std::memset(mat, 0, sizeof(*mat));
+  MxpWs* ws = new(std::nothrow) MxpWs();
+  if(!ws) return HPLX_ERR_ALLOC;
+  mat->n       = n;
+  mat->nb      = nb;
+  mat->ld      = hpl_ld(n);
+  mat->storage = storage;
+  mat->ws      = ws;
+  const size_t es   = elem_size(storage);
+  const size_t pack = std::max(hplx::mxp_pack_bytes(HPLX_UPDATE_BF16, 3, n, nb),
+                               hplx::mxp_pack_bytes(HPLX_UPDATE_FP8, 1, n, nb));
+  const size_t nsc  = hplx::mxp_scale_count(n, nb);
+  bool ok = hipMalloc(&mat->A, es * mat->ld * static_cast<size_t>(n)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&mat->X), 8 * static_cast<size_t>(n)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&mat->ipiv), 4 * static_cast<size_t>(n)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->W), 4 * static_cast<size_t>(nb) * n) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->linv), 4 * static_cast<size_t>(nb) * nb) == hipSuccess &&
+            hipMalloc(&ws->pa, pack) == hipSuccess && hipMalloc(&ws->pb, pack) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->ea), 4 * nsc) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->eb), 4 * nsc) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->r), 8 * static_cast<size_t>(n)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->part), 8 * hplx::ir_part_len(n)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->out), 16 * 8) == hipSuccess;
+  if(ok && storage == HPLX_STORE_FP16) {
+    ws->ldp = static_cast<int>(hplx::align_up(n, 64));
+    ok = hipMalloc(reinterpret_cast<void**>(&ws->P32), 4 * static_cast<size_t>(ws->ldp) * nb) == hipSuccess &&
+         hipMalloc(reinterpret_cast<void**>(&ws->U), 4 * static_cast<size_t>(nb) * n) == hipSuccess;
+  }
+  if(!ok) {
+    hplx_mxp_free(ctx, mat);
+    return fail(c, HPLX_ERR_ALLOC, "mxp_alloc: out of device memory (n=%d)", n);
+  }
+  return HPLX_SUCCESS;
+}
+
+int hplx_mxp_free(hplx_ctx* ctx, hplx_mxp_pmat* mat) {
+  int r = use(ctx);
+  if(r) return r;
+  if(!mat) return HPLX_SUCCESS;
+  hipStreamSynchronize(ctx->c.stream);
+  if(MxpWs* ws = static_cast<MxpWs*>(mat->ws)) {
+    for(void* p : {static_cast<void*>(ws->P32), static_cast<void*>(ws->W),
+                   static_cast<void*>(ws->U), static_cast<void*>(ws->linv), ws->pa,
+                   ws->pb, static_cast<void*>(ws->ea), static_cast<void*>(ws->eb),
+                   static_cast<void*>(ws->r), static_cast<void*>(ws->part),
+                   static_cast<void*>(ws->out)})
+      if(p) hipFree(p);
+    delete ws;
+  }
+  if(mat->A) hipFree(mat->A);
+  if(mat->X) hipFree(mat->X);
+  if(mat->ipiv) hipFree(mat->ipiv);
+  std::memset(mat, 0, sizeof(*mat));
+  return HPLX_SUCCESS;
+}
+
+int hplx_mxp_matgen(hplx_ctx* ctx, hplx_mxp_pmat* mat, int seed,
+                    double diag_shift) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if((r = check_mat(c, mat))) return r;
+  if(seed < 0) return fail(c, HPLX_ERR_INVALID, "seed %d", seed);
+  mat->seed       = seed;
+  mat->diag_shift = diag_shift;
+  if(mat->storage == HPLX_STORE_FP16)
+    HIPCHK(c, hplx::launch_matgen(c.stream, mat->n, mat->n,
+                                  static_cast<__half*>(mat->A), mat->ld, seed,
+                                  diag_shift));
+  else
+    HIPCHK(c, hplx::launch_matgen(c.stream, mat->n, mat->n,
+                                  static_cast<float*>(mat->A), mat->ld, seed,
+                                  diag_shift));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_mxp_pdgetrf(hplx_ctx* ctx, const hplx_palg* palg,
+                     const hplx_mxp_opts* opts, hplx_mxp_pmat* mat) {
+  int r = use(ctx);
+  if(r) return r;
+  return factor_any(ctx->c, palg, opts, mat);
+}
+
+int hplx_mxp_pdgetrs(hplx_ctx* ctx, const hplx_mxp_pmat* mat, double* x) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if((r = check_mat(c, mat))) return r;
+  if(!x) return fail(c, HPLX_ERR_INVALID, "null x");
+  if((r = solve_any(c, mat, x))) return r;
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_mxp_residual(hplx_ctx* ctx, const hplx_mxp_pmat* mat,
+                      const double* x, double* out) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if((r = check_mat(c, mat))) return r;
+  if(!x || !out) return fail(c, HPLX_ERR_INVALID, "null x/out");
+  MxpWs* ws = static_cast<MxpWs*>(mat->ws);
+  HIPCHK(c, hplx::launch_ir_residual(c.stream, mat->n, mat->seed, mat->diag_shift,
+                                     x, ws->r, ws->part, ws->out));
+  HIPCHK(c, hipMemcpyAsync(out, ws->out, 5 * sizeof(double),
+                           hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_mxp_pdgesv(hplx_ctx* ctx, const hplx_palg* palg,
+                    const hplx_mxp_opts* opts, hplx_mxp_pmat* mat,
+                    hplx_mxp_result* res) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  hplx_mxp_result local;
+  hplx_mxp_result& R = res ? *res : local;
+  std::memset(&R, 0, sizeof(R));
+  const auto t0 = std::chrono::steady_clock::now();
+  r = factor_any(c, palg, opts, mat);
+  if(r) return r;
+  const auto t1 = std::chrono::steady_clock::now();
+  R.factor_ms       = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  R.update_ms       = gemm_ms(c);
+  R.update_flops    = c.gemm_flops;
+  R.update_launches = c.ev_pairs;
+
+  MxpWs*      ws = static_cast<MxpWs*>(mat->ws);
+  hipStream_t s  = c.stream;
+  const int   N  = mat->n;
+  HIPCHK(c, hplx::launch_gen_b(s, N, mat->seed, mat->X));
+  if((r = solve_any(c, mat, mat->X))) return r;
+  for(int it = 0;; ++it) {
+    HIPCHK(c, hplx::launch_ir_residual(s, N, mat->seed, mat->diag_shift, mat->X,
+                                       ws->r, ws->part, ws->out));
+    double resid = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&resid, ws->out, sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    R.history[it] = resid;
+    R.resid       = resid;
+    R.its         = it;
+    if(resid < opts->threshold) {
+      R.converged = 1;
+      break;
+    }
+    if(it >= opts->max_its || !(resid == resid)) break;
+    if((r = solve_any(c, mat, ws->r))) return r;
+    HIPCHK(c, hplx::launch_axpy(s, N, ws->r, mat->X));
+  }
+  R.solve_ms = std::chrono::duration<double, std::milli>(
+                   std::chrono::steady_clock::now() - t1)
+                   .count();
+  return R.converged ? HPLX_SUCCESS : 1 + N;
+}
+
+int hplx_mxp_update(hplx_ctx* ctx, int update, int terms, int storage, int m,
+                    int n, int k, const float* L, int ldl, const float* U,
+                    int ldu, void* C, int ldc) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(m < 0 || n < 0 || k < 0 || k > hplx::kMaxNB || !L || !U || !C ||
+     ldl < m || ldu < k || ldc < m ||
+     (storage != HPLX_STORE_FP32 && storage != HPLX_STORE_FP16))
+    return fail(c, HPLX_ERR_INVALID, "mxp_update: bad arguments");
+  if(update == HPLX_UPDATE_BF16 && (terms < 1 || terms > 3))
+    return fail(c, HPLX_ERR_INVALID, "terms %d", terms);
+  if(m == 0 || n == 0 || k == 0) return HPLX_SUCCESS;
+  hipStream_t s = c.stream;
+  if(update == HPLX_UPDATE_FP32) {
+    if(storage != HPLX_STORE_FP32)
+      return fail(c, HPLX_ERR_UNSUPPORTED, "FP32 update needs FP32 storage");
+    HIPCHK(c, hplx::launch_sgemm(s, m, n, k, -1.0, L, ldl, U, ldu, 1.0,
+                                 static_cast<float*>(C), ldc, false));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return HPLX_SUCCESS;
+  }
+  if(update != HPLX_UPDATE_BF16 && update != HPLX_UPDATE_FP8)
+    return fail(c, HPLX_ERR_INVALID, "update %d", update);
+  hplx::MxpPacked pk;
+  const size_t ba = hplx::mxp_pack_bytes(update, terms, m, k);
+  const size_t bb = hplx::mxp_pack_bytes(update, terms, n, k);
+  const size_t sa = hplx::mxp_scale_count(m, k), sb = hplx::mxp_scale_count(n, k);
+  if(hipMalloc(&pk.a, ba) != hipSuccess || hipMalloc(&pk.b, bb) != hipSuccess ||
+     hipMalloc(reinterpret_cast<void**>(&pk.ea), 4 * sa) != hipSuccess ||
+     hipMalloc(reinterpret_cast<void**>(&pk.eb), 4 * sb) != hipSuccess) {
+    for(void* p : {pk.a, pk.b, static_cast<void*>(pk.ea), static_cast<void*>(pk.eb)})
+      if(p) hipFree(p);
+    return fail(c, HPLX_ERR_ALLOC, "mxp_update: scratch allocation");
+  }
+  hipError_t e = hplx::launch_mxp_pack(s, update, terms, m, k, L, ldl, n, U,
+                                       ldu, pk);
+  if(e == hipSuccess)
+    e = storage == HPLX_STORE_FP16
+            ? hplx::launch_mxp_gemm(s, update, terms, m, n, k, pk,
+                                    static_cast<__half*>(C), ldc)
+            : hplx::launch_mxp_gemm(s, update, terms, m, n, k, pk,
+                                    static_cast<float*>(C), ldc);
+  const hipError_t es = hipStreamSynchronize(s);
+  for(void* p : {pk.a, pk.b, static_cast<void*>(pk.ea), static_cast<void*>(pk.eb)})
+    hipFree(p);
+  HIPCHK(c, e);
+  HIPCHK(c, es);
+  return HPLX_SUCCESS;
+}
+
+int hplx_spdfact(hplx_ctx* ctx, int nbmin, int nbdiv, float* P, int lda,
+                 int m, int jb, int* ipiv, int row_offset) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(!P || !ipiv || m < 1 || jb < 1 || jb > m || lda < m || row_offset < 0)
+    return fail(c, HPLX_ERR_INVALID, "spdfact: m=%d jb=%d lda=%d", m, jb, lda);
+  if(m > hplx::max_leaf_rows(c))
+    return fail(c, HPLX_ERR_UNSUPPORTED, "spdfact: m=%d too tall", m);
+  HIPCHK(c, hplx::reset_panel_state(c));
+  HIPCHK(c, hplx::panel_factor(c, nbmin, nbdiv, P, lda, m, jb, ipiv, row_offset));
+  int flags[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
+                           c.stream));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  if(flags[1]) return fail(c, HPLX_ERR_HIP, "panel kernel wait timed out");
+  return flags[0];
+}
+
+}  // extern "C"

@@ -9,8 +9,8 @@ import pytest
 
 import rochpl_amd as rp
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include",
-                      "hplx.h")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hplx.h")
 
 
 def declared_functions():
@@ -54,6 +54,38 @@ def test_struct_layouts_match_header():
     # hplx_pmat: 3 pointers, 6 ints, a pointer -> 56 bytes on LP64
     assert ctypes.sizeof(rp.PMat) == 56
     assert rp.PMat.ws.offset == 48
+
+
+def test_struct_layouts_match_a_c_compiler(tmp_path):
+    """sizeof / offsetof of every header struct, as gcc lays them out, equal
+    the ctypes mirrors'."""
+    import shutil
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    structs = {"hplx_palg": rp.PAlg, "hplx_pmat": rp.PMat,
+               "hplx_mxp_opts": rp.MxpOpts, "hplx_mxp_pmat": rp.MxpPMat,
+               "hplx_mxp_result": rp.MxpResult}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hplx.h"',
+             'int main(void) {']
+    for cname, py in structs.items():
+        lines.append('printf("%s sizeof %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));'
+                         % (cname, fname, cname, fname))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True,
+                         check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, \
+                (cname, fname)
 
 
 def test_errors_without_device_are_codes_not_exits():

@@ -1,0 +1,253 @@
+"""Mixed-precision path through the C ABI (hplx_mxp_*, hplx_spdfact).
+
+The reference has no low-precision path.  The checks are anchored on:
+  * the update: oracle/mxp_oracle.py's exact product of the rounded operands
+    (pinned against torch's BF16 / e4m3 conversions), with the FP32
+    accumulation error bounded;
+  * the FP32 panel: the defining properties of LU with partial pivoting
+    (|L| <= 1, P A = L U to FP32 rounding);
+  * the solves with FP32 / FP16 factors: an FP64 CPU solve with the same
+    downloaded factors;
+  * the refinement residual: the FP64 oracle residual (hplo_residual);
+  * end to end: the HPL scaled residual < 16 after refinement, pivots equal
+    to the FP64 oracle's where the matrix fixes them (diagonal dominance).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import rochpl_amd as rp
+from oracle import mxp_oracle as mo
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+U24 = 2.0 ** -24
+
+
+def say(msg):
+    print("[gpu-case] " + msg, flush=True)
+
+
+def upload(ctx, arr):
+    arr = np.asfortranarray(arr)
+    d = ctx.alloc(arr.nbytes)
+    d.upload(arr)
+    return d
+
+
+def download_factors(ctx, mat):
+    dt = np.float16 if mat.storage == rp.HPLX_STORE_FP16 else np.float32
+    out = np.empty((mat.ld, mat.n), dtype=dt, order="F")
+    rp.lib().hplx_memcpy_d2h(ctx.h, out.ctypes.data_as(ctypes.c_void_p),
+                             ctypes.c_void_p(mat.A), out.nbytes)
+    return out[:mat.n, :].astype(np.float64)
+
+
+def hpl_lu_solve(F, ipiv, nb, b):
+    """x = U^{-1} L^{-1} P b with HPL's panel-wise (unpivoted) L, in FP64."""
+    n = len(b)
+    y = np.array(b, dtype=np.float64)
+    for j in range(0, n, nb):
+        jb = min(nb, n - j)
+        for i in range(j, j + jb):
+            p = ipiv[i]
+            y[i], y[p] = y[p], y[i]
+        y[j:j + jb] = sl.solve_triangular(F[j:j + jb, j:j + jb], y[j:j + jb],
+                                          lower=True, unit_diagonal=True)
+        y[j + jb:] -= F[j + jb:, j:j + jb] @ y[j:j + jb]
+    return sl.solve_triangular(np.triu(F), y, lower=False)
+
+
+# ---------------------------------------------------------------------------
+# the low-precision trailing update
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("update,terms,storage,m,n,k", [
+    (rp.HPLX_UPDATE_BF16, 1, rp.HPLX_STORE_FP32, 300, 517, 200),
+    (rp.HPLX_UPDATE_BF16, 2, rp.HPLX_STORE_FP32, 300, 517, 512),
+    (rp.HPLX_UPDATE_BF16, 3, rp.HPLX_STORE_FP32, 260, 130, 96),
+    (rp.HPLX_UPDATE_BF16, 2, rp.HPLX_STORE_FP16, 384, 256, 256),
+    (rp.HPLX_UPDATE_FP8, 1, rp.HPLX_STORE_FP32, 300, 517, 200),
+    (rp.HPLX_UPDATE_FP8, 1, rp.HPLX_STORE_FP16, 513, 300, 1024),
+    (rp.HPLX_UPDATE_FP32, 1, rp.HPLX_STORE_FP32, 200, 100, 64),
+])
+def test_mxp_update_matches_oracle(ctx, update, terms, storage, m, n, k):
+    say("mxp_update upd=%d terms=%d storage=%d %dx%dx%d" % (update, terms,
+                                                           storage, m, n, k))
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    ldl, ldu, ldc = m + 3, k + 5, m + 7
+    L = np.zeros((ldl, k), np.float32, order="F")
+    U = np.zeros((ldu, n), np.float32, order="F")
+    L[:m] = rng.uniform(-1, 1, (m, k))
+    # rows of U with different magnitudes exercise the per-tile scales
+    U[:k] = rng.uniform(-0.5, 0.5, (k, n)) * \
+        np.where(np.arange(n) % 300 < 150, 1.0, 1e-3)[None, :]
+    cdt = np.float16 if storage == rp.HPLX_STORE_FP16 else np.float32
+    C = np.zeros((ldc, n), cdt, order="F")
+    C[:m] = rng.uniform(-1, 1, (m, n))
+    dL, dU, dC = upload(ctx, L), upload(ctx, U), upload(ctx, C)
+    try:
+        ctx.mxp_update(update, terms, storage, m, n, k, dL.ptr, ldl, dU.ptr,
+                       ldu, dC.ptr, ldc)
+        out = dC.download((ldc, n), cdt)
+    finally:
+        for d in (dL, dU, dC):
+            d.free()
+    ref = C[:m].astype(np.float64) - mo.update_product(update, terms, L[:m],
+                                                       U[:k])
+    absp = mo.update_abs_product(update, terms, L[:m], U[:k])
+    bound = U24 * (k + 8) * (np.abs(C[:m].astype(np.float64)) + absp)
+    if storage == rp.HPLX_STORE_FP16:
+        bound += 2.0 ** -11 * np.abs(ref)
+    err = np.abs(out[:m].astype(np.float64) - ref)
+    print("[mxp_update] max err / bound = %.3e" % (err / bound).max(),
+          flush=True)
+    assert np.all(err <= bound)
+    # rows beyond m and the padding rows of C are untouched
+    assert np.array_equal(out[m:], C[m:])
+
+
+# ---------------------------------------------------------------------------
+# FP32 panel factorisation
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("m,jb,nbmin", [(3000, 256, 32), (777, 96, 16),
+                                        (40000, 128, 32)])
+def test_spdfact_is_partial_pivoting_lu(ctx, m, jb, nbmin):
+    say("spdfact m=%d jb=%d" % (m, jb))
+    rng = np.random.default_rng(m + jb)
+    lda = m + 13
+    A = np.zeros((lda, jb), np.float32, order="F")
+    A[:m] = rng.uniform(-0.5, 0.5, (m, jb))
+    dA = upload(ctx, A)
+    dp = ctx.alloc(4 * jb)
+    try:
+        info = ctx.spdfact(dA.ptr, lda, m, jb, dp.ptr, nbmin=nbmin)
+        F = dA.download((lda, jb), np.float32)[:m].astype(np.float64)
+        ipiv = dp.download((jb,), np.int32)
+    finally:
+        dA.free()
+        dp.free()
+    assert info == 0
+    perm = np.arange(m)
+    for i in range(jb):
+        p = ipiv[i]
+        assert i <= p < m
+        perm[i], perm[p] = perm[p], perm[i]
+    Lf = np.tril(F, -1)
+    Lf[:jb, :jb] += np.eye(jb)
+    Uf = np.triu(F[:jb])
+    # partial pivoting: every multiplier is at most 1 in magnitude
+    assert np.abs(np.tril(F, -1)).max() <= 1.0
+    # P A = L U to FP32 rounding (backward error of Gaussian elimination)
+    R = A[:m].astype(np.float64)[perm] - Lf @ Uf
+    bound = 2 * (jb + 2) * U24 * (np.abs(Lf) @ np.abs(Uf))
+    print("[spdfact] max |PA-LU| / bound = %.3e" % (np.abs(R) / bound).max(),
+          flush=True)
+    assert np.all(np.abs(R) <= bound)
+    # agreement with the FP64 oracle's pivots on the same (FP32) panel:
+    # columns may differ only where FP32 rounding can decide a near tie
+    ref_piv, _ = o.pdfact(np.asfortranarray(A[:m].astype(np.float64)),
+                          nbmin=nbmin)
+    agree = np.mean(ref_piv == ipiv)
+    print("[spdfact] pivot agreement with the FP64 oracle %.4f" % agree,
+          flush=True)
+    assert agree >= 0.9
+
+
+# ---------------------------------------------------------------------------
+# solves with the stored low-precision factors, refinement residual
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("storage,update,n,nb,shift", [
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_FP32, 2048, 256, 0.0),
+    (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1536, 512, 384.0),
+])
+def test_mxp_getrs_and_residual(ctx, storage, update, n, nb, shift):
+    say("mxp getrs storage=%d n=%d nb=%d" % (storage, n, nb))
+    mat = ctx.mxp_alloc(n, nb, storage)
+    opts = rp.mxp_opts_default(update, 1)
+    try:
+        ctx.mxp_matgen(mat, rp.HPL_ISEED, shift)
+        # the stored matrix is the generator's, rounded to the storage type
+        A64 = o.matgen(n, n, diag_shift=shift)
+        A0 = download_factors(ctx, mat)
+        if storage == rp.HPLX_STORE_FP32:
+            assert np.array_equal(A0, A64.astype(np.float32).astype(np.float64))
+        else:
+            assert np.array_equal(
+                A0, A64.astype(np.float32).astype(np.float16).astype(np.float64))
+        assert ctx.mxp_pdgetrf(rp.palg_default(), opts, mat) == 0
+        F = download_factors(ctx, mat)
+        ipiv = ctx.download_vec(mat.ipiv, n, np.int32)
+        b = o.matgen_tile(n, 0, n, n, 1)[:, 0]
+        d = upload(ctx, b)
+        ctx.mxp_pdgetrs(mat, d.ptr)
+        x = d.download((n,), np.float64)
+        # the same solve on the CPU in FP64 with the downloaded factors
+        xr = hpl_lu_solve(F, ipiv, nb, b)
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+        # refinement residual against the FP64 oracle's
+        res = ctx.mxp_residual(mat, d.ptr)
+        ref = o.residual(n, x, diag_shift=shift)
+        d.free()
+        assert abs(res["rnorm"] - ref["rnorm"]) <= 1e-12 * ref["rnorm"] + 1e-300
+        assert abs(res["resid"] - ref["resid"]) <= 1e-10 * ref["resid"]
+        assert res["AnormI"] == pytest.approx(ref["AnormI"], rel=1e-13)
+        assert res["BnormI"] == ref["BnormI"]
+    finally:
+        ctx.mxp_free(mat)
+
+
+# ---------------------------------------------------------------------------
+# end to end: factor + FP64 refinement
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("storage,update,terms,n,nb,shift", [
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_FP32, 1, 4096, 256, 0.0),
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_BF16, 2, 4096, 256, 0.0),
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_BF16, 3, 4096, 512, 0.0),
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_BF16, 1, 3000, 256, 750.0),
+    (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1, 4096, 1024, 1024.0),
+    (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_BF16, 2, 2500, 256, 625.0),
+])
+def test_mxp_pdgesv_converges(ctx, storage, update, terms, n, nb, shift):
+    say("mxp_pdgesv storage=%d update=%d terms=%d n=%d nb=%d shift=%g" %
+        (storage, update, terms, n, nb, shift))
+    mat = ctx.mxp_alloc(n, nb, storage)
+    opts = rp.mxp_opts_default(update, terms)
+    try:
+        ctx.mxp_matgen(mat, rp.HPL_ISEED, shift)
+        rc, res = ctx.mxp_pdgesv(rp.palg_default(), opts, mat)
+        x = ctx.download_vec(mat.X, n)
+        ipiv = ctx.download_vec(mat.ipiv, n, np.int32)
+    finally:
+        ctx.mxp_free(mat)
+    print("[mxp] its=%d resid=%.4f history=%s" % (
+        res["its"], res["resid"], ["%.3g" % h for h in res["history"]]),
+        flush=True)
+    assert rc == 0 and res["converged"] == 1
+    ref = o.residual(n, x, diag_shift=shift)
+    assert ref["resid"] < 16.0
+    assert ref["resid"] == pytest.approx(res["resid"], rel=1e-8)
+    r64 = o.pdgesv(n, nb, diag_shift=shift)
+    err = np.linalg.norm(x - r64["x"]) / np.linalg.norm(r64["x"])
+    print("[mxp] ||x - x_fp64|| / ||x_fp64|| = %.3e" % err, flush=True)
+    assert err < 1e-9
+    if shift >= n / 4:
+        # diagonally dominant: partial pivoting keeps every diagonal pivot,
+        # as the FP64 oracle does
+        assert np.array_equal(ipiv, np.arange(n))
+        assert np.array_equal(ipiv, r64["ipiv"])
+
+
+def test_mxp_rejects_bad_options(ctx):
+    mat = ctx.mxp_alloc(256, 64, rp.HPLX_STORE_FP16)
+    try:
+        with pytest.raises(rp.HplxError):
+            ctx.mxp_pdgetrf(rp.palg_default(),
+                            rp.mxp_opts_default(rp.HPLX_UPDATE_FP32, 1), mat)
+        with pytest.raises(rp.HplxError):
+            ctx.mxp_pdgetrf(rp.palg_default(),
+                            rp.mxp_opts_default(rp.HPLX_UPDATE_BF16, 4), mat)
+    finally:
+        ctx.mxp_free(mat)
