@@ -5,8 +5,18 @@ matrix, timed as the reference times it (src/HPL_pdtest.cpp:128-136): the
 matrix is generated before the timed region and the residual check runs after
 it.  Every step must pass the HPL scaled-residual test (< 16).
 
-Default workload: N=193280, NB=896 in FP64 on one GPU (the [A|b] matrix fills
-278.4 GiB of HBM).  Prints ONE JSON line.
+Default: the FP64 solve at N=193280, NB=896 on one GPU (the [A|b] matrix
+fills 278.4 GiB of HBM) -- the line's `value` -- followed by the two
+mixed-precision forms, reported in the same JSON line:
+  * "bf16": N=131072, NB=512, FP32 factors, BF16 MFMA update (operands split
+    into --terms BF16 numbers), FP32 panel, FP64 refinement;
+  * "fp8":  N=229376, NB=1024, FP16 factors, per-128x128-tile power-of-two
+    scaled e4m3 MFMA update, FP32 panel, FP64 refinement,
+    A = U(-0.5,0.5) + (N/4) I.
+Their timed region is the whole hplx_mxp_pdgesv (factor, first solve and
+every refinement step); HPL-equivalent TFLOP/s = (2/3 N^3 + 2 N^2) / t.
+--mode fp64|bf16|fp8 runs one form alone and makes it the line's value.
+Prints ONE JSON line.
 """
 import argparse
 import json
@@ -23,8 +33,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("HPL-equiv TFLOP/s at residual<16, 1 GPU: FP64 LU vs BF16/FP8 "
           "LU+FP64 IR")
-FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix peak
+BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
+FP8_PEAK_TFLOPS = 5000.0   # dense FP8 MFMA
+HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "trailing_update_traffic.json")
+
+FORMS = {
+    # name: (N, NB, storage, update, diag shift as a fraction of N)
+    "bf16": (131072, 512, "fp32", "bf16", 0.0),
+    "fp8": (229376, 1024, "fp16", "fp8", 0.25),
+}
 
 
 def log(msg):
@@ -69,14 +88,201 @@ def load_traffic(n, nb):
     return None
 
 
+def run_fp64(ctx, rp, args):
+    import numpy as np
+    n, nb = args.n, args.nb
+    palg = rp.palg_default()
+    palg.frac = args.frac
+
+    # kernel pre-warm on a small matrix (cf. HPL_pdwarmup); not a step.  Only
+    # for small N: at large N the one-off code-object load is lost in a
+    # minutes-long solve, and without it every trailing-update launch in a
+    # kernel trace of this command belongs to the timed solve.
+    prewarm = args.prewarm
+    if prewarm is None:
+        prewarm = min(n, 4096) if n <= 32768 else 0
+    if prewarm > 0:
+        wmat = ctx.pmat_alloc(prewarm, min(nb, prewarm))
+        ctx.pdrandmat(prewarm, prewarm + 1, wmat.nb, wmat.A, wmat.ld,
+                      rp.HPL_ISEED, 0.0)
+        ctx.pdgesv(palg, wmat)
+        ctx.pmat_free(wmat)
+
+    log("fp64: allocating N=%d NB=%d (%.1f GiB)" % (
+        n, nb, rp.hpl_ld(n) * (n + 1) * 8 / 2 ** 30))
+    mat = ctx.pmat_alloc(n, nb)
+    times, checks, stats = [], [], []
+    for step in range(args.warmup + args.steps):
+        ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        info = ctx.pdgesv(palg, mat)
+        t = time.perf_counter() - t0
+        if info != 0:
+            raise SystemExit("pdgesv: exact zero pivot at column %d" % info)
+        st = ctx.update_stats()
+        chk = ctx.pdtest_check(mat)
+        timed = step >= args.warmup
+        log("fp64 %s step %d: %.3f s  %.2f TFLOP/s  resid %.4f  trailing GEMM "
+            "%.2f TFLOP/s over %d launches" % (
+                "timed" if timed else "warmup", step, t,
+                hpl_flops(n) / t / 1e12, chk["resid"],
+                st[2] / (st[0] * 1e-3) / 1e12 if st[0] > 0 else 0.0, st[1]))
+        if not chk["resid"] < 16.0:
+            raise SystemExit("residual check FAILED: %r" % chk)
+        if timed:
+            times.append(t)
+            checks.append(chk)
+            stats.append(st)
+
+    pivots = None
+    gold = os.path.join(ROOT, "tests", "golden", "hpl_golden.npz")
+    if args.check_pivots == 0 and os.path.exists(gold):
+        # the first 64 pivots against LAPACK dgetrf on the N x 64 column
+        # prefix of the same matrix (committed fixture)
+        key = "prefix_ipiv_%d_64" % n
+        fx = np.load(gold, allow_pickle=False)
+        if key in fx.files:
+            gpiv = ctx.download_vec(mat.ipiv, 64, np.int32)
+            pivots = {"prefix": 64, "equal": bool(np.array_equal(gpiv, fx[key])),
+                      "against": "LAPACK dgetrf on the N x 64 column prefix"}
+    if args.check_pivots > 0:
+        import scipy.linalg.lapack as la
+
+        from oracle import oracle as o
+        k = min(args.check_pivots, n)
+        log("prefix pivot check: first %d pivots vs LAPACK dgetrf" % k)
+        gpiv = ctx.download_vec(mat.ipiv, k, np.int32)
+        pref = np.asfortranarray(o.matgen_tile(n, 0, 0, n, k))
+        _, lpiv, _ = la.dgetrf(pref, overwrite_a=1)
+        pivots = {"prefix": k, "equal": bool(np.array_equal(gpiv, lpiv[:k]))}
+    ctx.pmat_free(mat)
+
+    t_mean = sum(times) / len(times)
+    gemm_ms = sum(s[0] for s in stats)
+    gemm_launch = sum(s[1] for s in stats)
+    gemm_flops = sum(s[2] for s in stats)
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    return {
+        "value": hpl_flops(n) / t_mean / 1e12, "t_mean": t_mean,
+        "config": {"workload": "hpl_pdgesv_fp64_p1q1", "N": n, "NB": nb,
+                   "P": 1, "Q": 1, "pfact": "right", "rfact": "right",
+                   "nbmin": 32, "ndiv": 2, "depth": 1, "swap": "long",
+                   "frac": args.frac},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": load_traffic(n, nb),
+                     "kernel": "trailing-update FP64 MFMA GEMM "
+                               "(A22 -= L21*U12), %d launches, %.1f ms"
+                               % (gemm_launch, gemm_ms),
+                     "avg_launch_ms": round(gemm_ms / max(gemm_launch, 1), 4),
+                     "flops_per_launch": gemm_flops / max(gemm_launch, 1)},
+        "residual": round(max(c["resid"] for c in checks), 6),
+        "residual_r0_form": round(max(c["resid_r0_form"] for c in checks), 6),
+        "tflops_ref_formula": round(ref_flops(n) / t_mean / 1e12, 3),
+        "pivot_prefix_check": pivots,
+    }
+
+
+def update_traffic(n, nb, update, terms, storage_bytes):
+    """Algorithmic MFMA operations and HBM bytes of all trailing-update
+    launches of an n x n factorisation with panels of nb: C read and written
+    once in the storage type, the packed operands read once."""
+    ops = hbm = 0.0
+    prods = terms * (terms + 1) // 2 if update == "bf16" else 1
+    opb = 2 * terms if update == "bf16" else 1  # packed bytes per element
+    for j in range(0, n, nb):
+        jb = min(nb, n - j)
+        m = n - j - jb
+        if m <= 0:
+            continue
+        ops += 2.0 * m * m * jb * prods
+        hbm += 2.0 * m * m * storage_bytes + 2.0 * m * jb * opb
+    return ops, hbm
+
+
+def run_mixed(ctx, rp, form, args):
+    import numpy as np
+    n, nb, storage, update, shift_frac = FORMS[form]
+    if form == "bf16" and args.bf16_n:
+        n = args.bf16_n
+    if form == "fp8" and args.fp8_n:
+        n = args.fp8_n
+    shift = shift_frac * n
+    st_code = rp.HPLX_STORE_FP32 if storage == "fp32" else rp.HPLX_STORE_FP16
+    up_code = rp.HPLX_UPDATE_BF16 if update == "bf16" else rp.HPLX_UPDATE_FP8
+    terms = args.terms if update == "bf16" else 1
+    opts = rp.mxp_opts_default(up_code, terms)
+    palg = rp.palg_default()
+    log("%s: allocating N=%d NB=%d (%s factors %.1f GiB)" % (
+        form, n, nb, storage, rp.hpl_ld(n) * n * (4 if storage == "fp32" else 2)
+        / 2 ** 30))
+    mat = ctx.mxp_alloc(n, nb, st_code)
+    try:
+        runs = []
+        for step in range(args.mixed_warmup + 1):
+            ctx.mxp_matgen(mat, rp.HPL_ISEED, shift)
+            t0 = time.perf_counter()
+            rc, res = ctx.mxp_pdgesv(palg, opts, mat)
+            t = time.perf_counter() - t0
+            log("%s step %d: %.3f s  %.2f TFLOP/s  its %d  resid %.4f  factor "
+                "%.1f ms  update %.1f ms over %d launches" % (
+                    form, step, t, hpl_flops(n) / t / 1e12, res["its"],
+                    res["resid"], res["factor_ms"], res["update_ms"],
+                    res["update_launches"]))
+            if rc != 0 or not res["resid"] < 16.0:
+                raise SystemExit("%s: refinement did not reach residual < 16: "
+                                 "%r" % (form, res))
+            runs.append((t, res))
+        t, res = runs[-1]
+        ipiv = ctx.download_vec(mat.ipiv, n, np.int32)
+        ident = int(np.sum(ipiv != np.arange(n)))
+    finally:
+        ctx.mxp_free(mat)
+    sb = 4 if storage == "fp32" else 2
+    ops, hbm = update_traffic(n, nb, update, terms, sb)
+    sec = res["update_ms"] * 1e-3
+    peak = BF16_PEAK_TFLOPS if update == "bf16" else FP8_PEAK_TFLOPS
+    mfma = ops / sec / 1e12
+    gbs = hbm / sec / 1e9
+    fr_m, fr_h = mfma / peak, gbs / HBM_PEAK_GBS
+    return {
+        "value": round(hpl_flops(n) / t / 1e12, 3), "ms": round(t * 1e3, 1),
+        "config": {"workload": "hpl_mxp_%s_p1q1" % form, "N": n, "NB": nb,
+                   "storage": storage, "update": update, "terms": terms,
+                   "panel": "fp32", "refinement": "fp64 classic",
+                   "diag_shift": shift},
+        "refinement_its": res["its"], "residual": round(res["resid"], 6),
+        "residual_history": [float("%.4g" % h) for h in res["history"]],
+        "factor_ms": round(res["factor_ms"], 1),
+        "solve_ms": round(res["solve_ms"], 1),
+        "pivots_off_diagonal": ident,
+        "update_roofline": {
+            "bound": "mfma" if fr_m >= fr_h else "hbm",
+            "mfma_achieved": round(mfma, 2), "mfma_peak": peak,
+            "mfma_frac": round(fr_m, 4), "hbm_achieved_gbs": round(gbs, 1),
+            "hbm_peak_gbs": HBM_PEAK_GBS, "hbm_frac": round(fr_h, 4),
+            "launches": res["update_launches"],
+            "ms": round(res["update_ms"], 1)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--mode", choices=["all", "fp64", "bf16", "fp8"],
+                    default="all")
     ap.add_argument("--n", type=int, default=193280)
     ap.add_argument("--nb", type=int, default=896)
     ap.add_argument("--frac", type=float, default=0.6)
+    ap.add_argument("--terms", type=int, default=3,
+                    help="BF16 terms per operand of the bf16 form (1-3)")
+    ap.add_argument("--bf16-n", type=int, default=0)
+    ap.add_argument("--fp8-n", type=int, default=0)
+    ap.add_argument("--mixed-warmup", type=int, default=0)
     ap.add_argument("--cpu-n", type=int, default=24576)
     ap.add_argument("--cpu-nb", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
@@ -93,101 +299,58 @@ def main():
               "multi-GPU runs are not part of it" % args.gpus, flush=True)
         sys.exit(2)
 
-    import numpy as np
-
     import rochpl_amd as rp
 
-    n, nb = args.n, args.nb
     ctx = rp.Context(0)
-    palg = rp.palg_default()
-    palg.frac = args.frac
-
-    # kernel pre-warm on a small matrix (cf. HPL_pdwarmup); not a step.  Only
-    # for small N: at large N the one-off code-object load is lost in a
-    # minutes-long solve, and without it every trailing-update launch in a
-    # kernel trace of this command belongs to the timed solve.
-    if args.prewarm is None:
-        args.prewarm = min(n, 4096) if n <= 32768 else 0
-    if args.prewarm > 0:
-        wn = args.prewarm
-        wmat = ctx.pmat_alloc(wn, min(nb, wn))
-        ctx.pdrandmat(wn, wn + 1, wmat.nb, wmat.A, wmat.ld, rp.HPL_ISEED, 0.0)
-        ctx.pdgesv(palg, wmat)
-        ctx.pmat_free(wmat)
-
-    log("allocating N=%d NB=%d (%.1f GiB)" % (n, nb, rp.hpl_ld(n) * (n + 1)
-                                               * 8 / 2 ** 30))
-    mat = ctx.pmat_alloc(n, nb)
-    times, checks, stats = [], [], []
-    for step in range(args.warmup + args.steps):
-        ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        info = ctx.pdgesv(palg, mat)
-        t = time.perf_counter() - t0
-        if info != 0:
-            raise SystemExit("pdgesv: exact zero pivot at column %d" % info)
-        st = ctx.update_stats()
-        chk = ctx.pdtest_check(mat)
-        timed = step >= args.warmup
-        log("%s step %d: %.3f s  %.2f TFLOP/s  resid %.4f  trailing GEMM "
-            "%.2f TFLOP/s over %d launches" % (
-                "timed" if timed else "warmup", step, t,
-                hpl_flops(n) / t / 1e12, chk["resid"],
-                st[2] / (st[0] * 1e-3) / 1e12 if st[0] > 0 else 0.0, st[1]))
-        if not chk["resid"] < 16.0:
-            raise SystemExit("residual check FAILED: %r" % chk)
-        if timed:
-            times.append(t)
-            checks.append(chk)
-            stats.append(st)
-
-    pivots = None
-    if args.check_pivots > 0:
-        import scipy.linalg.lapack as la
-
-        from oracle import oracle as o
-        k = min(args.check_pivots, n)
-        log("prefix pivot check: first %d pivots vs LAPACK dgetrf" % k)
-        gpiv = ctx.download_vec(mat.ipiv, k, np.int32)
-        pref = np.asfortranarray(o.matgen_tile(n, 0, 0, n, k))
-        _, lpiv, _ = la.dgetrf(pref, overwrite_a=1)
-        pivots = {"prefix": k, "equal": bool(np.array_equal(gpiv, lpiv[:k]))}
-    ctx.pmat_free(mat)
+    out = {"metric": METRIC}
+    fp64 = None
+    if args.mode in ("all", "fp64"):
+        fp64 = run_fp64(ctx, rp, args)
+    mixed = {}
+    for form in ("bf16", "fp8"):
+        if args.mode in ("all", form):
+            mixed[form] = run_mixed(ctx, rp, form, args)
     ctx.close()
 
-    t_mean = sum(times) / len(times)
-    value = hpl_flops(n) / t_mean / 1e12
-    gemm_ms = sum(s[0] for s in stats)
-    gemm_launch = sum(s[1] for s in stats)
-    gemm_flops = sum(s[2] for s in stats)
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
-    traffic = load_traffic(n, nb)
-    out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(t_mean * 1e3, 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: HPL 64-bit LCG matgen, seed 100, uniform "
-                "[-0.5,0.5], b = column N",
-        "config": {"workload": "hpl_pdgesv_fp64_p1q1", "N": n, "NB": nb,
-                   "P": 1, "Q": 1, "pfact": "right", "rfact": "right",
-                   "nbmin": 32, "ndiv": 2, "depth": 1, "swap": "long",
-                   "frac": args.frac},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": traffic,
-                     "kernel": "trailing-update FP64 MFMA GEMM "
-                               "(A22 -= L21*U12), %d launches, %.1f ms"
-                               % (gemm_launch, gemm_ms),
-                     "avg_launch_ms": round(gemm_ms / max(gemm_launch, 1), 4),
-                     "flops_per_launch": gemm_flops / max(gemm_launch, 1)},
-        "residual": round(max(c["resid"] for c in checks), 6),
-        "residual_r0_form": round(max(c["resid_r0_form"] for c in checks), 6),
-        "tflops_ref_formula": round(ref_flops(n) / t_mean / 1e12, 3),
-        "pivot_prefix_check": pivots,
-    }
+    if fp64 is not None:
+        head = fp64
+        out.update({
+            "value": round(fp64["value"], 3), "unit": "TFLOP/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(fp64["t_mean"] * 1e3, 2),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic: HPL 64-bit LCG matgen, seed 100, uniform "
+                    "[-0.5,0.5], b = column N",
+            "config": head["config"], "roofline": head["roofline"]})
+        for k in ("residual", "residual_r0_form", "tflops_ref_formula",
+                  "pivot_prefix_check"):
+            out[k] = head[k]
+    else:
+        form = args.mode
+        m = mixed[form]
+        ur = m["update_roofline"]
+        bound = ur["bound"]
+        out.update({
+            "value": m["value"], "unit": "TFLOP/s", "n_gpus": 1, "steps": 1,
+            "warmup": args.mixed_warmup, "ms_per_step": m["ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": {"bf16": "bf16+f32+f64", "fp8": "fp8+f16+f32+f64"}[form],
+            "data": "synthetic: HPL 64-bit LCG matgen, seed 100",
+            "config": m["config"],
+            "roofline": {
+                "bound": bound,
+                "achieved": ur["mfma_achieved"] if bound == "mfma"
+                else ur["hbm_achieved_gbs"],
+                "peak": ur["mfma_peak"] if bound == "mfma" else HBM_PEAK_GBS,
+                "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
+                "frac": ur["mfma_frac"] if bound == "mfma" else ur["hbm_frac"],
+                "traffic": None,
+                "kernel": "low-precision trailing-update GEMM"}})
+    for form, m in mixed.items():
+        if fp64 is not None:
+            m["vs_fp64"] = round(m["value"] / fp64["value"], 3)
+        out[form] = m
     if not args.no_cpu:
         log("CPU baseline sample N=%d" % args.cpu_n)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_nb, _CORES)
