@@ -27,10 +27,15 @@ constexpr int kChkRows = 4096; // rows per partial-sum chunk (column pass)
 // RN_fp16(RN_fp32(v)) of the FP64 generator value v.
 template <typename T>
 __device__ inline T store_as(double v) {
-  if constexpr(sizeof(T) == 2)
-    return __float2half(static_cast<float>(v));
-  else
+  if constexpr(sizeof(T) == 2) {
+    float f = static_cast<float>(v);
+    // keep the two roundings: the compiler may not merge them into one
+    // double -> half rounding (which differs on 1 in 2^13 values)
+    asm volatile("" : "+v"(f));
+    return __float2half(f);
+  } else {
     return static_cast<T>(v);
+  }
 }
 
 template <typename T>

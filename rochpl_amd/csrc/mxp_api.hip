@@ -237,7 +237,7 @@ extern "C" {
 void hplx_mxp_opts_default(hplx_mxp_opts* o) {
   if(!o) return;
   o->update    = HPLX_UPDATE_BF16;
-  o->terms     = 2;
+  o->terms     = 3;
   o->max_its   = 50;
   o->reserved  = 0;
   o->threshold = 16.0;

@@ -175,8 +175,19 @@ def test_mxp_getrs_and_residual(ctx, storage, update, n, nb, shift):
         if storage == rp.HPLX_STORE_FP32:
             assert np.array_equal(A0, A64.astype(np.float32).astype(np.float64))
         else:
-            assert np.array_equal(
-                A0, A64.astype(np.float32).astype(np.float16).astype(np.float64))
+            want = A64.astype(np.float32).astype(np.float16).astype(np.float64)
+            bad = np.argwhere(A0 != want)
+            if len(bad):
+                one = A64.astype(np.float16).astype(np.float64)
+                sub = np.abs(want[tuple(bad.T)]) < 2.0 ** -14
+                print("[mxp] fp16 matgen: %d mismatches, %d subnormal, %d equal "
+                      "to a single double->half rounding; first %s: gpu %r "
+                      "want %r fp64 %r" % (
+                          len(bad), int(sub.sum()),
+                          int(np.sum(A0[tuple(bad.T)] == one[tuple(bad.T)])),
+                          bad[0], A0[tuple(bad[0])], want[tuple(bad[0])],
+                          A64[tuple(bad[0])]), flush=True)
+            assert len(bad) == 0
         assert ctx.mxp_pdgetrf(rp.palg_default(), opts, mat) == 0
         F = download_factors(ctx, mat)
         ipiv = ctx.download_vec(mat.ipiv, n, np.int32)
@@ -191,10 +202,13 @@ def test_mxp_getrs_and_residual(ctx, storage, update, n, nb, shift):
         res = ctx.mxp_residual(mat, d.ptr)
         ref = o.residual(n, x, diag_shift=shift)
         d.free()
-        assert abs(res["rnorm"] - ref["rnorm"]) <= 1e-12 * ref["rnorm"] + 1e-300
-        assert abs(res["resid"] - ref["resid"]) <= 1e-10 * ref["resid"]
+        # both sum b - A x in FP64 in different orders: they may differ by
+        # about eps * ||A||_oo ||x||_oo, whatever the size of r itself
+        scale = np.finfo(float).eps * ref["AnormI"] * ref["XnormI"]
+        assert abs(res["rnorm"] - ref["rnorm"]) <= 4 * scale
         assert res["AnormI"] == pytest.approx(ref["AnormI"], rel=1e-13)
         assert res["BnormI"] == ref["BnormI"]
+        assert res["XnormI"] == ref["XnormI"]
     finally:
         ctx.mxp_free(mat)
 
@@ -228,11 +242,18 @@ def test_mxp_pdgesv_converges(ctx, storage, update, terms, n, nb, shift):
     assert rc == 0 and res["converged"] == 1
     ref = o.residual(n, x, diag_shift=shift)
     assert ref["resid"] < 16.0
-    assert ref["resid"] == pytest.approx(res["resid"], rel=1e-8)
+    # the device's and the oracle's ||b - A x||_oo differ by rounding of
+    # order eps ||A|| ||x||, i.e. well below one unit of the scaled residual
+    assert abs(ref["resid"] - res["resid"]) <= 0.1
     r64 = o.pdgesv(n, nb, diag_shift=shift)
-    err = np.linalg.norm(x - r64["x"]) / np.linalg.norm(r64["x"])
-    print("[mxp] ||x - x_fp64|| / ||x_fp64|| = %.3e" % err, flush=True)
-    assert err < 1e-9
+    err = np.linalg.norm(x - r64["x"], np.inf) / np.linalg.norm(r64["x"], np.inf)
+    # forward error <= cond * backward error; the scaled residual s bounds
+    # the backward error by s * n * eps (for both x and the FP64 solution)
+    kappa = np.linalg.cond(o.matgen(n, n, diag_shift=shift), np.inf)
+    bound = kappa * (max(res["resid"], 0.1) + 0.1) * n * np.finfo(float).eps
+    print("[mxp] ||x - x_fp64|| / ||x_fp64|| = %.3e (cond_oo %.3g, bound %.3e)"
+          % (err, kappa, bound), flush=True)
+    assert err <= bound
     if shift >= n / 4:
         # diagonally dominant: partial pivoting keeps every diagonal pivot,
         # as the FP64 oracle does
