@@ -84,6 +84,91 @@ struct TileLoader {
   static constexpr int NQ = VEC2 ? 4 : 8;
   V    r[8];
   bool v[NQ];
+  // Fast path (a full tile and a full stage): per-lane byte offsets from a
+  // uniform stage base, fixed for the whole tile, so a stage's loads need no
+  // address arithmetic, clamping or masking.
+  uint32_t    off[NQ];
+  const char* base;    // the tile's first element (uniform)
+  size_t      kbytes;  // bytes per unit of k in the stage base (uniform)
+
+  __device__ void init_fast(const V* __restrict__ P, int ld, int o0) {
+    const int t = threadIdx.x;
+    if(!KFAST) {
+      base   = reinterpret_cast<const char*>(P + o0);
+      kbytes = static_cast<size_t>(ld) * sizeof(V);
+    } else {
+      base   = reinterpret_cast<const char*>(P + static_cast<size_t>(o0) * ld);
+      kbytes = sizeof(V);
+    }
+#pragma unroll
+    for(int q = 0; q < NQ; ++q) {
+      int o, k;
+      if(!KFAST && VEC2) {
+        o = 2 * (t & 63);
+        k = (t >> 6) + 4 * q;
+      } else if(!KFAST) {
+        o = t & 127;
+        k = (t >> 7) + 2 * q;
+      } else if(VEC2) {
+        o = (t >> 3) + 32 * q;
+        k = 2 * (t & 7);
+      } else {
+        o = (t >> 4) + 16 * q;
+        k = t & 15;
+      }
+      const size_t e = KFAST ? static_cast<size_t>(k) + static_cast<size_t>(o) * ld
+                             : static_cast<size_t>(o) + static_cast<size_t>(k) * ld;
+      off[q] = static_cast<uint32_t>(e * sizeof(V));
+    }
+  }
+
+  __device__ void load_fast(int k0) {
+    const char* s = base + static_cast<size_t>(k0) * kbytes;
+#pragma unroll
+    for(int q = 0; q < NQ; ++q) {
+      if(VEC2) {
+        const vec2_t<V> x = *reinterpret_cast<const vec2_t<V>*>(s + off[q]);
+        r[2 * q]          = x.x;
+        r[2 * q + 1]      = x.y;
+      } else {
+        r[q] = *reinterpret_cast<const V*>(s + off[q]);
+      }
+    }
+  }
+
+  __device__ void store_fast(V* S, bool neg) const {
+    const int t = threadIdx.x;
+    if(!KFAST && VEC2) {
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const int k = (t >> 6) + 4 * q;
+        vec2_t<V> x;
+        x.x = neg ? -r[2 * q] : r[2 * q];
+        x.y = neg ? -r[2 * q + 1] : r[2 * q + 1];
+        *reinterpret_cast<vec2_t<V>*>(S + k * LDS_LD + 2 * (t & 63)) = x;
+      }
+    } else if(!KFAST) {
+#pragma unroll
+      for(int q = 0; q < 8; ++q) {
+        const int k = (t >> 7) + 2 * q;
+        S[k * LDS_LD + (t & 127)] = neg ? -r[q] : r[q];
+      }
+    } else if(VEC2) {
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const int o = (t >> 3) + 32 * q;
+        const int k = 2 * (t & 7);
+        S[k * LDS_LD + o]       = neg ? -r[2 * q] : r[2 * q];
+        S[(k + 1) * LDS_LD + o] = neg ? -r[2 * q + 1] : r[2 * q + 1];
+      }
+    } else {
+#pragma unroll
+      for(int q = 0; q < 8; ++q) {
+        const int o = (t >> 4) + 16 * q;
+        S[(t & 15) * LDS_LD + o] = neg ? -r[q] : r[q];
+      }
+    }
+  }
 
   __device__ void load(const V* __restrict__ P, int ld, int o0, int O,
                        int k0, int kend) {
@@ -214,13 +299,30 @@ __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
 
   TileLoader<V, TA, VA>  la;   // A: o = row of C; k-contiguous when TA
   TileLoader<V, !TB, VB> lb;   // B: o = column of C; k-contiguous unless TB
+  // interior tiles take the fast path for every full stage
+  const bool full_tile = m0 + BM <= M && n0 + BN <= N;
+  if(full_tile) {
+    la.init_fast(A, lda, m0);
+    lb.init_fast(B, ldb, n0);
+  }
+  auto fast_stage = [&](int k0) { return full_tile && k0 + BK <= kend; };
   auto load_stage = [&](int k0) {
-    la.load(A, lda, m0, M, k0, kend);
-    lb.load(B, ldb, n0, N, k0, kend);
+    if(fast_stage(k0)) {
+      la.load_fast(k0);
+      lb.load_fast(k0);
+    } else {
+      la.load(A, lda, m0, M, k0, kend);
+      lb.load(B, ldb, n0, N, k0, kend);
+    }
   };
-  auto store_stage = [&](int buf) {
-    la.store(sA[buf], NEG);
-    lb.store(sB[buf], false);
+  auto store_stage = [&](int buf, int k0) {
+    if(fast_stage(k0)) {
+      la.store_fast(sA[buf], NEG);
+      lb.store_fast(sB[buf], false);
+    } else {
+      la.store(sA[buf], NEG);
+      lb.store(sB[buf], false);
+    }
   };
 
   if(kend > 0) load_stage(0);
@@ -233,7 +335,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
 #pragma unroll
     for(int j = 0; j < TN; ++j) acc[i][j] = vec4_t<V>{0, 0, 0, 0};
 
-  if(kend > 0) store_stage(0);
+  if(kend > 0) store_stage(0, 0);
   __syncthreads();
 
   int buf = 0;
@@ -264,7 +366,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_kernel(
           // D = B^T-block x A^T-block: D[n][m] accumulates C[m][n]
           acc[i][j] = mfma16x16x4<V>(fb[cur][j], fa[cur][i], acc[i][j]);
     }
-    if(more) store_stage(buf ^ 1);
+    if(more) store_stage(buf ^ 1, k0 + BK);
     __syncthreads();
     buf ^= 1;
   }

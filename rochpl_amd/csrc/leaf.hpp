@@ -10,9 +10,10 @@
 //   2. it publishes the candidate's value, row index and 32-wide row segment
 //      in its slot, then a sequence number with release ordering; workgroup
 //      0, which holds the current top row, publishes that row likewise;
-//   3. every workgroup polls all slots' sequence numbers, then reads all
-//      candidates (values and row segments) in one pass into LDS, reduces
-//      them itself and takes the winner's row from LDS;
+//   3. one wave of every workgroup polls all slots' sequence numbers and
+//      takes one acquire fence; then the workgroup reads all candidates
+//      (values and row segments) in one pass into LDS, reduces them itself
+//      and takes the winner's row from LDS;
 //   4. registers are updated: row interchange, division by the pivot,
 //      rank-1 update of the remaining leaf columns.
 // Slots are double-buffered by column parity, so a workgroup that runs ahead
@@ -183,20 +184,26 @@ __device__ __attribute__((noinline)) bool exchange(const LeafArgs* ap, int jj,
     // the top row c0 + jj < c0 + 32 always lives in workgroup 0
     tslot[kSlotData + (t - 64)] = sh.top[t - 64];
   }
-  __threadfence();  // this thread's slot stores reach the coherence point
+  // The workgroup barrier orders every wave's slot stores before thread 0's
+  // release store of the sequence number, so one release per slot (one L2
+  // write-back) publishes them all; no per-wave fence is needed.
   __syncthreads();
   if(t == 0) {
     st_seq(slot + kSlotSeq, seq);
     if(b == 0) st_seq(tslot + kSlotSeq, seq);
   }
-  // ---- wait for every candidate and the top row, then read them all
+  // ---- wave 0 waits for every candidate and the top row (lane l polls
+  // slots l, l+64, ...; lane 0 also the top row), then takes one acquire
+  // fence; the barrier extends it to the other waves, which then read the
+  // candidates.
   int ok = 1;
-  if(t < G)
-    ok = poll_slot(a.slots + (par + t) * kSlotStride, seq, a.err);
-  else if(t == G)
-    ok = poll_slot(tslot, seq, a.err);
+  if(t < 64) {
+    for(int g = t; g < G && ok; g += 64)
+      ok = poll_slot(a.slots + (par + g) * kSlotStride, seq, a.err);
+    if(ok && t == 0) ok = poll_slot(tslot, seq, a.err);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
   if(!__syncthreads_and(ok)) return false;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   for(int i = t; i < G * kLeafMax; i += kLeafThreads) {
     const int g = i / kLeafMax, c = i % kLeafMax;
     sh.cand[g][c] = a.slots[(par + g) * kSlotStride + kSlotData + c];
