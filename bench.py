@@ -309,7 +309,15 @@ def main():
     mixed = {}
     for form in ("bf16", "fp8"):
         if args.mode in ("all", form):
-            mixed[form] = run_mixed(ctx, rp, form, args)
+            try:
+                mixed[form] = run_mixed(ctx, rp, form, args)
+            except (SystemExit, rp.HplxError) as e:
+                # a mixed form that fails is reported, not silently dropped;
+                # the FP64 line stands on its own
+                if args.mode == form:
+                    raise
+                log("%s form failed: %s" % (form, e))
+                mixed[form] = {"error": str(e)}
     ctx.close()
 
     if fp64 is not None:
@@ -348,7 +356,7 @@ def main():
                 "traffic": None,
                 "kernel": "low-precision trailing-update GEMM"}})
     for form, m in mixed.items():
-        if fp64 is not None:
+        if fp64 is not None and "value" in m:
             m["vs_fp64"] = round(m["value"] / fp64["value"], 3)
         out[form] = m
     if not args.no_cpu:

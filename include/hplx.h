@@ -195,7 +195,8 @@ int hplx_dlaswp00N(hplx_ctx* ctx, int jb, int n, double* A, int lda,
  * with HPL_pdlaswp_end (src/pgesv/HPL_pdlaswp.cpp:386-389): for the panel
  * whose top-left element is A[j + j*lda] (width jb, m = rows from j down)
  * and its n trailing columns: swap rows with permU, U12 = L11^{-1} A12,
- * A22 -= L21 * U12.  W is caller scratch of jb*n doubles (device). */
+ * A22 -= L21 * U12.  W is caller scratch of jb*(n+4) doubles (device): the
+ * gathered rows, then U12 transposed for the update. */
 int hplx_pdupdate(hplx_ctx* ctx, double* A, int lda, int m, int jb, int n,
                   const int* permU, double* W);
 

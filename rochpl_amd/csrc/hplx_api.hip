@@ -306,9 +306,22 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
                                    0.0, A12, lda, true));
       if(m - jb > 0) {
         hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
-        HIPCHK(c, hipEventRecord(e0, s));
-        HIPCHK(c, hplx::launch_dgemm(s, m - jb, nt, jb, -1.0, P + jb, lda, A12,
-                                     lda, 1.0, A12 + jb, lda, false, 1));
+        // U12^T into W (free again), then the streamed update; shapes the
+        // update kernel does not take (K not a multiple of 8) use the
+        // general GEMM
+        const int ldu    = hplx::ut_ld(nt);
+        const int a_rows = lda - j - jb;  // rows readable below P + jb
+        if(hplx::update_fast_ok(m - jb, nt, jb, P + jb, lda, ws->W, ldu,
+                                a_rows, ldu)) {
+          HIPCHK(c, hplx::launch_transpose(s, jb, nt, A12, lda, ws->W, ldu));
+          HIPCHK(c, hipEventRecord(e0, s));
+          HIPCHK(c, hplx::launch_update(s, m - jb, nt, jb, P + jb, lda, ws->W,
+                                        ldu, A12 + jb, lda, a_rows, ldu));
+        } else {
+          HIPCHK(c, hipEventRecord(e0, s));
+          HIPCHK(c, hplx::launch_dgemm(s, m - jb, nt, jb, -1.0, P + jb, lda,
+                                       A12, lda, 1.0, A12 + jb, lda, false, 1));
+        }
         HIPCHK(c, hipEventRecord(e1, s));
         c.ev_pairs++;
         c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
@@ -520,9 +533,19 @@ int hplx_pdupdate(hplx_ctx* ctx, double* A, int lda, int m, int jb, int n,
   HIPCHK(c, hplx::launch_swap_gather(s, jb, n, A12, lda, permU, W, jb));
   HIPCHK(c, hplx::launch_dgemm(s, jb, n, jb, 1.0, c.Linv, jb, W, jb, 0.0, A12,
                                lda, true));
-  if(m > jb)
-    HIPCHK(c, hplx::launch_dgemm(s, m - jb, n, jb, -1.0, A + jb, lda, A12, lda,
-                                 1.0, A12 + jb, lda, false));
+  if(m > jb) {
+    // the same two paths as the panel loop: U12^T into W and the streamed
+    // update, or the general GEMM
+    const int ldu = hplx::ut_ld(n);
+    if(hplx::update_fast_ok(m - jb, n, jb, A + jb, lda, W, ldu, lda - jb, ldu)) {
+      HIPCHK(c, hplx::launch_transpose(s, jb, n, A12, lda, W, ldu));
+      HIPCHK(c, hplx::launch_update(s, m - jb, n, jb, A + jb, lda, W, ldu,
+                                    A12 + jb, lda, lda - jb, ldu));
+    } else {
+      HIPCHK(c, hplx::launch_dgemm(s, m - jb, n, jb, -1.0, A + jb, lda, A12,
+                                   lda, 1.0, A12 + jb, lda, false));
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(s));
   return HPLX_SUCCESS;
 }

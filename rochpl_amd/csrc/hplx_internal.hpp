@@ -128,6 +128,22 @@ hipError_t launch_sgemm(hipStream_t s, int m, int n, int k, double alpha,
                         const float* A, int lda, const float* B, int ldb,
                         double beta, float* C, int ldc, bool lower_a);
 
+// The trailing update C -= A * Bt^T with Bt = U12^T stored n-contiguous
+// (k_update.hip): K a multiple of 8, 16-byte aligned operands, even leading
+// dimensions; a_rows / b_cols bound the (clamped) reads of A's columns and
+// Bt's rows.  update_fast_ok says whether a call qualifies.
+bool       update_fast_ok(int m, int n, int k, const double* A, int lda,
+                          const double* Bt, int ldb, int a_rows, int b_cols);
+hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,
+                         int lda, const double* Bt, int ldb, double* C,
+                         int ldc, int a_rows, int b_cols);
+// B (cols x rows, ldb) = A^T for A (rows x cols, lda), column-major both.
+hipError_t launch_transpose(hipStream_t s, int rows, int cols, const double* A,
+                            int lda, double* B, int ldb);
+// Leading dimension of the transposed U12 of n columns (even, with room for
+// the clamped pair reads).
+inline int ut_ld(int n) { return (n + 3) & ~1; }
+
 hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                        double* x);
 

@@ -1,0 +1,222 @@
+// The FP64 trailing update A22 -= L21 * U12 (rocblas_dgemm of
+// HPL_pdupdateTT, src/pgesv/HPL_pdupdateTT.cpp:131-144), with U12 given
+// transposed (n-contiguous), as the reference also multiplies by a
+// transposed copy of U (HPL_dlatcpy at :107).
+//
+// Both operands are then contiguous along the tile edge, which lets every
+// stage move from HBM straight into LDS without passing through registers:
+//   - 128 x 128 workgroup tile, 4 waves each 64 x 64 = 4 x 4 blocks of
+//     V_MFMA_F64_16X16X4_F64, the transposed block computed so that 16 lanes
+//     store 16 consecutive rows of a C column;
+//   - K in 8-deep stages through a ring of 4 LDS slots (66.5 KB, two
+//     workgroups per CU); each stage is 16 GLOBAL_LOAD_LDS_DWORDX4 (one
+//     128-double LDS row each, rows padded to 130 doubles), issued three
+//     stages ahead of its use;
+//   - per stage one wait for this wave's loads of the stage and one
+//     workgroup barrier, after which the slot of the previous stage is
+//     refilled;
+//   - rows of L21 / columns of U12^T beyond the matrix are read from a
+//     clamped (in-bounds) address: they only reach C entries outside the
+//     matrix, which are not stored, so no masking is needed; K must be a
+//     multiple of 8 (HPL's NB);
+//   - workgroup -> tile mapping XCD-aware (8 XCDs, groups of 8 tile rows).
+//
+// The stage loads are issued from inline assembly: with the compiler's
+// builtin the waitcnt pass cannot tell which LDS slot a load writes and
+// waits for all of them before every LDS read, which would serialise the
+// ring.  The kernel waits explicitly instead; no compiler-issued vector
+// memory access is in flight inside the K loop, and all stage loads are
+// drained before the epilogue.
+#include "hplx_internal.hpp"
+
+namespace hplx {
+namespace {
+
+constexpr int UBK  = 8;    // K per stage
+constexpr int UNS  = 4;    // stages in the LDS ring
+constexpr int UROW = 130;  // LDS row pitch in doubles (128 + 2)
+constexpr int kUXcds = 8, kUGroupM = 8;
+
+typedef double ud4 __attribute__((ext_vector_type(4)));
+
+// 16 bytes per lane from global memory into LDS: lane l writes lds + 16 l.
+__device__ inline void lds_load16(const double* g, double* lds) {
+  const uint32_t l = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(
+      (__attribute__((address_space(3))) double*)(lds)));
+  const uint32_t u = __builtin_amdgcn_readfirstlane(l);
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+               :
+               : "s"(u), "v"(g)
+               : "memory", "m0");
+}
+
+// s_waitcnt vmcnt(n) with the other counters left alone (gfx9 encoding).
+template <int N>
+__device__ inline void wait_vm() {
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) |
+                             (0xF << 8));
+}
+
+#define HPLX_USLOT(n)                                             \
+  __shared__ __attribute__((aligned(16))) double sA##n[UBK][UROW]; \
+  __shared__ __attribute__((aligned(16))) double sB##n[UBK][UROW];
+
+// C (M x N) -= A (M x K, column-major) * Bt^T (Bt: K rows of N, row-major,
+// i.e. column-major N x K).  a_rows / b_cols: readable extent of A's columns
+// / Bt's rows from the given base (clamp bound for edge tiles).
+__global__ __launch_bounds__(256, 2) void update_kernel(
+    int M, int N, int K, const double* __restrict__ A, int lda,
+    const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
+    int tiles_m, int tiles_n, int a_rows, int b_cols) {
+  HPLX_USLOT(0) HPLX_USLOT(1) HPLX_USLOT(2) HPLX_USLOT(3)
+
+  const int T = tiles_m * tiles_n, per = gridDim.x / kUXcds, bid = blockIdx.x;
+  const int tile = (bid % kUXcds) * per + bid / kUXcds;
+  if(tile >= T) return;
+  const int width = kUGroupM * tiles_n, group = tile / width;
+  const int firstm = group * kUGroupM, gsz = min(tiles_m - firstm, kUGroupM);
+  const int within = tile - group * width;
+  const int m0 = (firstm + within % gsz) * 128, n0 = (within / gsz) * 128;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  // wave w loads k rows 2w and 2w+1 of every stage, lane l the pair 2l
+  const int oa = min(m0 + 2 * lane, a_rows - 2);
+  const int ob = min(n0 + 2 * lane, b_cols - 2);
+  const double* pa0 = A + oa + static_cast<size_t>(2 * wave) * lda;
+  const double* pb0 = Bt + ob + static_cast<size_t>(2 * wave) * ldb;
+  const int nst = K / UBK;
+  // past the last stage the last one is loaded again, so every iteration
+  // waits for the same number of outstanding loads
+  auto issue = [&](int st, double (*a)[UROW], double (*b)[UROW]) {
+    const int s = st < nst ? st : nst - 1;
+    const double* pa = pa0 + static_cast<size_t>(s) * UBK * lda;
+    const double* pb = pb0 + static_cast<size_t>(s) * UBK * ldb;
+    lds_load16(pa, &a[2 * wave][0]);
+    lds_load16(pa + lda, &a[2 * wave + 1][0]);
+    lds_load16(pb, &b[2 * wave][0]);
+    lds_load16(pb + ldb, &b[2 * wave + 1][0]);
+  };
+  ud4 acc[4][4];
+#pragma unroll
+  for(int i = 0; i < 4; ++i)
+#pragma unroll
+    for(int j = 0; j < 4; ++j) acc[i][j] = ud4{0.0, 0.0, 0.0, 0.0};
+  auto compute = [&](double (*a)[UROW], double (*b)[UROW]) {
+#pragma unroll
+    for(int ks = 0; ks < UBK / 4; ++ks) {
+      double fa[4], fb[4];
+#pragma unroll
+      for(int i = 0; i < 4; ++i) fa[i] = a[ks * 4 + lk][wm0 + i * 16 + l16];
+#pragma unroll
+      for(int j = 0; j < 4; ++j) fb[j] = b[ks * 4 + lk][wn0 + j * 16 + l16];
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i],
+                                                           acc[i][j], 0, 0, 0);
+    }
+  };
+
+  issue(0, sA0, sB0);
+  issue(1, sA1, sB1);
+  issue(2, sA2, sB2);
+  // stage s lives in slot s % 4; unrolled by 4 so every slot is static
+  for(int s = 0; s < nst; s += UNS) {
+#define HPLX_USTEP(q, ca, cb, na, nb)                                        \
+  if(s + q < nst) {                                                          \
+    wait_vm<4 * (UNS - 2)>(); /* this wave's loads of stage s+q are in */    \
+    __builtin_amdgcn_s_barrier(); /* ... and every wave's; slot of s+q-1 free */ \
+    issue(s + q + UNS - 1, na, nb);                                          \
+    compute(ca, cb);                                                         \
+  }
+    HPLX_USTEP(0, sA0, sB0, sA3, sB3)
+    HPLX_USTEP(1, sA1, sB1, sA0, sB0)
+    HPLX_USTEP(2, sA2, sB2, sA1, sB1)
+    HPLX_USTEP(3, sA3, sB3, sA2, sB2)
+#undef HPLX_USTEP
+  }
+  wait_vm<0>();  // the trailing reloads
+
+  // D[4r + l/16][l%16] of block (i, j) is C[row = l16][col = 4r + lk]
+#pragma unroll
+  for(int i = 0; i < 4; ++i) {
+    const int row = m0 + wm0 + i * 16 + l16;
+    if(row >= M) continue;
+#pragma unroll
+    for(int j = 0; j < 4; ++j)
+#pragma unroll
+      for(int r = 0; r < 4; ++r) {
+        const int col = n0 + wn0 + j * 16 + 4 * r + lk;
+        if(col < N) {
+          double* c = C + row + static_cast<size_t>(col) * ldc;
+          *c        = *c - acc[i][j][r];
+        }
+      }
+  }
+}
+#undef HPLX_USLOT
+
+// B (cols x rows, row-major with ldb) = A^T for A (rows x cols, column-major
+// with lda); 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void transpose_kernel(
+    int rows, int cols, const double* __restrict__ A, int lda,
+    double* __restrict__ B, int ldb, int gx) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x % gx, by = blockIdx.x / gx;
+  const int r0 = bx * 32, c0 = by * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for(int q = 0; q < 4; ++q) {
+    const int c = c0 + ty + 8 * q, r = r0 + tx;
+    if(r < rows && c < cols) tile[ty + 8 * q][tx] = A[r + static_cast<size_t>(c) * lda];
+  }
+  __syncthreads();
+#pragma unroll
+  for(int q = 0; q < 4; ++q) {
+    const int r = r0 + ty + 8 * q, c = c0 + tx;
+    if(r < rows && c < cols) B[c + static_cast<size_t>(r) * ldb] = tile[tx][ty + 8 * q];
+  }
+}
+
+}  // namespace
+
+bool update_fast_ok(int m, int n, int k, const double* A, int lda,
+                    const double* Bt, int ldb, int a_rows, int b_cols) {
+  const auto al16 = [](const void* p) {
+    return reinterpret_cast<uintptr_t>(p) % 16 == 0;
+  };
+  return m > 0 && n > 0 && k > 0 && k % UBK == 0 && al16(A) && al16(Bt) &&
+         lda % 2 == 0 && ldb % 2 == 0 && a_rows >= 2 && b_cols >= 2 &&
+         a_rows >= m && b_cols >= n;
+}
+
+hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,
+                         int lda, const double* Bt, int ldb, double* C, int ldc,
+                         int a_rows, int b_cols) {
+  if(!update_fast_ok(m, n, k, A, lda, Bt, ldb, a_rows, b_cols))
+    return hipErrorInvalidValue;
+  const int  tiles_m = (m + 127) / 128, tiles_n = (n + 127) / 128;
+  const long T       = static_cast<long>(tiles_m) * tiles_n;
+  const long grid    = (T + kUXcds - 1) / kUXcds * kUXcds;
+  if(grid > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(update_kernel, dim3(static_cast<unsigned>(grid)),
+                     dim3(256), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
+                     tiles_m, tiles_n, a_rows, b_cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(hipStream_t s, int rows, int cols, const double* A,
+                            int lda, double* B, int ldb) {
+  if(rows <= 0 || cols <= 0) return hipSuccess;
+  const long gx = (rows + 31) / 32, gy = (cols + 31) / 32;
+  if(gx * gy > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(transpose_kernel, dim3(static_cast<unsigned>(gx * gy)),
+                     dim3(256), 0, s, rows, cols, A, lda, B, ldb,
+                     static_cast<int>(gx));
+  return hipGetLastError();
+}
+
+}  // namespace hplx

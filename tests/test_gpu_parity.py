@@ -161,10 +161,14 @@ def test_pipid_and_dlaswp_match_oracle(ctx, jb, m, seed):
 # ---------------------------------------------------------------------------
 # Trailing update: HPL_pdlaswp_end + HPL_pdupdateTT
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("m,jb,n", [(2000, 256, 1500), (900, 96, 901),
-                                    (896, 896, 1), (3000, 512, 333)])
-def test_pdupdate_matches_oracle(ctx, m, jb, n):
-    say("pdupdate m=%d jb=%d n=%d" % (m, jb, n))
+# pad: lda = m + pad.  Even leading dimensions with jb % 8 == 0 take the
+# streamed update kernel, odd ones (and other jb) the general GEMM.
+@pytest.mark.parametrize("m,jb,n,pad", [(2000, 256, 1500, 7), (900, 96, 901, 7),
+                                        (896, 896, 1, 7), (3000, 512, 333, 7),
+                                        (2048, 256, 1000, 8), (1001, 64, 777, 5),
+                                        (4000, 896, 3001, 32), (300, 100, 50, 4)])
+def test_pdupdate_matches_oracle(ctx, m, jb, n, pad):
+    say("pdupdate m=%d jb=%d n=%d lda=m+%d" % (m, jb, n, pad))
     A0 = np.asfortranarray(o.matgen_tile(m, 0, 0, m, jb + n, seed=11))
     # factor the panel on the CPU, then update on both sides
     A = A0.copy(order="F")
@@ -172,10 +176,10 @@ def test_pdupdate_matches_oracle(ctx, m, jb, n):
     ipiv, info = o.pdfact(P)
     A[:, :jb] = P
     permU = o.pipid(ipiv)
-    dA, lda = upload_matrix(ctx, A, m + 7)
+    dA, lda = upload_matrix(ctx, A, m + pad)
     dperm = ctx.alloc(8 * jb)
     dperm.upload(permU)
-    dW = ctx.alloc(8 * jb * n)
+    dW = ctx.alloc(8 * jb * (n + 4))
     ctx.pdupdate(dA.ptr, lda, m, jb, n, dperm.ptr, dW.ptr)
     got = dA.download((lda, jb + n), np.float64)[:m]
     # oracle: swap, L11^{-1}, A22 -= L21 U12
