@@ -30,7 +30,9 @@
 //
 // Tile: BM x BN = 64*WM x 64*WN with WM x WN waves, each wave 64 x 64 as
 // 4 x 4 blocks of 16 x 16.  The MFMA computes the transposed block
-// (D = B^T A^T) so that 16 lanes hold 16 consecutive rows of one C column.
+// (D = B^T A^T) so that 16 lanes hold 16 consecutive rows of one C column;
+// the C update of full tiles goes through LDS as 4-row vectors per lane
+// (epilogue_vec).
 // The next stage's global loads are in flight while the current stage's
 // MFMAs run; LDS is single-buffered (two barriers per stage).  Workgroup ids
 // are remapped XCD-aware as in the FP64 GEMM.
@@ -304,6 +306,60 @@ __device__ inline bool map_tile(int tiles_m, int tiles_n, int& tm, int& tn) {
   return true;
 }
 
+// Epilogue C -= acc for one wave's 64 x 64 sub-tile.
+//
+// The MFMA leaves lane l with rows l % 16 of four different columns, so a
+// direct update touches every column in 64-byte (FP32) or 32-byte (FP16)
+// pieces with 128 instructions per lane.  Instead the sub-tile goes through
+// a per-wave LDS scratch E (32 columns x kEpiPitch floats, two passes of 32
+// columns): each lane then reads 4 consecutive rows of one column, and C is
+// loaded and stored as 16-byte (FP32) / 8-byte (FP16) vectors, 256 (128)
+// contiguous bytes per column per instruction.  Full, aligned tiles only;
+// the others take the element-wise path.
+constexpr int kEpiPitch = 68;                       // floats per LDS column
+constexpr int kEpiWaveFloats = 32 * kEpiPitch;      // per wave
+constexpr int kEpiBytes = 4 * kEpiWaveFloats * 4;   // 4 waves
+
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4v __attribute__((ext_vector_type(4)));
+
+template <typename S>
+__device__ inline void epilogue_vec(f32x4 (&acc)[4][4], S* __restrict__ C,
+                                    int ldc, int r0, int c0, float* E) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for(int p = 0; p < 2; ++p) {
+#pragma unroll
+    for(int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+          E[(jj * 16 + 4 * lk + r) * kEpiPitch + 16 * i + l16] =
+              acc[i][2 * p + jj][r];
+    __syncthreads();
+#pragma unroll
+    for(int q = 0; q < 8; ++q) {
+      const int    cl = 4 * q + lk;  // local column 0..31
+      const int    rl = 4 * l16;     // local rows rl .. rl+3
+      const f32x4v v  = *reinterpret_cast<const f32x4v*>(&E[cl * kEpiPitch + rl]);
+      S* c = C + (r0 + rl) + static_cast<size_t>(c0 + 32 * p + cl) * ldc;
+      if constexpr(sizeof(S) == 4) {
+        f32x4v x = *reinterpret_cast<const f32x4v*>(c);
+        *reinterpret_cast<f32x4v*>(c) = x - v;
+      } else {
+        const f16x4v x = *reinterpret_cast<const f16x4v*>(c);
+        f16x4v       y;
+#pragma unroll
+        for(int e = 0; e < 4; ++e)
+          y[e] = static_cast<_Float16>(static_cast<float>(x[e]) - v[e]);
+        *reinterpret_cast<f16x4v*>(c) = y;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Pairs (a-term, b-term) of the split product, largest first.
 __device__ constexpr int kPairA[6] = {0, 0, 1, 0, 1, 2};
 __device__ constexpr int kPairB[6] = {0, 1, 0, 2, 1, 0};
@@ -312,7 +368,7 @@ template <int TERMS, int WM, int WN, typename S>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
     int M, int N, int nks, const uint16_t* __restrict__ Ap, int m_pad,
     size_t aplane, const uint16_t* __restrict__ Bp, int n_pad, size_t bplane,
-    S* __restrict__ C, int ldc, int tiles_m, int tiles_n) {
+    S* __restrict__ C, int ldc, int tiles_m, int tiles_n, int vec_ok) {
   constexpr int NT     = 64 * WM * WN;
   constexpr int BM     = 64 * WM;
   constexpr int BN     = 64 * WN;
@@ -320,8 +376,13 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
   constexpr int NPROD  = TERMS * (TERMS + 1) / 2;
   constexpr int ACH    = BM * 4 / NT;  // 16-byte chunks per thread, A, per term
   constexpr int BCH    = BN * 4 / NT;
-  __shared__ __attribute__((aligned(16))) uint16_t sA[TERMS][BM * RS];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[TERMS][BN * RS];
+  // one LDS arena: operand stages during the K loop, epilogue scratch after
+  constexpr int kOps   = TERMS * (BM + BN) * RS * 2;
+  __shared__ __attribute__((aligned(16)))
+      uint8_t smem[kOps > kEpiBytes ? kOps : kEpiBytes];
+  uint16_t(*sA)[BM * RS] = reinterpret_cast<uint16_t(*)[BM * RS]>(smem);
+  uint16_t(*sB)[BN * RS] =
+      reinterpret_cast<uint16_t(*)[BN * RS]>(smem + TERMS * BM * RS * 2);
 
   int tm, tn;
   if(!map_tile(tiles_m, tiles_n, tm, tn)) return;
@@ -393,6 +454,11 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
   }
 
   // epilogue: C -= acc; lane l16 is the row, 4*(l/16)+r the column
+  if(vec_ok && m0 + BM <= M && n0 + BN <= N) {
+    epilogue_vec(acc, C, ldc, m0 + wm0, n0 + wn0,
+                 reinterpret_cast<float*>(smem) + wave * kEpiWaveFloats);
+    return;
+  }
 #pragma unroll
   for(int i = 0; i < 4; ++i) {
     const int row = m0 + wm0 + i * 16 + l16;
@@ -415,15 +481,18 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
     int M, int N, int nks, const uint8_t* __restrict__ Ap, int m_pad,
     const int* __restrict__ ea, const uint8_t* __restrict__ Bp, int n_pad,
     const int* __restrict__ eb, int ntn128, S* __restrict__ C, int ldc,
-    int tiles_m, int tiles_n) {
+    int tiles_m, int tiles_n, int vec_ok) {
   constexpr int NT  = 64 * WM * WN;
   constexpr int BM  = 64 * WM;
   constexpr int BN  = 64 * WN;
   constexpr int RS  = 144;  // LDS row stride in bytes
   constexpr int ACH = BM * 8 / NT;  // 16-byte chunks per thread (128 B rows)
   constexpr int BCH = BN * 8 / NT;
-  __shared__ __attribute__((aligned(16))) uint8_t sA[BM * RS];
-  __shared__ __attribute__((aligned(16))) uint8_t sB[BN * RS];
+  constexpr int kOps = (BM + BN) * RS;
+  __shared__ __attribute__((aligned(16)))
+      uint8_t smem[kOps > kEpiBytes ? kOps : kEpiBytes];
+  uint8_t* sA = smem;
+  uint8_t* sB = smem + BM * RS;
 
   int tm, tn;
   if(!map_tile(tiles_m, tiles_n, tm, tn)) return;
@@ -499,6 +568,11 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
     __syncthreads();
   }
 
+  if(vec_ok && m0 + BM <= M && n0 + BN <= N) {
+    epilogue_vec(acc, C, ldc, m0 + wm0, n0 + wn0,
+                 reinterpret_cast<float*>(smem) + wave * kEpiWaveFloats);
+    return;
+  }
 #pragma unroll
   for(int i = 0; i < 4; ++i) {
     const int row = m0 + wm0 + i * 16 + l16;
@@ -523,6 +597,13 @@ inline long tile_grid(int tiles_m, int tiles_n) {
   return ((T + kXcds - 1) / kXcds) * kXcds;
 }
 
+// The vector epilogue needs C 16-byte aligned with ldc a multiple of 4
+// (every 4-row group of a column starts on a 4-element boundary).
+template <typename S>
+int vec_epilogue_ok(const S* C, int ldc) {
+  return reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % 4 == 0;
+}
+
 template <typename S>
 hipError_t gemm_bf16_t(hipStream_t s, int terms, int m, int n, int K,
                        const MxpPacked& P, S* C, int ldc) {
@@ -538,15 +619,18 @@ hipError_t gemm_bf16_t(hipStream_t s, int terms, int m, int n, int K,
   switch(terms) {
     case 1:
       hipLaunchKernelGGL((gemm_bf16_kernel<1, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
-                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n);
+                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n,
+                         vec_epilogue_ok(C, ldc));
       break;
     case 2:
       hipLaunchKernelGGL((gemm_bf16_kernel<2, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
-                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n);
+                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n,
+                         vec_epilogue_ok(C, ldc));
       break;
     case 3:
       hipLaunchKernelGGL((gemm_bf16_kernel<3, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
-                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n);
+                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n,
+                         vec_epilogue_ok(C, ldc));
       break;
     default:
       return hipErrorInvalidValue;
@@ -567,7 +651,8 @@ hipError_t gemm_fp8_t(hipStream_t s, int m, int n, int K, const MxpPacked& P,
   hipLaunchKernelGGL((gemm_fp8_kernel<kWM, kWN, S>), g, b, 0, s, m, n, nks,
                      static_cast<const uint8_t*>(P.a), P.m_pad, P.ea,
                      static_cast<const uint8_t*>(P.b), P.n_pad, P.eb,
-                     P.n_pad / 128, C, ldc, tiles_m, tiles_n);
+                     P.n_pad / 128, C, ldc, tiles_m, tiles_n,
+                     vec_epilogue_ok(C, ldc));
   return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // Small kernels of the mixed-precision path: storage <-> FP32 block copies,
-// FP64 vector row interchanges, x += d, and the right-hand side b generated
-// in FP64 from the matrix generator (column n of [A|b], state i + n*n).
+// FP64 vector row interchanges (a panel's interchanges as net moves),
+// x += d, and the right-hand side b generated in FP64 from the matrix
+// generator (column n of [A|b], state i + n*n).
 #include "lcg.hpp"
 #include "mxp.hpp"
 
@@ -30,16 +31,22 @@ __global__ __launch_bounds__(256) void copy_block_kernel(
         cvt<D>(A[i + static_cast<size_t>(j) * lda]);
 }
 
-// Sequential interchanges y[i] <-> y[ipiv[i]], i = k0 .. k0+jb-1.
-__global__ void vec_swaps_kernel(int k0, int jb, const int* __restrict__ ipiv,
-                                 double* __restrict__ y) {
-  for(int i = k0; i < k0 + jb; ++i) {
-    const int p = ipiv[i];
-    if(p != i) {
-      const double t = y[i];
-      y[i]           = y[p];
-      y[p]           = t;
-    }
+// y[dst_i] = y[src_i] for the cnt pairs (one workgroup; every source is read
+// before any destination is written).
+__global__ __launch_bounds__(1024) void apply_swaps_kernel(
+    int cnt, const int* __restrict__ pairs, double* __restrict__ y) {
+  constexpr int kPer = (kMaxSwapPairs + 1023) / 1024;
+  double        v[kPer];
+#pragma unroll
+  for(int q = 0; q < kPer; ++q) {
+    const int i = threadIdx.x + 1024 * q;
+    if(i < cnt) v[q] = y[pairs[2 * i + 1]];
+  }
+  __syncthreads();
+#pragma unroll
+  for(int q = 0; q < kPer; ++q) {
+    const int i = threadIdx.x + 1024 * q;
+    if(i < cnt) y[pairs[2 * i]] = v[q];
   }
 }
 
@@ -90,10 +97,12 @@ hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
   return copy_t(s, m, n, A, lda, B, ldb);
 }
 
-hipError_t launch_vec_swaps(hipStream_t s, int k0, int jb, const int* ipiv,
-                            double* y) {
-  if(jb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(vec_swaps_kernel, dim3(1), dim3(1), 0, s, k0, jb, ipiv, y);
+hipError_t launch_apply_swaps(hipStream_t s, int cnt, const int* pairs,
+                              double* y) {
+  if(cnt <= 0) return hipSuccess;
+  if(cnt > kMaxSwapPairs) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(apply_swaps_kernel, dim3(1), dim3(1024), 0, s, cnt, pairs,
+                     y);
   return hipGetLastError();
 }
 

@@ -12,7 +12,9 @@
 // direction: wave 0 solves the chunk's 64 x 64 triangle with the chunk's rows
 // of op(T) in registers (one row per lane) and x broadcast by lane shuffles;
 // then every thread applies the chunk's 64 columns to the rows still to be
-// solved.  One workgroup barrier per 64 rows.
+// solved, with all 64 loads of a row issued before the first product (the
+// block is latency bound: one workgroup per column).  One workgroup barrier
+// per 64 rows.
 #include "mxp.hpp"
 
 namespace hplx {
@@ -89,11 +91,27 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(
     __syncthreads();
     const int r0 = LOWER ? c0 + w : 0;
     const int r1 = LOWER ? k : c0;
-    for(int r = r0 + t; r < r1; r += blockDim.x) {
-      V acc = V(0);
-      for(int j = 0; j < w; ++j)
-        acc = fma(op_at<TRANS, V>(T, ldt, r, c0 + j), sx[c0 + j], acc);
-      sx[r] -= acc;
+    if(w == 64) {
+      // full chunk: the row's 64 entries are loaded before the first use, so
+      // their latencies overlap
+      for(int r = r0 + t; r < r1; r += blockDim.x) {
+        MT tv[64];
+#pragma unroll
+        for(int j = 0; j < 64; ++j)
+          tv[j] = TRANS ? T[c0 + j + static_cast<size_t>(r) * ldt]
+                        : T[r + static_cast<size_t>(c0 + j) * ldt];
+        V acc = V(0);
+#pragma unroll
+        for(int j = 0; j < 64; ++j) acc = fma(to_v<V>(tv[j]), sx[c0 + j], acc);
+        sx[r] -= acc;
+      }
+    } else {
+      for(int r = r0 + t; r < r1; r += blockDim.x) {
+        V acc = V(0);
+        for(int j = 0; j < w; ++j)
+          acc = fma(op_at<TRANS, V>(T, ldt, r, c0 + j), sx[c0 + j], acc);
+        sx[r] -= acc;
+      }
     }
     __syncthreads();
   }

@@ -52,9 +52,12 @@ hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
                              int lda, __half* B, int ldb);
 
 // ---- solves with low-precision factors and FP64 vectors
-// y[k0 : k0+jb] <-> y[ipiv[i]] for i = k0 .. k0+jb-1 in order.
-hipError_t launch_vec_swaps(hipStream_t s, int k0, int jb, const int* ipiv,
-                            double* y);
+// The interchanges y[i] <-> y[ipiv[i]] of one panel (i = k0 .. k0+jb-1 in
+// order) given as their net effect, cnt (dst, src) pairs:
+// y[dst] = y_old[src] (cnt <= kMaxSwapPairs, all reads before any write).
+constexpr int kMaxSwapPairs = 2 * kMaxNB;
+hipError_t launch_apply_swaps(hipStream_t s, int cnt, const int* pairs,
+                              double* y);
 // op(T) x = x for one FP64 vector, T in storage type (k <= kMaxNB):
 // lower unit (forward) or upper non-unit (backward).
 hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "api_common.hpp"
 #include "mxp.hpp"
@@ -40,6 +41,11 @@ struct MxpWs {
   double* r    = nullptr;  // refinement residual / correction, n
   double* part = nullptr;  // ir_part_len(n)
   double* out  = nullptr;  // 16 doubles
+  // each panel's row interchanges as net (dst, src) moves of the solve's
+  // vector: pairs of panel p at swp[2 swp_off[p] .. 2 swp_off[p+1])
+  int*             swp = nullptr;  // device, up to 2n pairs
+  std::vector<int> swp_off;
+  bool             swp_ready = false;
 };
 
 int hpl_ld(int n) { return ((n > 0 ? n : 0) + 95) / 128 * 128 + 32; }
@@ -71,6 +77,8 @@ int check_mat(Ctx& c, const hplx_mxp_pmat* m) {
   return HPLX_SUCCESS;
 }
 
+int build_swaps(Ctx& c, const hplx_mxp_pmat* mat);
+
 template <typename S>
 int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
                 hplx_mxp_pmat* mat) {
@@ -94,6 +102,7 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
   hplx::Progress prog(c, npanels, nb, N);
   HIPCHK(c, prog.init());
 
+  ws->swp_ready = false;
   HIPCHK(c, hplx::reset_panel_state(c));
   for(int j = 0; j < N; j += nb) {
     const int jb = (N - j) < nb ? (N - j) : nb;
@@ -170,7 +179,46 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
     return fail(c, HPLX_ERR_HIP,
                 "panel kernel wait timed out (workgroups not co-resident)");
   mat->info = flags[0];
+  if(const int r = build_swaps(c, mat)) return r;
   return flags[0];
+}
+
+// The interchanges of panel p (y[i] <-> y[ipiv[i]] for its rows i in order)
+// composed into the moves y[dst] = y_old[src] over the rows they touch, so
+// that the solve applies them in one parallel step per panel.
+int build_swaps(Ctx& c, const hplx_mxp_pmat* mat) {
+  MxpWs*    ws = static_cast<MxpWs*>(mat->ws);
+  const int N = mat->n, nb = mat->nb;
+  std::vector<int> piv(N), cur(N), pairs;
+  pairs.reserve(4 * static_cast<size_t>(N));
+  HIPCHK(c, hipMemcpyAsync(piv.data(), mat->ipiv, sizeof(int) * N,
+                           hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  for(int i = 0; i < N; ++i) cur[i] = i;
+  ws->swp_off.assign(1, 0);
+  for(int j = 0; j < N; j += nb) {
+    const int jb = std::min(nb, N - j);
+    for(int i = j; i < j + jb; ++i) {
+      const int p = piv[i];
+      if(p < i || p >= N)
+        return fail(c, HPLX_ERR_INVALID, "pivot %d of row %d out of range", p, i);
+      std::swap(cur[i], cur[p]);
+    }
+    // record every moved row once, restoring cur as it goes
+    for(int i = j; i < j + jb; ++i)
+      for(const int d : {i, piv[i]})
+        if(cur[d] != d) {
+          pairs.push_back(d);
+          pairs.push_back(cur[d]);
+          cur[d] = d;
+        }
+    ws->swp_off.push_back(static_cast<int>(pairs.size() / 2));
+  }
+  if(!pairs.empty())
+    HIPCHK(c, hipMemcpy(ws->swp, pairs.data(), sizeof(int) * pairs.size(),
+                        hipMemcpyHostToDevice));
+  ws->swp_ready = true;
+  return HPLX_SUCCESS;
 }
 
 template <typename S>
@@ -178,11 +226,17 @@ int solve(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
   const int   N = mat->n, nb = mat->nb, lda = mat->ld;
   const S*    A = static_cast<const S*>(mat->A);
   hipStream_t s = c.stream;
+  MxpWs* ws = static_cast<MxpWs*>(mat->ws);
+  if(!ws->swp_ready) {
+    const int r = build_swaps(c, mat);
+    if(r) return r;
+  }
   // L y = P b, panel by panel (each panel's interchanges, then its L11 and
   // L21; L is left unpivoted by later panels)
   for(int j = 0; j < N; j += nb) {
     const int jb = (N - j) < nb ? (N - j) : nb;
-    HIPCHK(c, hplx::launch_vec_swaps(s, j, jb, mat->ipiv, x));
+    const int p0 = ws->swp_off[j / nb], p1 = ws->swp_off[j / nb + 1];
+    HIPCHK(c, hplx::launch_apply_swaps(s, p1 - p0, ws->swp + 2 * p0, x));
     HIPCHK(c, hplx::launch_tri_solve_vec(s, true, jb,
                                          A + j + static_cast<size_t>(j) * lda,
                                          lda, x + j));
@@ -274,7 +328,8 @@ int hplx_mxp_alloc(hplx_ctx* ctx, int n, int nb, int storage,
             hipMalloc(reinterpret_cast<void**>(&ws->eb), 4 * nsc) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&ws->r), 8 * static_cast<size_t>(n)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&ws->part), 8 * hplx::ir_part_len(n)) == hipSuccess &&
-            hipMalloc(reinterpret_cast<void**>(&ws->out), 16 * 8) == hipSuccess;
+            hipMalloc(reinterpret_cast<void**>(&ws->out), 16 * 8) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->swp), 16 * static_cast<size_t>(n)) == hipSuccess;
   if(ok && storage == HPLX_STORE_FP16) {
     ws->ldp = static_cast<int>(hplx::align_up(n, 64));
     ok = hipMalloc(reinterpret_cast<void**>(&ws->P32), 4 * static_cast<size_t>(ws->ldp) * nb) == hipSuccess &&
@@ -291,19 +346,19 @@ int hplx_mxp_free(hplx_ctx* ctx, hplx_mxp_pmat* mat) {
   int r = use(ctx);
   if(r) return r;
   if(!mat) return HPLX_SUCCESS;
-  hipStreamSynchronize(ctx->c.stream);
+  (void)hipStreamSynchronize(ctx->c.stream);
   if(MxpWs* ws = static_cast<MxpWs*>(mat->ws)) {
     for(void* p : {static_cast<void*>(ws->P32), static_cast<void*>(ws->W),
                    static_cast<void*>(ws->U), static_cast<void*>(ws->linv), ws->pa,
                    ws->pb, static_cast<void*>(ws->ea), static_cast<void*>(ws->eb),
                    static_cast<void*>(ws->r), static_cast<void*>(ws->part),
-                   static_cast<void*>(ws->out)})
-      if(p) hipFree(p);
+                   static_cast<void*>(ws->out), static_cast<void*>(ws->swp)})
+      if(p) (void)hipFree(p);
     delete ws;
   }
-  if(mat->A) hipFree(mat->A);
-  if(mat->X) hipFree(mat->X);
-  if(mat->ipiv) hipFree(mat->ipiv);
+  if(mat->A) (void)hipFree(mat->A);
+  if(mat->X) (void)hipFree(mat->X);
+  if(mat->ipiv) (void)hipFree(mat->ipiv);
   std::memset(mat, 0, sizeof(*mat));
   return HPLX_SUCCESS;
 }
@@ -442,7 +497,7 @@ int hplx_mxp_update(hplx_ctx* ctx, int update, int terms, int storage, int m,
      hipMalloc(reinterpret_cast<void**>(&pk.ea), 4 * sa) != hipSuccess ||
      hipMalloc(reinterpret_cast<void**>(&pk.eb), 4 * sb) != hipSuccess) {
     for(void* p : {pk.a, pk.b, static_cast<void*>(pk.ea), static_cast<void*>(pk.eb)})
-      if(p) hipFree(p);
+      if(p) (void)hipFree(p);
     return fail(c, HPLX_ERR_ALLOC, "mxp_update: scratch allocation");
   }
   hipError_t e = hplx::launch_mxp_pack(s, update, terms, m, k, L, ldl, n, U,
@@ -455,7 +510,7 @@ int hplx_mxp_update(hplx_ctx* ctx, int update, int terms, int storage, int m,
                                     static_cast<float*>(C), ldc);
   const hipError_t es = hipStreamSynchronize(s);
   for(void* p : {pk.a, pk.b, static_cast<void*>(pk.ea), static_cast<void*>(pk.eb)})
-    hipFree(p);
+    (void)hipFree(p);
   HIPCHK(c, e);
   HIPCHK(c, es);
   return HPLX_SUCCESS;
