@@ -10,8 +10,8 @@
 //   2. it publishes the candidate's value, row index and 32-wide row segment
 //      in its slot, then a sequence number with release ordering; workgroup
 //      0, which holds the current top row, publishes that row likewise;
-//   3. one wave of every workgroup polls all slots' sequence numbers and
-//      takes one acquire fence; then the workgroup reads all candidates
+//   3. one wave of every workgroup polls all slots' sequence numbers (each
+//      lane its slots at once) and takes one acquire fence; then the workgroup reads all candidates
 //      (values and row segments) in one pass into LDS, reduces them itself
 //      and takes the winner's row from LDS;
 //   4. registers are updated: row interchange, division by the pivot,
@@ -133,11 +133,31 @@ __device__ inline void block_maxloc(double& v, int& r, LeafShared& sh) {
   __syncthreads();  // sh.wv may be rewritten by the next call
 }
 
-// Wait until slot s shows sequence `want`; false on timeout or error.
-__device__ inline bool poll_slot(const double* s, unsigned long long want,
-                                 int* err) {
+// Lane l of one wave waits for slots l, l+64, ... (< G) of `slots`, and for
+// `extra` if not null, to show sequence `want`.  All of a lane's slots are
+// read at once on every round, so the wait costs one memory latency once
+// they are published.  False on timeout or error.
+__device__ inline bool poll_slots(const double* slots, int G,
+                                  const double* extra, unsigned long long want,
+                                  int* err) {
+  constexpr int kPer = (kMaxGrid + 63) / 64;
+  const int     lane = threadIdx.x & 63;
+  unsigned      pending = extra ? 1u << kPer : 0u;
+#pragma unroll
+  for(int q = 0; q < kPer; ++q)
+    if(lane + 64 * q < G) pending |= 1u << q;
   uint32_t spins = 0;
-  while(ld_seq(s + kSlotSeq) != want) {
+  while(pending) {
+    unsigned long long sv[kPer + 1];
+#pragma unroll
+    for(int q = 0; q < kPer; ++q)
+      if(pending & (1u << q))
+        sv[q] = ld_seq(slots + (lane + 64 * q) * kSlotStride + kSlotSeq);
+    if(pending & (1u << kPer)) sv[kPer] = ld_seq(extra + kSlotSeq);
+#pragma unroll
+    for(int q = 0; q <= kPer; ++q)
+      if((pending & (1u << q)) && sv[q] == want) pending &= ~(1u << q);
+    if(!pending) break;
     ++spins;
     if((spins & 255u) == 0 &&
        __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
@@ -198,9 +218,8 @@ __device__ __attribute__((noinline)) bool exchange(const LeafArgs* ap, int jj,
   // candidates.
   int ok = 1;
   if(t < 64) {
-    for(int g = t; g < G && ok; g += 64)
-      ok = poll_slot(a.slots + (par + g) * kSlotStride, seq, a.err);
-    if(ok && t == 0) ok = poll_slot(tslot, seq, a.err);
+    ok = poll_slots(a.slots + par * kSlotStride, G, t == 0 ? tslot : nullptr,
+                    seq, a.err);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   if(!__syncthreads_and(ok)) return false;
@@ -411,14 +430,26 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
   __syncthreads();
   const int np     = sh.np;
   const int others = a.JB - a.w;
-  if(np > 0) {
-    for(int idx = b * kWaves + wave; idx < others; idx += G * kWaves) {
-      const int c   = idx < a.c0 ? idx : idx + a.w;
-      S*        col = P + static_cast<size_t>(c) * a.lda;
-      if(lane < np) {
-        // one instruction loads every source before any lane stores
-        const S x        = col[sh.src[lane]];
-        col[sh.dst[lane]] = x;
+  if(np > 0 && lane < np) {
+    // a wave takes columns idx, idx + stride, ...; each column's sources are
+    // all loaded (one instruction) before any of its destinations is stored,
+    // and kBatch columns are in flight at once
+    constexpr int kBatch = 8;
+    const int     stride = G * kWaves;
+    const int     src = sh.src[lane], dst = sh.dst[lane];
+    for(int idx0 = b * kWaves + wave; idx0 < others; idx0 += kBatch * stride) {
+      S x[kBatch];
+#pragma unroll
+      for(int u = 0; u < kBatch; ++u) {
+        const int idx = idx0 + u * stride;
+        const int c   = idx < a.c0 ? idx : idx + a.w;
+        if(idx < others) x[u] = P[src + static_cast<size_t>(c) * a.lda];
+      }
+#pragma unroll
+      for(int u = 0; u < kBatch; ++u) {
+        const int idx = idx0 + u * stride;
+        const int c   = idx < a.c0 ? idx : idx + a.w;
+        if(idx < others) P[dst + static_cast<size_t>(c) * a.lda] = x[u];
       }
     }
   }
