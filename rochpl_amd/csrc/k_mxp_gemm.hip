@@ -316,9 +316,9 @@ __device__ inline bool map_tile(int tiles_m, int tiles_n, int& tm, int& tn) {
 // loaded and stored as 16-byte (FP32) / 8-byte (FP16) vectors, 256 (128)
 // contiguous bytes per column per instruction.  Full, aligned tiles only;
 // the others take the element-wise path.
-constexpr int kEpiPitch = 68;                       // floats per LDS column
-constexpr int kEpiWaveFloats = 32 * kEpiPitch;      // per wave
-constexpr int kEpiBytes = 4 * kEpiWaveFloats * 4;   // 4 waves
+constexpr int kEpiPitch      = 68;                // floats per LDS column
+constexpr int kEpiWaveFloats = 32 * kEpiPitch;   // per wave
+constexpr int epi_bytes(int waves) { return waves * kEpiWaveFloats * 4; }
 
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4v __attribute__((ext_vector_type(4)));
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
   // one LDS arena: operand stages during the K loop, epilogue scratch after
   constexpr int kOps   = TERMS * (BM + BN) * RS * 2;
   __shared__ __attribute__((aligned(16)))
-      uint8_t smem[kOps > kEpiBytes ? kOps : kEpiBytes];
+      uint8_t smem[kOps > epi_bytes(WM * WN) ? kOps : epi_bytes(WM * WN)];
   uint16_t(*sA)[BM * RS] = reinterpret_cast<uint16_t(*)[BM * RS]>(smem);
   uint16_t(*sB)[BN * RS] =
       reinterpret_cast<uint16_t(*)[BN * RS]>(smem + TERMS * BM * RS * 2);
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
   constexpr int BCH = BN * 8 / NT;
   constexpr int kOps = (BM + BN) * RS;
   __shared__ __attribute__((aligned(16)))
-      uint8_t smem[kOps > kEpiBytes ? kOps : kEpiBytes];
+      uint8_t smem[kOps > epi_bytes(WM * WN) ? kOps : epi_bytes(WM * WN)];
   uint8_t* sA = smem;
   uint8_t* sB = smem + BM * RS;
 
@@ -604,38 +604,35 @@ int vec_epilogue_ok(const S* C, int ldc) {
   return reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % 4 == 0;
 }
 
+template <int TERMS, int WM, int WN, typename S>
+hipError_t gemm_bf16_launch(hipStream_t s, int m, int n, int K,
+                            const MxpPacked& P, S* C, int ldc) {
+  constexpr int BM = 64 * WM, BN = 64 * WN;
+  const int  tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+  const long grid    = tile_grid(tiles_m, tiles_n);
+  if(grid > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((gemm_bf16_kernel<TERMS, WM, WN, S>),
+                     dim3(static_cast<unsigned>(grid)), dim3(64 * WM * WN), 0, s,
+                     m, n, (K + 31) / 32, static_cast<const uint16_t*>(P.a),
+                     P.m_pad, P.aplane, static_cast<const uint16_t*>(P.b), P.n_pad,
+                     P.bplane, C, ldc, tiles_m, tiles_n, vec_epilogue_ok(C, ldc));
+  return hipGetLastError();
+}
+
+// Tiles: 128 x 128 for 2 and 3 terms; 256 x 128 (8 waves) for plain BF16,
+// whose stages are small enough that the taller tile's operand reuse pays
+// (2.17 against 2.60 ms at 32768 x 32768 x 512; for 3 terms the 128 x 128
+// tile is the faster, 6.5 against 6.9 ms).  Rows are padded to 256.
 template <typename S>
 hipError_t gemm_bf16_t(hipStream_t s, int terms, int m, int n, int K,
                        const MxpPacked& P, S* C, int ldc) {
   if(m <= 0 || n <= 0 || K <= 0) return hipSuccess;
-  constexpr int BM = 64 * kWM, BN = 64 * kWN;
-  const int  tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
-  const long grid    = tile_grid(tiles_m, tiles_n);
-  if(grid > 0x7fffffffL) return hipErrorInvalidValue;
-  const int nks = (K + 31) / 32;
-  const dim3 g(static_cast<unsigned>(grid)), b(64 * kWM * kWN);
-  const uint16_t* A = static_cast<const uint16_t*>(P.a);
-  const uint16_t* B = static_cast<const uint16_t*>(P.b);
   switch(terms) {
-    case 1:
-      hipLaunchKernelGGL((gemm_bf16_kernel<1, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
-                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n,
-                         vec_epilogue_ok(C, ldc));
-      break;
-    case 2:
-      hipLaunchKernelGGL((gemm_bf16_kernel<2, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
-                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n,
-                         vec_epilogue_ok(C, ldc));
-      break;
-    case 3:
-      hipLaunchKernelGGL((gemm_bf16_kernel<3, kWM, kWN, S>), g, b, 0, s, m, n, nks, A,
-                         P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n,
-                         vec_epilogue_ok(C, ldc));
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case 1: return gemm_bf16_launch<1, 4, 2>(s, m, n, K, P, C, ldc);
+    case 2: return gemm_bf16_launch<2, kWM, kWN>(s, m, n, K, P, C, ldc);
+    case 3: return gemm_bf16_launch<3, kWM, kWN>(s, m, n, K, P, C, ldc);
+    default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 template <typename S>

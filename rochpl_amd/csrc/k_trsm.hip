@@ -39,21 +39,21 @@ __device__ inline V op_at(const MT* T, int ldt, int i, int j) {
 // V: element type of X and of the arithmetic (double, or float for the FP32
 // panel); MT: element type of T (V, or FP32 / FP16 factors applied to FP64
 // vectors in the mixed-precision solves).
+// One workgroup solves for the column xc (column `col` of the identity when
+// IDENTITY).
 template <typename MT, typename V, bool LOWER, bool TRANS, bool UNIT,
           bool IDENTITY>
-__global__ __launch_bounds__(256) void tri_solve_kernel(
-    int k, const MT* __restrict__ T, int ldt, V* __restrict__ X, int ldx) {
+__device__ inline void tri_solve_col(int k, const MT* __restrict__ T, int ldt,
+                                     V* __restrict__ xc, int col) {
   __shared__ V sx[kMaxNB];
-  const int col  = blockIdx.x;
   const int t    = threadIdx.x;
   const int lane = t & 63;
-  V*        xc   = X + static_cast<size_t>(col) * ldx;
   for(int i = t; i < k; i += blockDim.x)
     sx[i] = IDENTITY ? (i == col ? V(1) : V(0)) : xc[i];
   __syncthreads();
   const int nchunk = (k + 63) / 64;
-  // with the identity and a forward sweep, rows above `col` stay zero
-  const int first = (LOWER && IDENTITY) ? col / 64 : 0;
+  // with the identity, the rows before `col` in sweep order stay zero
+  const int first = IDENTITY ? (LOWER ? col / 64 : nchunk - 1 - col / 64) : 0;
   for(int s = first; s < nchunk; ++s) {
     const int c0 = LOWER ? s * 64 : (nchunk - 1 - s) * 64;
     const int w  = min(64, k - c0);
@@ -116,6 +116,29 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(
     __syncthreads();
   }
   for(int i = t; i < k; i += blockDim.x) xc[i] = sx[i];
+}
+
+template <typename MT, typename V, bool LOWER, bool TRANS, bool UNIT,
+          bool IDENTITY>
+__global__ __launch_bounds__(256) void tri_solve_kernel(
+    int k, const MT* __restrict__ T, int ldt, V* __restrict__ X, int ldx) {
+  tri_solve_col<MT, V, LOWER, TRANS, UNIT, IDENTITY>(
+      k, T, ldt, X + static_cast<size_t>(blockIdx.x) * ldx, blockIdx.x);
+}
+
+// FP64 inverses of consecutive kb x kb diagonal blocks of T: block J
+// (blockIdx.y) starts at T[J kb, J kb]; column blockIdx.x of its inverse goes
+// to Inv + (J kb + col) kb (each inverse column-major with leading dimension
+// kb).  LOWER: unit lower triangle; else the upper triangle.
+template <typename MT, bool LOWER>
+__global__ __launch_bounds__(256) void diag_inv_kernel(int kb,
+                                                       const MT* __restrict__ T,
+                                                       int ldt,
+                                                       double* __restrict__ Inv) {
+  const size_t J = blockIdx.y;
+  tri_solve_col<MT, double, LOWER, false, LOWER, true>(
+      kb, T + J * kb * (1 + static_cast<size_t>(ldt)), ldt,
+      Inv + (J * kb + blockIdx.x) * kb, blockIdx.x);
 }
 
 template <typename V>
@@ -183,6 +206,29 @@ hipError_t tri_solve_vec_t(hipStream_t s, bool lower, int k, const MT* T,
     hipLaunchKernelGGL((tri_solve_kernel<MT, double, false, false, false, false>),
                        dim3(1), dim3(256), 0, s, k, T, ldt, x, k);
   return hipGetLastError();
+}
+
+template <typename MT>
+hipError_t diag_inverse_t(hipStream_t s, bool lower, int nblk, int kb,
+                          const MT* T, int ldt, double* Inv) {
+  if(nblk <= 0 || kb <= 0) return hipSuccess;
+  if(kb > kMaxNB || nblk > 65535) return hipErrorInvalidValue;
+  const dim3 g(kb, nblk);
+  if(lower)
+    hipLaunchKernelGGL((diag_inv_kernel<MT, true>), g, dim3(256), 0, s, kb, T, ldt, Inv);
+  else
+    hipLaunchKernelGGL((diag_inv_kernel<MT, false>), g, dim3(256), 0, s, kb, T, ldt, Inv);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const float* T, int ldt, double* Inv) {
+  return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
+}
+
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const __half* T, int ldt, double* Inv) {
+  return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
 }
 
 hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,

@@ -21,13 +21,11 @@
 //     multiple of 8 (HPL's NB);
 //   - workgroup -> tile mapping XCD-aware (8 XCDs, groups of 8 tile rows).
 //
-// The stage loads are issued from inline assembly: with the compiler's
-// builtin the waitcnt pass cannot tell which LDS slot a load writes and
-// waits for all of them before every LDS read, which would serialise the
-// ring.  The kernel waits explicitly instead; no compiler-issued vector
-// memory access is in flight inside the K loop, and all stage loads are
-// drained before the epilogue.
+// The stage loads come from lds_stream.hpp; the kernel waits explicitly, no
+// compiler-issued vector memory access is in flight inside the K loop, and
+// all stage loads are drained before the epilogue.
 #include "hplx_internal.hpp"
+#include "lds_stream.hpp"
 
 namespace hplx {
 namespace {
@@ -38,24 +36,6 @@ constexpr int UROW = 130;  // LDS row pitch in doubles (128 + 2)
 constexpr int kUXcds = 8, kUGroupM = 8;
 
 typedef double ud4 __attribute__((ext_vector_type(4)));
-
-// 16 bytes per lane from global memory into LDS: lane l writes lds + 16 l.
-__device__ inline void lds_load16(const double* g, double* lds) {
-  const uint32_t l = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(
-      (__attribute__((address_space(3))) double*)(lds)));
-  const uint32_t u = __builtin_amdgcn_readfirstlane(l);
-  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
-               :
-               : "s"(u), "v"(g)
-               : "memory", "m0");
-}
-
-// s_waitcnt vmcnt(n) with the other counters left alone (gfx9 encoding).
-template <int N>
-__device__ inline void wait_vm() {
-  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) |
-                             (0xF << 8));
-}
 
 #define HPLX_USLOT(n)                                             \
   __shared__ __attribute__((aligned(16))) double sA##n[UBK][UROW]; \

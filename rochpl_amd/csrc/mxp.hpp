@@ -64,13 +64,26 @@ hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
                                 const float* T, int ldt, double* x);
 hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
                                 const __half* T, int ldt, double* x);
-// y[0:m] -= M[0:m, 0:k] * x[0:k], M in storage type, FP64 accumulation.
+// y[0:m] -= M[0:m, 0:k] * x[0:k], M in storage type, FP64 accumulation;
+// if xback is not null, x[0:k] is also copied there.
 hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const float* M,
-                           int ldm, const double* x, double* y);
+                           int ldm, const double* x, double* y,
+                           double* xback = nullptr);
 hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const __half* M,
-                           int ldm, const double* x, double* y);
+                           int ldm, const double* x, double* y,
+                           double* xback = nullptr);
 hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const double* M,
-                           int ldm, const double* x, double* y);
+                           int ldm, const double* x, double* y,
+                           double* xback = nullptr);
+// FP64 inverses of nblk consecutive kb x kb diagonal blocks of T (unit lower
+// triangle or upper triangle), block J at Inv + J kb kb, leading dimension kb.
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const float* T, int ldt, double* Inv);
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const __half* T, int ldt, double* Inv);
+// y[0:k] = Inv x[0:k], Inv k x k FP64 (leading dimension k).
+hipError_t launch_inv_gemv(hipStream_t s, int k, const double* Inv,
+                           const double* x, double* y);
 
 // The generator's matrix rounded to FP32 / FP16 storage.
 hipError_t launch_matgen(hipStream_t s, int m, int n, float* A, int lda,

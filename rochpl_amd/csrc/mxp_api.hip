@@ -10,7 +10,8 @@
 //   4. row swaps of the trailing columns; new top rows -> W (FP32)
 //   5. U12 = L11^{-1} W in FP32, stored back in S
 //   6. A22 -= L21 U12 in BF16 / FP8 (or FP32) on MFMA, FP32 accumulation
-// b is not carried through the loop.  The solve then runs in FP64 on the
+// b is not carried through the loop.  After the loop the diagonal blocks of
+// L and U are inverted in FP64.  The solve then runs in FP64 on the
 // device: x = U^{-1} L^{-1} P b with b generated in FP64 from the seed, and
 // refinement r = b - A x (A regenerated in FP64 tile by tile), d = LU \ r,
 // x += d until the HPL scaled residual is below the threshold.
@@ -46,6 +47,11 @@ struct MxpWs {
   int*             swp = nullptr;  // device, up to 2n pairs
   std::vector<int> swp_off;
   bool             swp_ready = false;
+  // FP64 inverses of the diagonal blocks of L (unit lower) and U, block J at
+  // J nb nb (leading dimension = the block's size), and a block-sized vector
+  double* dinv_l = nullptr;
+  double* dinv_u = nullptr;
+  double* ytmp   = nullptr;
 };
 
 int hpl_ld(int n) { return ((n > 0 ? n : 0) + 95) / 128 * 128 + 32; }
@@ -78,6 +84,27 @@ int check_mat(Ctx& c, const hplx_mxp_pmat* m) {
 }
 
 int build_swaps(Ctx& c, const hplx_mxp_pmat* mat);
+
+// Inverses of the diagonal blocks of the stored factors (all final once the
+// factorisation is done: later interchanges do not touch earlier columns).
+template <typename S>
+int build_inverses(Ctx& c, const hplx_mxp_pmat* mat) {
+  MxpWs*    ws = static_cast<MxpWs*>(mat->ws);
+  const int N = mat->n, nb = mat->nb, lda = mat->ld;
+  const S*  A     = static_cast<const S*>(mat->A);
+  const int nfull = N / nb, rest = N - nfull * nb;
+  const size_t tail = static_cast<size_t>(nfull) * nb * nb;
+  for(const bool lower : {true, false}) {
+    double* inv = lower ? ws->dinv_l : ws->dinv_u;
+    HIPCHK(c, hplx::launch_diag_inverse(c.stream, lower, nfull, nb, A, lda, inv));
+    if(rest > 0)
+      HIPCHK(c, hplx::launch_diag_inverse(
+                    c.stream, lower, 1, rest,
+                    A + static_cast<size_t>(nfull) * nb * (1 + static_cast<size_t>(lda)),
+                    lda, inv + tail));
+  }
+  return HPLX_SUCCESS;
+}
 
 template <typename S>
 int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
@@ -179,6 +206,7 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
     return fail(c, HPLX_ERR_HIP,
                 "panel kernel wait timed out (workgroups not co-resident)");
   mat->info = flags[0];
+  if(const int r = build_inverses<S>(c, mat)) return r;
   if(const int r = build_swaps(c, mat)) return r;
   return flags[0];
 }
@@ -221,6 +249,10 @@ int build_swaps(Ctx& c, const hplx_mxp_pmat* mat) {
   return HPLX_SUCCESS;
 }
 
+// x = U^{-1} L^{-1} P x in FP64 with the stored factors.  Each diagonal
+// block is applied as its FP64 inverse (one bandwidth-bound product) into
+// ytmp, then the block's columns below (L) or above (U) are applied from
+// ytmp, the same launch copying ytmp back into x.
 template <typename S>
 int solve(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
   const int   N = mat->n, nb = mat->nb, lda = mat->ld;
@@ -231,31 +263,36 @@ int solve(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
     const int r = build_swaps(c, mat);
     if(r) return r;
   }
+  double* y = ws->ytmp;
   // L y = P b, panel by panel (each panel's interchanges, then its L11 and
   // L21; L is left unpivoted by later panels)
   for(int j = 0; j < N; j += nb) {
     const int jb = (N - j) < nb ? (N - j) : nb;
     const int p0 = ws->swp_off[j / nb], p1 = ws->swp_off[j / nb + 1];
     HIPCHK(c, hplx::launch_apply_swaps(s, p1 - p0, ws->swp + 2 * p0, x));
-    HIPCHK(c, hplx::launch_tri_solve_vec(s, true, jb,
-                                         A + j + static_cast<size_t>(j) * lda,
-                                         lda, x + j));
+    HIPCHK(c, hplx::launch_inv_gemv(s, jb, ws->dinv_l + static_cast<size_t>(j) * nb,
+                                    x + j, y));
     if(N - j - jb > 0)
       HIPCHK(c, hplx::launch_gemv_sub(s, N - j - jb, jb,
                                       A + j + jb + static_cast<size_t>(j) * lda,
-                                      lda, x + j, x + j + jb));
+                                      lda, y, x + j + jb, x + j));
+    else
+      HIPCHK(c, hipMemcpyAsync(x + j, y, sizeof(double) * jb,
+                               hipMemcpyDeviceToDevice, s));
   }
   // U x = y, blocks from the bottom
   const int nblk = (N + nb - 1) / nb;
   for(int J = nblk - 1; J >= 0; --J) {
     const int s0 = J * nb;
     const int kb = (N - s0) < nb ? (N - s0) : nb;
-    HIPCHK(c, hplx::launch_tri_solve_vec(s, false, kb,
-                                         A + s0 + static_cast<size_t>(s0) * lda,
-                                         lda, x + s0));
+    HIPCHK(c, hplx::launch_inv_gemv(s, kb, ws->dinv_u + static_cast<size_t>(s0) * nb,
+                                    x + s0, y));
     if(s0 > 0)
       HIPCHK(c, hplx::launch_gemv_sub(s, s0, kb, A + static_cast<size_t>(s0) * lda,
-                                      lda, x + s0, x));
+                                      lda, y, x, x + s0));
+    else
+      HIPCHK(c, hipMemcpyAsync(x + s0, y, sizeof(double) * kb,
+                               hipMemcpyDeviceToDevice, s));
   }
   return HPLX_SUCCESS;
 }
@@ -318,6 +355,8 @@ int hplx_mxp_alloc(hplx_ctx* ctx, int n, int nb, int storage,
   const size_t pack = std::max(hplx::mxp_pack_bytes(HPLX_UPDATE_BF16, 3, n, nb),
                                hplx::mxp_pack_bytes(HPLX_UPDATE_FP8, 1, n, nb));
   const size_t nsc  = hplx::mxp_scale_count(n, nb);
+  // diagonal-block inverses: whole blocks of nb x nb, the last one smaller
+  const size_t dinv = static_cast<size_t>((n + nb - 1) / nb) * nb * nb;
   bool ok = hipMalloc(&mat->A, es * mat->ld * static_cast<size_t>(n)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&mat->X), 8 * static_cast<size_t>(n)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&mat->ipiv), 4 * static_cast<size_t>(n)) == hipSuccess &&
@@ -329,7 +368,10 @@ int hplx_mxp_alloc(hplx_ctx* ctx, int n, int nb, int storage,
             hipMalloc(reinterpret_cast<void**>(&ws->r), 8 * static_cast<size_t>(n)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&ws->part), 8 * hplx::ir_part_len(n)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&ws->out), 16 * 8) == hipSuccess &&
-            hipMalloc(reinterpret_cast<void**>(&ws->swp), 16 * static_cast<size_t>(n)) == hipSuccess;
+            hipMalloc(reinterpret_cast<void**>(&ws->swp), 16 * static_cast<size_t>(n)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->dinv_l), 8 * dinv) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->dinv_u), 8 * dinv) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->ytmp), 8 * static_cast<size_t>(nb)) == hipSuccess;
   if(ok && storage == HPLX_STORE_FP16) {
     ws->ldp = static_cast<int>(hplx::align_up(n, 64));
     ok = hipMalloc(reinterpret_cast<void**>(&ws->P32), 4 * static_cast<size_t>(ws->ldp) * nb) == hipSuccess &&
@@ -352,7 +394,9 @@ int hplx_mxp_free(hplx_ctx* ctx, hplx_mxp_pmat* mat) {
                    static_cast<void*>(ws->U), static_cast<void*>(ws->linv), ws->pa,
                    ws->pb, static_cast<void*>(ws->ea), static_cast<void*>(ws->eb),
                    static_cast<void*>(ws->r), static_cast<void*>(ws->part),
-                   static_cast<void*>(ws->out), static_cast<void*>(ws->swp)})
+                   static_cast<void*>(ws->out), static_cast<void*>(ws->swp),
+                   static_cast<void*>(ws->dinv_l), static_cast<void*>(ws->dinv_u),
+                   static_cast<void*>(ws->ytmp)})
       if(p) (void)hipFree(p);
     delete ws;
   }
