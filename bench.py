@@ -77,15 +77,20 @@ def cpu_baseline(n_sample, nb_sample, cores):
 
 
 def load_traffic(n, nb):
+    """(bytes per launch of this run's shape or None, the committed counter
+    measurement or None)."""
     if not os.path.exists(PROFILE_TRAFFIC):
-        return None
+        return None, None
     try:
         d = json.load(open(PROFILE_TRAFFIC))
     except ValueError:
-        return None
+        return None, None
+    probe = {k: d[k] for k in ("update_probe", "N", "NB", "bytes_per_launch",
+                               "algorithmic_bytes_per_launch",
+                               "traffic_over_algorithmic") if k in d}
     if d.get("N") == n and d.get("NB") == nb:
-        return d.get("bytes_per_launch")
-    return None
+        return d.get("bytes_per_launch"), probe
+    return None, probe
 
 
 def run_fp64(ctx, rp, args):
@@ -159,6 +164,7 @@ def run_fp64(ctx, rp, args):
     ctx.pmat_free(mat)
 
     t_mean = sum(times) / len(times)
+    traffic, traffic_probe = load_traffic(n, nb)
     gemm_ms = sum(s[0] for s in stats)
     gemm_launch = sum(s[1] for s in stats)
     gemm_flops = sum(s[2] for s in stats)
@@ -172,7 +178,8 @@ def run_fp64(ctx, rp, args):
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": load_traffic(n, nb),
+                     "traffic": traffic,
+                     "traffic_measured": traffic_probe,
                      "kernel": "trailing-update FP64 MFMA GEMM "
                                "(A22 -= L21*U12), %d launches, %.1f ms"
                                % (gemm_launch, gemm_ms),
@@ -268,6 +275,109 @@ def run_mixed(ctx, rp, form, args):
     }
 
 
+def copy_probe(gib):
+    """A plain device copy of known size (gib GiB read, gib GiB written), the
+    calibration of the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = int(gib * 2 ** 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "copies": 3}), flush=True)
+
+
+def _pmc_per_dispatch(path, counter, match):
+    import csv
+    vals, names = {}, {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter or not match(r["Kernel_Name"]):
+                continue
+            d = r["Dispatch_Id"]
+            vals[d] = vals.get(d, 0.0) + float(r["Counter_Value"])
+            names[d] = r["Kernel_Name"]
+    return vals, names
+
+
+def update_probe(m, n, jb):
+    """The FP64 trailing update (HPL_pdupdateTT: swaps, U12 = L11^-1 A12,
+    A22 -= L21 U12) alone on an m x (jb + n) block, three times: the shape
+    the HBM counters are read on (the whole solve does not run under counter
+    collection)."""
+    import numpy as np
+    import rochpl_amd as rp
+    ctx = rp.Context(0)
+    lda = m + 32
+    dA = ctx.alloc(8 * lda * (jb + n))
+    ctx.pdrandmat(m, jb + n, jb, dA.ptr, lda, rp.HPL_ISEED, 0.0)
+    perm = np.concatenate([np.arange(jb), -np.ones(jb)]).astype(np.int32)
+    dperm = ctx.alloc(8 * jb)
+    dperm.upload(perm)
+    dW = ctx.alloc(8 * jb * (n + 4))
+    for _ in range(3):
+        ctx.pdupdate(dA.ptr, lda, m, jb, n, dperm.ptr, dW.ptr)
+    ctx.synchronize()
+    ctx.close()
+    print(json.dumps({"update_probe": {"m": m - jb, "n": n, "k": jb},
+                      "launches": 3}), flush=True)
+
+
+def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, n, nb,
+                    copy_bytes, probe=None):
+    """HBM bytes per trailing-update launch from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
+    counter reads for the copy probe of known size.  probe = (m, n, k) of the
+    update probe, or None for the FP64 bench at (n, nb)."""
+    upd = lambda k: "update_kernel" in k
+    cpy = lambda k: "copy" in k.lower()
+    f, _ = _pmc_per_dispatch(fetch_csv, "FETCH_SIZE", upd)
+    w, _ = _pmc_per_dispatch(write_csv, "WRITE_SIZE", upd)
+    cf, _ = _pmc_per_dispatch(cfetch_csv, "FETCH_SIZE", cpy)
+    cw, _ = _pmc_per_dispatch(cwrite_csv, "WRITE_SIZE", cpy)
+    if not (f and w and cf and cw):
+        raise SystemExit("traffic summary: missing dispatches")
+    cf_kib = sum(cf.values()) / len(cf)
+    cw_kib = sum(cw.values()) / len(cw)
+    ff = copy_bytes / (cf_kib * 1024.0)
+    wf = copy_bytes / (cw_kib * 1024.0)
+    fb = sum(f.values()) / len(f) * 1024.0 * ff
+    wb = sum(w.values()) / len(w) * 1024.0 * wf
+    # algorithmic bytes of the average launch: C (m x nt) read and written,
+    # L21 and U12^T read once
+    alg = 0.0
+    launches = 0
+    if probe is not None:
+        pm, pn, pk = probe
+        alg, launches = 8.0 * (2.0 * pm * pn + pm * pk + pn * pk), 1
+    else:
+        for j in range(0, n, nb):
+            jb = min(nb, n - j)
+            m = n - j - jb
+            if m <= 0:
+                continue
+            alg += 8.0 * (2.0 * m * m + 2.0 * m * jb)
+            launches += 1
+    shape = ({"update_probe": {"m": probe[0], "n": probe[1], "k": probe[2]}}
+             if probe is not None else {"N": n, "NB": nb})
+    d = dict(shape, **{"kernel": "update_kernel",
+         "launches": len(f), "fetch_kib_per_launch_raw": sum(f.values()) / len(f),
+         "write_kib_per_launch_raw": sum(w.values()) / len(w),
+         "copy_probe_bytes": copy_bytes, "copy_fetch_kib_raw": cf_kib,
+         "copy_write_kib_raw": cw_kib, "fetch_factor": round(ff, 4),
+         "write_factor": round(wf, 4),
+         "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
+         "bytes_per_launch": fb + wb,
+         "algorithmic_bytes_per_launch": alg / max(launches, 1)})
+    d["traffic_over_algorithmic"] = round(d["bytes_per_launch"] /
+                                          d["algorithmic_bytes_per_launch"], 4)
+    os.makedirs(os.path.dirname(PROFILE_TRAFFIC), exist_ok=True)
+    with open(PROFILE_TRAFFIC, "w") as fh:
+        json.dump(d, fh, indent=1)
+    print(json.dumps(d), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,8 +402,35 @@ def main():
     ap.add_argument("--check-pivots", type=int, default=0,
                     help="compare the first K pivots with LAPACK dgetrf on the "
                          "N x K column prefix (prefix oracle)")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="only run a device copy of this size (counter "
+                         "calibration) and exit")
+    ap.add_argument("--update-probe", nargs=2, type=int, default=None,
+                    metavar=("M", "N"),
+                    help="only run the FP64 trailing update on an M x (NB+N) "
+                         "block (counter shape) and exit")
+    ap.add_argument("--traffic-summary", nargs=4, default=None,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
+                             "COPY_WRITE_CSV"),
+                    help="write %s from rocprofv3 --pmc counter files of this "
+                         "bench (--mode fp64) and of --copy-probe-gib, then "
+                         "exit" % os.path.relpath(PROFILE_TRAFFIC, ROOT))
     args = ap.parse_args()
 
+    if args.traffic_summary:
+        probe = None
+        if args.update_probe:
+            probe = (args.update_probe[0] - args.nb, args.update_probe[1],
+                     args.nb)
+        traffic_summary(*args.traffic_summary, args.n, args.nb,
+                        int(args.copy_probe_gib * 2 ** 30) // 8 * 8, probe)
+        return
+    if args.update_probe:
+        update_probe(args.update_probe[0], args.update_probe[1], args.nb)
+        return
+    if args.copy_probe_gib > 0:
+        copy_probe(args.copy_probe_gib)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (got --gpus %d); "
               "multi-GPU runs are not part of it" % args.gpus, flush=True)
