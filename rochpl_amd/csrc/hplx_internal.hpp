@@ -22,7 +22,7 @@ constexpr int kLeafRPT     = 3;     // rows per thread in the leaf kernel
 constexpr int kMaxLeafGrid = 256;   // workgroups a cooperative leaf may use
 constexpr int kMaxGrid     = kMaxLeafGrid;  // candidate slots per parity
 constexpr int kMaxNB       = 1024;  // widest panel (pivot composition, Linv)
-constexpr int kSlotStride  = 40;    // doubles per pivot candidate slot
+constexpr int kSlotStride  = 72;    // doubles per pivot candidate slot
 
 // Per-context device state.  The small workspace (candidate slots, barrier
 // counter, flags, L11 inverse, composed pivots) lives with the context; the

@@ -6,14 +6,17 @@
 //
 // Every workgroup keeps its slice of the leaf (up to 512*RPT rows x 32
 // columns) in registers for the whole leaf.  Per column:
-//   1. each workgroup finds its candidate max |a| (wave64 shuffles, LDS);
-//   2. it publishes the candidate's value, row index and 32-wide row segment
-//      in its slot, then a sequence number with release ordering; workgroup
-//      0, which holds the current top row, publishes that row likewise;
+//   1. each workgroup finds its candidate max |a| (DPP lane exchanges within
+//      a wave, then the 8 waves through LDS);
+//   2. the thread holding the candidate row stores it straight from its
+//      registers into the workgroup's slot (value, row index, 32-wide row);
+//      in workgroup 0 the thread holding the current top row stores it into
+//      the same slot; after a workgroup barrier one thread publishes a
+//      sequence number with release ordering;
 //   3. one wave of every workgroup polls all slots' sequence numbers (each
-//      lane its slots at once) and takes one acquire fence; then the workgroup reads all candidates
-//      (values and row segments) in one pass into LDS, reduces them itself
-//      and takes the winner's row from LDS;
+//      lane its slots at once) and takes one acquire fence; then every thread
+//      loads a share of all candidate rows and the top row into LDS, and each
+//      wave reduces the candidates itself (no further barrier);
 //   4. registers are updated: row interchange, division by the pivot,
 //      rank-1 update of the remaining leaf columns.
 // Slots are double-buffered by column parity, so a workgroup that runs ahead
@@ -22,8 +25,7 @@
 // pinned host flag once per column and lets a host thread run HPL_pdmxswp
 // (:94-117,455-463).
 //
-// Steps 1 and 4 touch the register tile and are instantiated per column
-// (static register indices); steps 2-3 are one out-of-line function.
+// Each column's steps are instantiated per column (static register indices).
 //
 // Every wait is bounded: after kSpinLimit polls a workgroup raises *err and
 // all workgroups leave the kernel, so a grid that is not co-resident cannot
@@ -39,8 +41,11 @@ namespace leaf {
 
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int      kWaves     = kLeafThreads / 64;
-// slot layout (doubles): [0] |value|, [1] row, [2] sequence, [4..35] row
-constexpr int kSlotVal = 0, kSlotRow = 1, kSlotSeq = 2, kSlotData = 4;
+// slot layout (doubles): [0] |value|, [1] row, [2] sequence, [4..35] the
+// candidate row, [36..67] the top row (workgroup 0's slot only)
+constexpr int kSlotVal = 0, kSlotRow = 1, kSlotSeq = 2, kSlotData = 4,
+              kSlotTop = 36;
+static_assert(kSlotTop + kLeafMax <= kSlotStride, "slot too small");
 
 __device__ inline bool better(double v, int r, double bv, int br) {
   return v > bv || (v == bv && r < br);
@@ -80,11 +85,10 @@ struct LeafShared {
   double cand[kMaxLeafGrid][kLeafMax];  // every workgroup's candidate row
   double cv[kMaxLeafGrid];
   int    cr[kMaxLeafGrid];
-  double mine[kLeafMax];  // this workgroup's candidate row
-  double u[kLeafMax];     // pivot row
-  double top[kLeafMax];   // current top row
+  double top[kLeafMax];  // current top row
   double wv[kWaves];
   int    wr[kWaves];
+  int    ok;
   int    piv[kLeafMax];
   int    lab[kLeafMax];
   int    ppos[kLeafMax];
@@ -92,7 +96,6 @@ struct LeafShared {
   int    dst[2 * kLeafMax];
   int    src[2 * kLeafMax];
   int    np;
-  int    p;
 };
 
 template <typename T, typename S>
@@ -105,18 +108,48 @@ __device__ inline void store_s(S* p, T v) {
   *p = static_cast<S>(v);
 }
 
-// Workgroup max-loc of (v, r); every thread gets the result.
+// One step of a wave max-loc: combine (v, r) with the lane that the DPP
+// control CTRL selects (rows outside ROW_MASK keep their value).
+template <int CTRL, int ROW_MASK>
+__device__ inline void maxloc_dpp(double& v, int& r) {
+  const long long b  = __double_as_longlong(v);
+  const int       lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
+  const int ol = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+  const int oh = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+  const int orr = __builtin_amdgcn_update_dpp(r, r, CTRL, ROW_MASK, 0xF, false);
+  const double ov = __longlong_as_double(
+      (static_cast<long long>(oh) << 32) | static_cast<unsigned int>(ol));
+  if(better(ov, orr, v, r)) {
+    v = ov;
+    r = orr;
+  }
+}
+
+// Wave max-loc of (v, r); every lane gets the result.  Within a row of 16
+// lanes: xor 1, xor 2 (quad permutes), half-row and row mirrors; across
+// rows: broadcasts of lane 15 (into rows 1, 3) and lane 31 (into rows 2,
+// 3), leaving the result in lane 63.
+__device__ inline void wave_maxloc(double& v, int& r) {
+  maxloc_dpp<0xB1, 0xF>(v, r);   // quad_perm [1,0,3,2]
+  maxloc_dpp<0x4E, 0xF>(v, r);   // quad_perm [2,3,0,1]
+  maxloc_dpp<0x141, 0xF>(v, r);  // row_half_mirror
+  maxloc_dpp<0x140, 0xF>(v, r);  // row_mirror
+  maxloc_dpp<0x142, 0xA>(v, r);  // row_bcast:15
+  maxloc_dpp<0x143, 0xC>(v, r);  // row_bcast:31
+  const long long b  = __double_as_longlong(v);
+  const int       lo = __builtin_amdgcn_readlane(static_cast<int>(b), 63);
+  const int       hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
+  r                  = __builtin_amdgcn_readlane(r, 63);
+  v = __longlong_as_double((static_cast<long long>(hi) << 32) |
+                           static_cast<unsigned int>(lo));
+}
+
+// Workgroup max-loc of (v, r); every thread gets the result.  sh.wv is next
+// written after further barriers of the same column, so none is needed here
+// after the read.
 __device__ inline void block_maxloc(double& v, int& r, LeafShared& sh) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for(int o = 32; o > 0; o >>= 1) {
-    const double ov  = __shfl_xor(v, o);
-    const int    orr = __shfl_xor(r, o);
-    if(better(ov, orr, v, r)) {
-      v = ov;
-      r = orr;
-    }
-  }
+  wave_maxloc(v, r);
   if(lane == 0) {
     sh.wv[wave] = v;
     sh.wr[wave] = r;
@@ -130,7 +163,6 @@ __device__ inline void block_maxloc(double& v, int& r, LeafShared& sh) {
       v = sh.wv[k];
       r = sh.wr[k];
     }
-  __syncthreads();  // sh.wv may be rewritten by the next call
 }
 
 // Lane l of one wave waits for slots l, l+64, ... (< G) of `slots`, and for
@@ -171,83 +203,56 @@ __device__ inline bool poll_slots(const double* slots, int G,
   return true;
 }
 
-// Steps 2-3 of a column: sh.mine holds this workgroup's candidate row (value
-// bv, row br) and, in workgroup 0, sh.top holds the top row.  On return
-// sh.p is the pivot row, sh.u its segment and sh.top the top row in every
-// workgroup.  Returns false if the grid has to stop.
-__device__ __attribute__((noinline)) bool exchange(const LeafArgs* ap, int jj,
-                                                   double bv, int br,
-                                                   LeafShared* shp) {
-  const LeafArgs& a  = *ap;
-  LeafShared&     sh = *shp;
-  const int       t  = threadIdx.x;
-  const int       b  = blockIdx.x;
-  const int       G  = gridDim.x;
-  if(G == 1) {
-    if(t < kLeafMax) sh.u[t] = sh.mine[t];
-    if(t == 0) sh.p = br;
-    __syncthreads();
-    return true;
-  }
-  // ---- publish (value, row, segment), then the sequence number
+// Steps 2-3 of a column after the candidate row (and in workgroup 0 the
+// top row) has been stored into this workgroup's slot: publish, wait for
+// every workgroup, load all candidates and the top row, reduce.  Returns
+// the pivot row (sh.cand[its workgroup] holds its segment, sh.top the top
+// row), or -1 if the grid has to stop.
+__device__ __attribute__((noinline)) int exchange(const LeafArgs* ap, int jj,
+                                                  LeafShared* shp) {
+  const LeafArgs& a    = *ap;
+  LeafShared&     sh   = *shp;
+  const int       t    = threadIdx.x;
+  const int       lane = t & 63;
+  const int       b    = blockIdx.x;
+  const int       G    = gridDim.x;
   const unsigned long long seq =
       static_cast<unsigned long long>(a.seq_base) + jj + 1;
-  const size_t par   = static_cast<size_t>(jj & 1) * (kMaxGrid + 1);
-  double*      slot  = a.slots + (par + b) * kSlotStride;
-  double*      tslot = a.slots + (par + kMaxGrid) * kSlotStride;
-  if(t < kLeafMax) {
-    slot[kSlotData + t] = sh.mine[t];
-  } else if(t == kLeafMax) {
-    slot[kSlotVal] = bv;
-    slot[kSlotRow] = static_cast<double>(br);
-  } else if(b == 0 && t >= 64 && t < 64 + kLeafMax) {
-    // the top row c0 + jj < c0 + 32 always lives in workgroup 0
-    tslot[kSlotData + (t - 64)] = sh.top[t - 64];
-  }
+  const double* slots = a.slots + static_cast<size_t>(jj & 1) * (kMaxGrid + 1) * kSlotStride;
   // The workgroup barrier orders every wave's slot stores before thread 0's
   // release store of the sequence number, so one release per slot (one L2
-  // write-back) publishes them all; no per-wave fence is needed.
+  // write-back) publishes them all.
   __syncthreads();
-  if(t == 0) {
-    st_seq(slot + kSlotSeq, seq);
-    if(b == 0) st_seq(tslot + kSlotSeq, seq);
-  }
-  // ---- wave 0 waits for every candidate and the top row (lane l polls
-  // slots l, l+64, ...; lane 0 also the top row), then takes one acquire
-  // fence; the barrier extends it to the other waves, which then read the
-  // candidates.
-  int ok = 1;
+  if(t == 0) st_seq(const_cast<double*>(slots) + b * kSlotStride + kSlotSeq, seq);
   if(t < 64) {
-    ok = poll_slots(a.slots + par * kSlotStride, G, t == 0 ? tslot : nullptr,
-                    seq, a.err);
+    const int ok = poll_slots(slots, G, nullptr, seq, a.err);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if(t == 0) sh.ok = ok;
   }
-  if(!__syncthreads_and(ok)) return false;
+  __syncthreads();
+  if(!sh.ok) return -1;
   for(int i = t; i < G * kLeafMax; i += kLeafThreads) {
     const int g = i / kLeafMax, c = i % kLeafMax;
-    sh.cand[g][c] = a.slots[(par + g) * kSlotStride + kSlotData + c];
+    sh.cand[g][c] = slots[g * kSlotStride + kSlotData + c];
   }
   if(t < G) {
-    const double* s = a.slots + (par + t) * kSlotStride;
-    sh.cv[t]        = s[kSlotVal];
-    sh.cr[t]        = static_cast<int>(s[kSlotRow]);
-  } else if(t >= kLeafThreads - kLeafMax && b != 0) {
+    sh.cv[t] = slots[t * kSlotStride + kSlotVal];
+    sh.cr[t] = static_cast<int>(slots[t * kSlotStride + kSlotRow]);
+  } else if(t >= kLeafThreads - kLeafMax) {
     const int c = t - (kLeafThreads - kLeafMax);
-    sh.top[c]   = tslot[kSlotData + c];
+    sh.top[c]   = slots[kSlotTop + c];
   }
   __syncthreads();
+  // every wave reduces the candidates itself
   double cv = -1.0;
   int    cr = INT_MAX;
-  if(t < G) {
-    cv = sh.cv[t];
-    cr = sh.cr[t];
-  }
-  block_maxloc(cv, cr, sh);
-  const int wb = (cr - a.c0) / a.rows_per_wg;
-  if(t < kLeafMax) sh.u[t] = sh.cand[wb][t];
-  if(t == 0) sh.p = cr;
-  __syncthreads();
-  return true;
+  for(int g = lane; g < G; g += 64)
+    if(better(sh.cv[g], sh.cr[g], cv, cr)) {
+      cv = sh.cv[g];
+      cr = sh.cr[g];
+    }
+  wave_maxloc(cv, cr);
+  return cr;
 }
 
 // One pivot step for compile-time column JJ: every register of the row tile
@@ -274,26 +279,48 @@ __device__ __forceinline__ void leaf_step(const LeafArgs& a,
     }
   }
   block_maxloc(bv, br, sh);
-#pragma unroll
-  for(int q = 0; q < RPT; ++q) {
-    if(rows[q] == br) {
-#pragma unroll
-      for(int c = 0; c < kLeafMax; ++c) sh.mine[c] = static_cast<double>(v[q][c]);
-    }
-    if(rows[q] == gc) {
-#pragma unroll
-      for(int c = 0; c < kLeafMax; ++c) sh.top[c] = static_cast<double>(v[q][c]);
-    }
-  }
-  __syncthreads();
   // ---- steps 2-3
-  if(!exchange(&a, JJ, bv, br, &sh)) {
-    alive = false;
-    return;
+  int           p;
+  const double* u;
+  if(gridDim.x == 1) {
+#pragma unroll
+    for(int q = 0; q < RPT; ++q)
+      if(rows[q] == br) {
+#pragma unroll
+        for(int c = 0; c < kLeafMax; ++c) sh.cand[0][c] = static_cast<double>(v[q][c]);
+      }
+    if(t == JJ) {  // the top row c0 + JJ: workgroup 0, first row set
+#pragma unroll
+      for(int c = 0; c < kLeafMax; ++c) sh.top[c] = static_cast<double>(v[0][c]);
+    }
+    __syncthreads();
+    p = br;
+    u = sh.cand[0];
+  } else {
+    double* slot = a.slots + (static_cast<size_t>(JJ & 1) * (kMaxGrid + 1) + b) * kSlotStride;
+#pragma unroll
+    for(int q = 0; q < RPT; ++q)
+      if(rows[q] == br) {
+#pragma unroll
+        for(int c = 0; c < kLeafMax; ++c) slot[kSlotData + c] = static_cast<double>(v[q][c]);
+      }
+    if(t == 0) {
+      slot[kSlotVal] = bv;
+      slot[kSlotRow] = static_cast<double>(br);
+    }
+    if(b == 0 && t == JJ) {  // the top row c0 + JJ
+#pragma unroll
+      for(int c = 0; c < kLeafMax; ++c) slot[kSlotTop + c] = static_cast<double>(v[0][c]);
+    }
+    p = exchange(&a, JJ, &sh);
+    if(p < 0) {
+      alive = false;
+      return;
+    }
+    u = sh.cand[(p - a.c0) / a.rows_per_wg];
   }
-  const int p = sh.p;
   if(t == 0) sh.piv[JJ] = p;
-  const T piv = static_cast<T>(sh.u[JJ]);
+  const T piv = static_cast<T>(u[JJ]);
   if(b == 0 && t == 0) {
     a.ipiv[gc] = p + a.row_offset;
     if(piv == T(0) && *a.info == 0) *a.info = a.info_base + gc + 1;
@@ -303,7 +330,7 @@ __device__ __forceinline__ void leaf_step(const LeafArgs& a,
   for(int q = 0; q < RPT; ++q) {
     if(rows[q] == gc) {
 #pragma unroll
-      for(int c = 0; c < kLeafMax; ++c) v[q][c] = static_cast<T>(sh.u[c]);
+      for(int c = 0; c < kLeafMax; ++c) v[q][c] = static_cast<T>(u[c]);
     } else if(rows[q] == p) {
 #pragma unroll
       for(int c = 0; c < kLeafMax; ++c) v[q][c] = static_cast<T>(sh.top[c]);
@@ -317,10 +344,11 @@ __device__ __forceinline__ void leaf_step(const LeafArgs& a,
       v[q][JJ] = l;
 #pragma unroll
       for(int c = JJ + 1; c < kLeafMax; ++c)
-        v[q][c] = fma(-l, static_cast<T>(sh.u[c]), v[q][c]);
+        v[q][c] = fma(-l, static_cast<T>(u[c]), v[q][c]);
     }
   }
-  __syncthreads();  // sh.u / sh.top / sh.mine are rewritten next column
+  // sh.cand / sh.top are rewritten next column only after the barrier in
+  // its block_maxloc, which every wave reaches after this update
 }
 
 template <typename T, int RPT, int JJ>
