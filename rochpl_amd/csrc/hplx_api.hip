@@ -80,7 +80,7 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   {
     int per_cu = 0;
     if(hplx::leaf_occupancy(&per_cu) != hipSuccess || per_cu < 1) {
-      hipStreamDestroy(c.own_stream);
+      (void)hipStreamDestroy(c.own_stream);
       delete ctx;
       return HPLX_ERR_HIP;
     }
@@ -98,7 +98,7 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   size_t off_dout  = off_flags + 256;
   size_t total     = off_dout + 256;
   if(hipMalloc(&c.small_base, total) != hipSuccess) {
-    hipStreamDestroy(c.own_stream);
+    (void)hipStreamDestroy(c.own_stream);
     delete ctx;
     return HPLX_ERR_ALLOC;
   }
@@ -110,8 +110,8 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   c.dout    = reinterpret_cast<double*>(b + off_dout);
   if(hipMemset(c.small_base, 0, total) != hipSuccess ||
      hipDeviceSynchronize() != hipSuccess) {
-    hipFree(c.small_base);
-    hipStreamDestroy(c.own_stream);
+    (void)hipFree(c.small_base);
+    (void)hipStreamDestroy(c.own_stream);
     delete ctx;
     return HPLX_ERR_HIP;
   }
@@ -122,12 +122,12 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
 int hplx_ctx_destroy(hplx_ctx* ctx) {
   if(!ctx) return HPLX_SUCCESS;
   Ctx& c = ctx->c;
-  hipSetDevice(c.device);
-  hipStreamSynchronize(c.stream);
-  for(hipEvent_t e : c.ev) hipEventDestroy(e);
-  for(hipEvent_t e : c.ev_prog) hipEventDestroy(e);
-  if(c.small_base) hipFree(c.small_base);
-  if(c.own_stream) hipStreamDestroy(c.own_stream);
+  (void)hipSetDevice(c.device);
+  (void)hipStreamSynchronize(c.stream);
+  for(hipEvent_t e : c.ev) (void)hipEventDestroy(e);
+  for(hipEvent_t e : c.ev_prog) (void)hipEventDestroy(e);
+  if(c.small_base) (void)hipFree(c.small_base);
+  if(c.own_stream) (void)hipStreamDestroy(c.own_stream);
   delete ctx;
   return HPLX_SUCCESS;
 }
@@ -219,9 +219,9 @@ int hplx_pmat_alloc(hplx_ctx* ctx, int n, int nb, hplx_pmat* mat) {
      hipMalloc(&mat->ipiv, static_cast<size_t>(n) * sizeof(int)) !=
          hipSuccess ||
      hipMalloc(&ws->W, ws->W_len * sizeof(double)) != hipSuccess) {
-    if(mat->X) hipFree(mat->X);
-    if(mat->ipiv) hipFree(mat->ipiv);
-    hipFree(mat->A);
+    if(mat->X) (void)hipFree(mat->X);
+    if(mat->ipiv) (void)hipFree(mat->ipiv);
+    (void)hipFree(mat->A);
     delete ws;
     std::memset(mat, 0, sizeof(*mat));
     return fail(c, HPLX_ERR_ALLOC, "hipMalloc of workspace failed");
@@ -234,15 +234,15 @@ int hplx_pmat_free(hplx_ctx* ctx, hplx_pmat* mat) {
   int r = use(ctx);
   if(r) return r;
   if(!mat) return HPLX_ERR_INVALID;
-  hipStreamSynchronize(ctx->c.stream);
+  (void)hipStreamSynchronize(ctx->c.stream);
   hplx::MatWs* ws = static_cast<hplx::MatWs*>(mat->ws);
   if(ws) {
-    if(ws->W) hipFree(ws->W);
+    if(ws->W) (void)hipFree(ws->W);
     delete ws;
   }
-  if(mat->ipiv) hipFree(mat->ipiv);
-  if(mat->X) hipFree(mat->X);
-  if(mat->A) hipFree(mat->A);
+  if(mat->ipiv) (void)hipFree(mat->ipiv);
+  if(mat->X) (void)hipFree(mat->X);
+  if(mat->A) (void)hipFree(mat->A);
   std::memset(mat, 0, sizeof(*mat));
   return HPLX_SUCCESS;
 }

@@ -208,7 +208,7 @@ __device__ inline bool poll_slots(const double* slots, int G,
 // every workgroup, load all candidates and the top row, reduce.  Returns
 // the pivot row (sh.cand[its workgroup] holds its segment, sh.top the top
 // row), or -1 if the grid has to stop.
-__device__ __attribute__((noinline)) int exchange(const LeafArgs* ap, int jj,
+__device__ __forceinline__ int exchange(const LeafArgs* ap, int jj,
                                                   LeafShared* shp) {
   const LeafArgs& a    = *ap;
   LeafShared&     sh   = *shp;
