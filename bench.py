@@ -93,6 +93,45 @@ def load_traffic(n, nb):
     return None, probe
 
 
+def prefix_matrix(ctx, rp, n, k, nb, shift):
+    """Columns 0..k-1 of the seeded N x N matrix (plus shift on the
+    diagonal), generated on the device by the library's matgen and copied to
+    the host in FP64."""
+    import numpy as np
+    d = ctx.alloc(8 * n * k)
+    try:
+        ctx.pdrandmat(n, k, nb, d.ptr, n, rp.HPL_ISEED, shift)
+        return np.asfortranarray(d.download((n, k), np.float64))
+    finally:
+        d.free()
+
+
+def gold_prefix(n, shift):
+    """Committed LAPACK prefix pivots of the N x 64 prefix, or None."""
+    import numpy as np
+    path = os.path.join(ROOT, "tests", "golden", "hpl_golden.npz")
+    key = "prefix_ipiv_%d_64" % n + ("_shift%d" % shift if shift else "")
+    if not os.path.exists(path):
+        return None
+    fx = np.load(path, allow_pickle=False)
+    return fx[key] if key in fx.files else None
+
+
+def x_fp64(ctx, rp, n, nb):
+    """x of the FP64 solve (hplx_pdgesv) of the same seeded system."""
+    import numpy as np
+    mat = ctx.pmat_alloc(n, nb)
+    try:
+        ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+        info = ctx.pdgesv(rp.palg_default(), mat)
+        if info != 0:
+            raise SystemExit("FP64 reference solve: zero pivot at %d" % info)
+        chk = ctx.pdtest_check(mat)
+        return ctx.download_vec(mat.X, n, np.float64), chk["resid"]
+    finally:
+        ctx.pmat_free(mat)
+
+
 def run_fp64(ctx, rp, args):
     import numpy as np
     n, nb = args.n, args.nb
@@ -153,14 +192,13 @@ def run_fp64(ctx, rp, args):
                       "against": "LAPACK dgetrf on the N x 64 column prefix"}
     if args.check_pivots > 0:
         import scipy.linalg.lapack as la
-
-        from oracle import oracle as o
         k = min(args.check_pivots, n)
         log("prefix pivot check: first %d pivots vs LAPACK dgetrf" % k)
         gpiv = ctx.download_vec(mat.ipiv, k, np.int32)
-        pref = np.asfortranarray(o.matgen_tile(n, 0, 0, n, k))
+        pref = prefix_matrix(ctx, rp, n, k, nb, 0.0)
         _, lpiv, _ = la.dgetrf(pref, overwrite_a=1)
-        pivots = {"prefix": k, "equal": bool(np.array_equal(gpiv, lpiv[:k]))}
+        pivots = {"prefix": k, "equal": bool(np.array_equal(gpiv, lpiv[:k])),
+                  "against": "LAPACK dgetrf on the N x %d column prefix" % k}
     ctx.pmat_free(mat)
 
     t_mean = sum(times) / len(times)
@@ -245,8 +283,26 @@ def run_mixed(ctx, rp, form, args):
         t, res = runs[-1]
         ipiv = ctx.download_vec(mat.ipiv, n, np.int32)
         ident = int(np.sum(ipiv != np.arange(n)))
+        x = ctx.download_vec(mat.X, n, np.float64) if form == "bf16" else None
     finally:
         ctx.mxp_free(mat)
+    gold = gold_prefix(n, shift)
+    pivots = None
+    if gold is not None:
+        k = len(gold)
+        agree = int(np.sum(ipiv[:k] == gold))
+        pivots = {"prefix": k, "agree": agree, "equal": agree == k,
+                  "against": "LAPACK dgetrf (FP64) on the N x %d column "
+                             "prefix (committed fixture)" % k}
+    xcmp = None
+    if x is not None and args.x_check:
+        log("%s: FP64 reference solve at N=%d for the x comparison" % (form, n))
+        x64, r64 = x_fp64(ctx, rp, n, args.nb)
+        d = x - x64
+        xcmp = {"rel_inf": float(np.max(np.abs(d)) / np.max(np.abs(x64))),
+                "rel_2": float(np.linalg.norm(d) / np.linalg.norm(x64)),
+                "fp64_residual": round(r64, 6),
+                "fp64_nb": args.nb}
     sb = 4 if storage == "fp32" else 2
     ops, hbm = update_traffic(n, nb, update, terms, sb)
     sec = res["update_ms"] * 1e-3
@@ -265,6 +321,8 @@ def run_mixed(ctx, rp, form, args):
         "factor_ms": round(res["factor_ms"], 1),
         "solve_ms": round(res["solve_ms"], 1),
         "pivots_off_diagonal": ident,
+        "pivot_prefix_check": pivots,
+        "x_vs_fp64": xcmp,
         "update_roofline": {
             "bound": "mfma" if fr_m >= fr_h else "hbm",
             "mfma_achieved": round(mfma, 2), "mfma_peak": peak,
@@ -401,7 +459,10 @@ def main():
                          "for N <= 32768, none above)")
     ap.add_argument("--check-pivots", type=int, default=0,
                     help="compare the first K pivots with LAPACK dgetrf on the "
-                         "N x K column prefix (prefix oracle)")
+                         "N x K column prefix")
+    ap.add_argument("--no-x-check", dest="x_check", action="store_false",
+                    help="skip the FP64 solve that the bf16 form's x is "
+                         "compared with")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only run a device copy of this size (counter "
                          "calibration) and exit")

@@ -12,7 +12,8 @@ this script against an independent computation before it is written:
     (LCG jump by repeated squaring in Python integers);
   * the first 64 pivots of the N=193280 matrix, from LAPACK dgetrf on the
     N x 64 column prefix (the first K pivots of LU(A) are those of
-    LU(A[:, :K])).
+    LU(A[:, :K])), and likewise of the N=131072 matrix and of the N=229376
+    matrix with N/4 added to its diagonal (the mixed-precision forms).
 
 Run from the repository root: python tests/golden/make_golden.py
 """
@@ -90,6 +91,18 @@ def main():
     _, lpiv, info = la.dgetrf(pref, overwrite_a=1)
     assert info == 0
     out["prefix_ipiv_193280_64"] = lpiv[:k].astype(np.int32)
+    # ... and of the mixed-precision forms' matrices (the FP8 form's with
+    # N/4 added to the diagonal)
+    for n, shift in ((131072, 0.0), (229376, 229376 / 4.0)):
+        pref = np.asfortranarray(o.matgen_tile(n, 0, 0, n, k))
+        for j in range(k):
+            pref[j, j] += shift
+        _, lpiv, info = la.dgetrf(pref, overwrite_a=1)
+        assert info == 0
+        key = "prefix_ipiv_%d_64" % n + ("_shift%d" % shift if shift else "")
+        out[key] = lpiv[:k].astype(np.int32)
+        if shift:  # column diagonal dominance keeps every pivot on the diagonal
+            assert np.array_equal(out[key], np.arange(k))
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "hpl_golden.npz")
     np.savez_compressed(path, **out)
