@@ -26,7 +26,8 @@
 // largest e such that max|tile| * 2^e <= 448 (the e4m3 maximum), is rounded
 // to e4m3 (round to nearest even) and keeps e.  Each 128-deep MFMA product
 // of an A tile with a B tile is multiplied by 2^-(eA + eB) and added to the
-// FP32 accumulators.
+// FP32 accumulators: the MFMA applies the factor itself through its E8M0
+// operand scales (127 - eA, 127 - eB) and accumulates in place.
 //
 // Tile: BM x BN = 64*WM x 64*WN with WM x WN waves, each wave 64 x 64 as
 // 4 x 4 blocks of 16 x 16.  The MFMA computes the transposed block
@@ -532,14 +533,16 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
   for(int i = 0; i < 4; ++i)
 #pragma unroll
     for(int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
 
   load(0);
   for(int ks = 0; ks < nks; ++ks) {
     store();
     __syncthreads();
     if(ks + 1 < nks) load(ks + 1);
-    const float s = ldexpf(1.0f, -(ea[atile * nks + ks] + eb[ks * ntn128 + btile]));
+    // the tiles' power-of-two scales as the MFMA's E8M0 block scales
+    // (2^(e - 127) per operand): 2^-(eA + eB) on every product
+    const int sa = 127 - ea[atile * nks + ks];
+    const int sb = 127 - eb[ks * ntn128 + btile];
     i32x8 fa[4], fb[4];
 #pragma unroll
     for(int i = 0; i < 4; ++i) {
@@ -559,11 +562,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
     for(int i = 0; i < 4; ++i)
 #pragma unroll
       for(int j = 0; j < 4; ++j) {
-        // format 0 = e4m3 for both operands, E8M0 scales 127 = 2^0
-        const f32x4 d = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-            fb[j], fa[i], zero, 0, 0, 0, 127, 0, 127);
-#pragma unroll
-        for(int r = 0; r < 4; ++r) acc[i][j][r] = fmaf(d[r], s, acc[i][j][r]);
+        // format 0 = e4m3 for both operands
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+            fb[j], fa[i], acc[i][j], 0, 0, 0, sb, 0, sa);
       }
     __syncthreads();
   }
