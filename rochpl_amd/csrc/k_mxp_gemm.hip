@@ -477,7 +477,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
   }
 }
 
-template <int WM, int WN, typename S>
+template <int WM, int WN, int KS, typename S>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
     int M, int N, int nks, const uint8_t* __restrict__ Ap, int m_pad,
     const int* __restrict__ ea, const uint8_t* __restrict__ Bp, int n_pad,
@@ -486,8 +486,10 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
   constexpr int NT  = 64 * WM * WN;
   constexpr int BM  = 64 * WM;
   constexpr int BN  = 64 * WN;
-  constexpr int RS  = 144;  // LDS row stride in bytes
-  constexpr int ACH = BM * 8 / NT;  // 16-byte chunks per thread (128 B rows)
+  // LDS row: KS stages of 128 bytes, padded by 16 (68 dwords for KS = 2: the
+  // 16 rows of a fragment read start 4 banks apart)
+  constexpr int RS  = KS * 128 + 16;
+  constexpr int ACH = BM * 8 / NT;  // 16-byte chunks per thread per stage
   constexpr int BCH = BN * 8 / NT;
   constexpr int kOps = (BM + BN) * RS;
   __shared__ __attribute__((aligned(16)))
@@ -503,28 +505,37 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
   const int l16 = lane & 15, lk = lane >> 4;
   const int atile = (m0 + wm0) / 128;  // this wave's scale tiles
   const int btile = (n0 + wn0) / 128;
+  const int nit   = (nks + KS - 1) / KS;
 
-  i32x4 ra[ACH], rb[BCH];
-  auto load = [&](int ks) {
-    const uint8_t* a = Ap + (static_cast<size_t>(ks) * m_pad + m0) * 128;
-    const uint8_t* b = Bp + (static_cast<size_t>(ks) * n_pad + n0) * 128;
+  i32x4 ra[KS][ACH], rb[KS][BCH];
+  auto load = [&](int it) {
 #pragma unroll
-    for(int c = 0; c < ACH; ++c)
-      ra[c] = *reinterpret_cast<const i32x4*>(a + (t + c * NT) * 16);
+    for(int h = 0; h < KS; ++h) {
+      const int ks = it * KS + h;
+      if(ks >= nks) break;
+      const uint8_t* a = Ap + (static_cast<size_t>(ks) * m_pad + m0) * 128;
+      const uint8_t* b = Bp + (static_cast<size_t>(ks) * n_pad + n0) * 128;
 #pragma unroll
-    for(int c = 0; c < BCH; ++c)
-      rb[c] = *reinterpret_cast<const i32x4*>(b + (t + c * NT) * 16);
+      for(int c = 0; c < ACH; ++c)
+        ra[h][c] = *reinterpret_cast<const i32x4*>(a + (t + c * NT) * 16);
+#pragma unroll
+      for(int c = 0; c < BCH; ++c)
+        rb[h][c] = *reinterpret_cast<const i32x4*>(b + (t + c * NT) * 16);
+    }
   };
   auto store = [&]() {
 #pragma unroll
-    for(int c = 0; c < ACH; ++c) {
-      const int ch = t + c * NT;
-      *reinterpret_cast<i32x4*>(&sA[(ch >> 3) * RS + (ch & 7) * 16]) = ra[c];
-    }
+    for(int h = 0; h < KS; ++h) {
 #pragma unroll
-    for(int c = 0; c < BCH; ++c) {
-      const int ch = t + c * NT;
-      *reinterpret_cast<i32x4*>(&sB[(ch >> 3) * RS + (ch & 7) * 16]) = rb[c];
+      for(int c = 0; c < ACH; ++c) {
+        const int ch = t + c * NT;
+        *reinterpret_cast<i32x4*>(&sA[(ch >> 3) * RS + h * 128 + (ch & 7) * 16]) = ra[h][c];
+      }
+#pragma unroll
+      for(int c = 0; c < BCH; ++c) {
+        const int ch = t + c * NT;
+        *reinterpret_cast<i32x4*>(&sB[(ch >> 3) * RS + h * 128 + (ch & 7) * 16]) = rb[h][c];
+      }
     }
   };
 
@@ -535,37 +546,41 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
     for(int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   load(0);
-  for(int ks = 0; ks < nks; ++ks) {
+  for(int it = 0; it < nit; ++it) {
     store();
     __syncthreads();
-    if(ks + 1 < nks) load(ks + 1);
-    // the tiles' power-of-two scales as the MFMA's E8M0 block scales
-    // (2^(e - 127) per operand): 2^-(eA + eB) on every product
-    const int sa = 127 - ea[atile * nks + ks];
-    const int sb = 127 - eb[ks * ntn128 + btile];
-    i32x8 fa[4], fb[4];
+    if(it + 1 < nit) load(it + 1);
 #pragma unroll
-    for(int i = 0; i < 4; ++i) {
-      const uint8_t* p = &sA[(wm0 + i * 16 + l16) * RS + lk * 32];
-      const i32x4    lo = *reinterpret_cast<const i32x4*>(p);
-      const i32x4    hi = *reinterpret_cast<const i32x4*>(p + 16);
-      fa[i] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-    }
+    for(int h = 0; h < KS; ++h) {
+      const int ks = it * KS + h;
+      if(ks >= nks) break;
+      // the tiles' power-of-two scales as the MFMA's E8M0 block scales
+      // (2^(e - 127) per operand): 2^-(eA + eB) on every product
+      const int sa = 127 - ea[atile * nks + ks];
+      const int sb = 127 - eb[ks * ntn128 + btile];
+      i32x8 fa[4], fb[4];
 #pragma unroll
-    for(int j = 0; j < 4; ++j) {
-      const uint8_t* p = &sB[(wn0 + j * 16 + l16) * RS + lk * 32];
-      const i32x4    lo = *reinterpret_cast<const i32x4*>(p);
-      const i32x4    hi = *reinterpret_cast<const i32x4*>(p + 16);
-      fb[j] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-    }
-#pragma unroll
-    for(int i = 0; i < 4; ++i)
+      for(int i = 0; i < 4; ++i) {
+        const uint8_t* p = &sA[(wm0 + i * 16 + l16) * RS + h * 128 + lk * 32];
+        const i32x4    lo = *reinterpret_cast<const i32x4*>(p);
+        const i32x4    hi = *reinterpret_cast<const i32x4*>(p + 16);
+        fa[i] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      }
 #pragma unroll
       for(int j = 0; j < 4; ++j) {
-        // format 0 = e4m3 for both operands
-        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-            fb[j], fa[i], acc[i][j], 0, 0, 0, sb, 0, sa);
+        const uint8_t* p = &sB[(wn0 + j * 16 + l16) * RS + h * 128 + lk * 32];
+        const i32x4    lo = *reinterpret_cast<const i32x4*>(p);
+        const i32x4    hi = *reinterpret_cast<const i32x4*>(p + 16);
+        fb[j] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
       }
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+          // format 0 = e4m3 for both operands
+          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+              fb[j], fa[i], acc[i][j], 0, 0, 0, sb, 0, sa);
+    }
     __syncthreads();
   }
 
@@ -592,6 +607,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
 }
 
 constexpr int kWM = 2, kWN = 2;  // 128 x 128 workgroup tiles
+// 128-deep FP8 stages per LDS fill: one (2.36 ms at 32768 x 32768 x 1024
+// with FP16 C; two stages per fill took 3.06 ms, four 4.71 ms)
+constexpr int kFp8Stages = 1;
 
 inline long tile_grid(int tiles_m, int tiles_n) {
   const long T = static_cast<long>(tiles_m) * tiles_n;
@@ -646,7 +664,7 @@ hipError_t gemm_fp8_t(hipStream_t s, int m, int n, int K, const MxpPacked& P,
   if(grid > 0x7fffffffL) return hipErrorInvalidValue;
   const int nks = (K + 127) / 128;
   const dim3 g(static_cast<unsigned>(grid)), b(64 * kWM * kWN);
-  hipLaunchKernelGGL((gemm_fp8_kernel<kWM, kWN, S>), g, b, 0, s, m, n, nks,
+  hipLaunchKernelGGL((gemm_fp8_kernel<kWM, kWN, kFp8Stages, S>), g, b, 0, s, m, n, nks,
                      static_cast<const uint8_t*>(P.a), P.m_pad, P.ea,
                      static_cast<const uint8_t*>(P.b), P.n_pad, P.eb,
                      P.n_pad / 128, C, ldc, tiles_m, tiles_n,
