@@ -5,8 +5,9 @@
 // trailing update (rocblas_dtrsm at src/pfact/HPL_pdrpanrlT.cpp:124-135 and
 // src/pgesv/HPL_pdupdateTT.cpp:95-106), for the explicit inverse of L11, for
 // the diagonal blocks of the back-solve (rocblas_dtrsv at
-// src/pgesv/HPL_pdtrsv_device.cpp:170-178,279-287), and for the transposed
-// solves of the adjoint.
+// src/pgesv/HPL_pdtrsv_device.cpp:170-178,279-287), for the transposed
+// solves of the adjoint, and (diag_inv_kernel) for the FP64 inverses of the
+// diagonal blocks of the mixed-precision factors.
 //
 // x lives in LDS.  The block is processed in 64-row chunks along the sweep
 // direction: wave 0 solves the chunk's 64 x 64 triangle with the chunk's rows
@@ -37,8 +38,8 @@ __device__ inline V op_at(const MT* T, int ldt, int i, int j) {
 // LOWER: op(T) is lower triangular (forward sweep), else upper (backward).
 // UNIT: unit diagonal (not read).  IDENTITY: b is column `col` of I.
 // V: element type of X and of the arithmetic (double, or float for the FP32
-// panel); MT: element type of T (V, or FP32 / FP16 factors applied to FP64
-// vectors in the mixed-precision solves).
+// panel); MT: element type of T (V, or the FP32 / FP16 factors whose
+// diagonal blocks are inverted in FP64).
 // One workgroup solves for the column xc (column `col` of the identity when
 // IDENTITY).
 template <typename MT, typename V, bool LOWER, bool TRANS, bool UNIT,
@@ -192,53 +193,6 @@ hipError_t launch_linv(hipStream_t s, int jb, const double* L, int ldl,
 hipError_t launch_linv(hipStream_t s, int jb, const float* L, int ldl,
                        float* Linv, int ldi) {
   return linv_t(s, jb, L, ldl, Linv, ldi);
-}
-
-template <typename MT>
-hipError_t tri_solve_vec_t(hipStream_t s, bool lower, int k, const MT* T,
-                           int ldt, double* x) {
-  if(k <= 0) return hipSuccess;
-  if(k > kMaxNB) return hipErrorInvalidValue;
-  if(lower)
-    hipLaunchKernelGGL((tri_solve_kernel<MT, double, true, false, true, false>),
-                       dim3(1), dim3(256), 0, s, k, T, ldt, x, k);
-  else
-    hipLaunchKernelGGL((tri_solve_kernel<MT, double, false, false, false, false>),
-                       dim3(1), dim3(256), 0, s, k, T, ldt, x, k);
-  return hipGetLastError();
-}
-
-template <typename MT>
-hipError_t diag_inverse_t(hipStream_t s, bool lower, int nblk, int kb,
-                          const MT* T, int ldt, double* Inv) {
-  if(nblk <= 0 || kb <= 0) return hipSuccess;
-  if(kb > kMaxNB || nblk > 65535) return hipErrorInvalidValue;
-  const dim3 g(kb, nblk);
-  if(lower)
-    hipLaunchKernelGGL((diag_inv_kernel<MT, true>), g, dim3(256), 0, s, kb, T, ldt, Inv);
-  else
-    hipLaunchKernelGGL((diag_inv_kernel<MT, false>), g, dim3(256), 0, s, kb, T, ldt, Inv);
-  return hipGetLastError();
-}
-
-hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
-                               const float* T, int ldt, double* Inv) {
-  return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
-}
-
-hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
-                               const __half* T, int ldt, double* Inv) {
-  return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
-}
-
-hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
-                                const float* T, int ldt, double* x) {
-  return tri_solve_vec_t(s, lower, k, T, ldt, x);
-}
-
-hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
-                                const __half* T, int ldt, double* x) {
-  return tri_solve_vec_t(s, lower, k, T, ldt, x);
 }
 
 hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const double* L,

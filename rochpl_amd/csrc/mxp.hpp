@@ -58,12 +58,6 @@ hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
 constexpr int kMaxSwapPairs = 2 * kMaxNB;
 hipError_t launch_apply_swaps(hipStream_t s, int cnt, const int* pairs,
                               double* y);
-// op(T) x = x for one FP64 vector, T in storage type (k <= kMaxNB):
-// lower unit (forward) or upper non-unit (backward).
-hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
-                                const float* T, int ldt, double* x);
-hipError_t launch_tri_solve_vec(hipStream_t s, bool lower, int k,
-                                const __half* T, int ldt, double* x);
 // y[0:m] -= M[0:m, 0:k] * x[0:k], M in storage type, FP64 accumulation;
 // if xback is not null, x[0:k] is also copied there.
 hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const float* M,
