@@ -195,6 +195,29 @@ hipError_t launch_linv(hipStream_t s, int jb, const float* L, int ldl,
   return linv_t(s, jb, L, ldl, Linv, ldi);
 }
 
+template <typename MT>
+hipError_t diag_inverse_t(hipStream_t s, bool lower, int nblk, int kb,
+                          const MT* T, int ldt, double* Inv) {
+  if(nblk <= 0 || kb <= 0) return hipSuccess;
+  if(kb > kMaxNB || nblk > 65535) return hipErrorInvalidValue;
+  const dim3 g(kb, nblk);
+  if(lower)
+    hipLaunchKernelGGL((diag_inv_kernel<MT, true>), g, dim3(256), 0, s, kb, T, ldt, Inv);
+  else
+    hipLaunchKernelGGL((diag_inv_kernel<MT, false>), g, dim3(256), 0, s, kb, T, ldt, Inv);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const float* T, int ldt, double* Inv) {
+  return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
+}
+
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const __half* T, int ldt, double* Inv) {
+  return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
+}
+
 hipError_t launch_trsm_colsweep(hipStream_t s, int k, int n, const double* L,
                                 int ldl, double* B, int ldb) {
   return launch_tri_solve(s, true, false, true, k, n, L, ldl, B, ldb);
