@@ -15,8 +15,9 @@
 //      sequence number with release ordering;
 //   3. one wave of every workgroup polls all slots' sequence numbers (each
 //      lane its slots at once) and takes one acquire fence; then every thread
-//      loads a share of all candidate rows and the top row into LDS, and each
-//      wave reduces the candidates itself (no further barrier);
+//      loads a share of all candidate rows (16-byte loads, all in flight
+//      before the first LDS store) and the top row into LDS, and each wave
+//      reduces the candidates itself (no further barrier);
 //   4. registers are updated: row interchange, division by the pivot,
 //      rank-1 update of the remaining leaf columns.
 // Slots are double-buffered by column parity, so a workgroup that runs ahead
@@ -82,7 +83,7 @@ struct LeafArgs {
 // Shared-memory scratch of the leaf kernel (the exchange works in double;
 // FP32 rows convert exactly).
 struct LeafShared {
-  double cand[kMaxLeafGrid][kLeafMax];  // every workgroup's candidate row
+  alignas(16) double cand[kMaxLeafGrid][kLeafMax];  // every workgroup's candidate row
   double cv[kMaxLeafGrid];
   int    cr[kMaxLeafGrid];
   double top[kLeafMax];  // current top row
@@ -231,16 +232,32 @@ __device__ __forceinline__ int exchange(const LeafArgs* ap, int jj,
   }
   __syncthreads();
   if(!sh.ok) return -1;
-  for(int i = t; i < G * kLeafMax; i += kLeafThreads) {
-    const int g = i / kLeafMax, c = i % kLeafMax;
-    sh.cand[g][c] = slots[g * kSlotStride + kSlotData + c];
-  }
   if(t < G) {
     sh.cv[t] = slots[t * kSlotStride + kSlotVal];
     sh.cr[t] = static_cast<int>(slots[t * kSlotStride + kSlotRow]);
   } else if(t >= kLeafThreads - kLeafMax) {
     const int c = t - (kLeafThreads - kLeafMax);
     sh.top[c]   = slots[kSlotTop + c];
+  }
+  // all candidate rows as 16-byte loads, issued four per thread before
+  // their LDS stores (one memory latency per 128 workgroups)
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  constexpr int kHalf = kLeafMax / 2;
+  for(int i0 = t; i0 < G * kHalf; i0 += 4 * kLeafThreads) {
+    d2 rv[4];
+#pragma unroll
+    for(int k = 0; k < 4; ++k) {
+      const int i = i0 + k * kLeafThreads;
+      if(i < G * kHalf)
+        rv[k] = *reinterpret_cast<const d2*>(
+            &slots[(i / kHalf) * kSlotStride + kSlotData + 2 * (i % kHalf)]);
+    }
+#pragma unroll
+    for(int k = 0; k < 4; ++k) {
+      const int i = i0 + k * kLeafThreads;
+      if(i < G * kHalf)
+        *reinterpret_cast<d2*>(&sh.cand[i / kHalf][2 * (i % kHalf)]) = rv[k];
+    }
   }
   __syncthreads();
   // every wave reduces the candidates itself
