@@ -118,6 +118,43 @@ def test_pdfact_matches_oracle(ctx, m, jb, nbmin, ndiv):
     assert np.max(np.abs(got - P)) < 1e-11
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_pdfact_exact_ties_take_the_first_row(ctx, precision):
+    """Exact ties of max |a| go to the first row, as in the oracle and
+    LAPACK's idamax.  (The reference compares with a strict '>' in its
+    reductions, include/hpl_hip_ex.hpp:689-730,768-777, so its winner on an
+    exact tie depends on its reduction order; the generator's 64-bit states
+    make exact ties impossible in HPL matrices.)  Column 0 holds |a| = 0.9
+    at rows that share a wave (900, 901), lie in another wave (964) and in
+    other workgroups (2500, 3100, 4000; a workgroup holds 1536 rows), with
+    both signs; every other entry is below 0.5 in magnitude, so only this
+    tie occurs."""
+    m, jb = 5000, 64
+    say("pdfact ties %s m=%d jb=%d" % (precision, m, jb))
+    rng = np.random.default_rng(42)
+    P0 = np.asfortranarray(rng.uniform(-0.45, 0.45, (m, jb)))
+    for r, sgn in ((4000, 1), (2500, -1), (964, 1), (901, -1), (900, 1),
+                   (3100, -1)):
+        P0[r, 0] = 0.9 * sgn
+    dpiv = ctx.alloc(4 * jb)
+    if precision == "fp64":
+        dP, lda = upload_matrix(ctx, P0, m + 11)
+        info = ctx.pdfact(dP.ptr, lda, m, jb, dpiv.ptr, 32, 2, 0)
+        gpiv = dpiv.download(jb, np.int32)
+        wpiv, winfo = o.pdfact(P0.copy(order="F"))
+        assert info == winfo == 0
+        assert np.array_equal(gpiv, wpiv)
+    else:
+        lda = m + 11
+        P32 = np.zeros((lda, jb), np.float32, order="F")
+        P32[:m] = P0
+        dP = ctx.alloc(4 * lda * jb)
+        dP.upload(P32)
+        assert ctx.spdfact(dP.ptr, lda, m, jb, dpiv.ptr) == 0
+        gpiv = dpiv.download(jb, np.int32)
+    assert gpiv[0] == 900
+
+
 def test_pdfact_zero_pivot_info(ctx):
     m, jb = 300, 40
     say("pdfact zero pivot m=%d jb=%d" % (m, jb))
