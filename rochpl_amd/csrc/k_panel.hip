@@ -266,15 +266,29 @@ hipError_t launch_leaf(Ctx& c, V* P, int lda, int m, int JB, int c0, int w,
                          0, c.stream);
 }
 
-inline hipError_t panel_gemm(hipStream_t s, int m, int n, int k,
-                             const double* A, int lda, const double* B,
-                             int ldb, double* C, int ldc) {
-  return launch_dgemm(s, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc, false);
+// C (m x n) -= A (m x k) B (k x n) inside the panel recursion
+// (rocblas_dgemm at src/pfact/HPL_pdrpanrlT.cpp:142-154).  FP64 products at
+// least one 128-column tile wide go through the streamed trailing-update
+// kernel with B transposed into the context's L11^{-1} buffer, which is idle
+// while a panel is factored; the others, and FP32, through the generic GEMM.
+inline hipError_t panel_gemm(Ctx& c, int m, int n, int k, const double* A,
+                             int lda, const double* B, int ldb, double* C,
+                             int ldc) {
+  const int ldt = ut_ld(n);
+  if(n >= 128 && static_cast<size_t>(k) * ldt <= static_cast<size_t>(kMaxNB) * kMaxNB &&
+     update_fast_ok(m, n, k, A, lda, c.Linv, ldt, m, n)) {
+    hipError_t e = launch_transpose(c.stream, k, n, B, ldb, c.Linv, ldt);
+    if(e != hipSuccess) return e;
+    return launch_update(c.stream, m, n, k, A, lda, c.Linv, ldt, C, ldc, m, n);
+  }
+  return launch_dgemm(c.stream, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc,
+                      false);
 }
-inline hipError_t panel_gemm(hipStream_t s, int m, int n, int k,
-                             const float* A, int lda, const float* B, int ldb,
-                             float* C, int ldc) {
-  return launch_sgemm(s, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc, false);
+inline hipError_t panel_gemm(Ctx& c, int m, int n, int k, const float* A,
+                             int lda, const float* B, int ldb, float* C,
+                             int ldc) {
+  return launch_sgemm(c.stream, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc,
+                      false);
 }
 
 // Columns [c0, c0+n) of the panel, rows [c0, m).
@@ -297,7 +311,7 @@ hipError_t leaf_or_chunks(Ctx& c, V* P, int lda, int m, int JB, int c0, int n,
       V* A12 = P + top + static_cast<size_t>(top + w) * lda;
       e = launch_trsm_colsweep(c.stream, w, rest, L11, lda, A12, lda);
       if(e != hipSuccess) return e;
-      e = panel_gemm(c.stream, m - top - w, rest, w, L11 + w, lda, A12, lda,
+      e = panel_gemm(c, m - top - w, rest, w, L11 + w, lda, A12, lda,
                      A12 + w, lda);
       if(e != hipSuccess) return e;
     }
@@ -324,7 +338,7 @@ hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, V* P, int lda, int m, int JB,
       V* A12 = P + top + static_cast<size_t>(top + jb) * lda;
       e = launch_trsm_colsweep(c.stream, jb, nrem, L11, lda, A12, lda);
       if(e != hipSuccess) return e;
-      e = panel_gemm(c.stream, m - top - jb, nrem, jb, L11 + jb, lda, A12, lda,
+      e = panel_gemm(c, m - top - jb, nrem, jb, L11 + jb, lda, A12, lda,
                      A12 + jb, lda);
       if(e != hipSuccess) return e;
     }
