@@ -162,6 +162,9 @@ def test_spdfact_is_partial_pivoting_lu(ctx, m, jb, nbmin):
 @pytest.mark.parametrize("storage,update,n,nb,shift", [
     (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_FP32, 2048, 256, 0.0),
     (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1536, 512, 384.0),
+    # a last block smaller than NB (its inverse and interchanges apart)
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_FP32, 1000, 128, 0.0),
+    (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1300, 256, 325.0),
 ])
 def test_mxp_getrs_and_residual(ctx, storage, update, n, nb, shift):
     say("mxp getrs storage=%d n=%d nb=%d" % (storage, n, nb))
