@@ -283,6 +283,36 @@ def test_pdgesv_matches_oracle(ctx, n, nb):
     assert chk["resid_r0_form"] <= chk["resid"]
 
 
+def test_pdgesv_full_pivot_vector_matches_lapack_at_16384(ctx):
+    """The whole pivot vector of a larger solve against LAPACK dgetrf on the
+    host (partial pivoting, first row on ties; exact ties cannot occur in the
+    generator's matrices), with NB = 896 so that the last panel is ragged, and
+    x against LAPACK's solve to cond * eps."""
+    import scipy.linalg as sl
+    import scipy.linalg.lapack as la
+    n, nb = 16384, 896
+    say("pdgesv full pivots n=%d nb=%d vs LAPACK" % (n, nb))
+    mat = ctx.pmat_alloc(n, nb)
+    try:
+        ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+        assert ctx.pdgesv(rp.palg_default(), mat) == 0
+        x = ctx.download_vec(mat.X, n)
+        ipiv = ctx.download_vec(mat.ipiv, n, np.int32)
+        chk = ctx.pdtest_check(mat)
+    finally:
+        ctx.pmat_free(mat)
+    Ab = o.matgen(n, n + 1)
+    lu, lpiv, info = la.dgetrf(np.asfortranarray(Ab[:, :n]))
+    assert info == 0
+    assert np.array_equal(ipiv, lpiv)
+    xl = sl.lu_solve((lu, lpiv), Ab[:, n])
+    err = np.max(np.abs(x - xl)) / np.max(np.abs(xl))
+    print("[parity] n=%d: |x - x_lapack|_oo / |x|_oo = %.3e, resid %.4f" % (
+        n, err, chk["resid"]), flush=True)
+    assert err < 1e-7
+    assert chk["resid"] < 16.0
+
+
 def test_pdgesv_zero_pivot_returns_info(ctx):
     n, nb = 200, 32
     say("pdgesv zero pivot n=%d" % n)
