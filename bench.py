@@ -107,14 +107,18 @@ def prefix_matrix(ctx, rp, n, k, nb, shift):
 
 
 def gold_prefix(n, shift):
-    """Committed LAPACK prefix pivots of the N x 64 prefix, or None."""
+    """Committed LAPACK pivots of the longest N x K column prefix (K = 4096
+    or 64) of the seeded matrix plus shift on the diagonal, or None."""
     import numpy as np
     path = os.path.join(ROOT, "tests", "golden", "hpl_golden.npz")
-    key = "prefix_ipiv_%d_64" % n + ("_shift%d" % shift if shift else "")
     if not os.path.exists(path):
         return None
     fx = np.load(path, allow_pickle=False)
-    return fx[key] if key in fx.files else None
+    for k in (4096, 64):
+        key = "prefix_ipiv_%d_%d" % (n, k) + ("_shift%d" % shift if shift else "")
+        if key in fx.files:
+            return fx[key]
+    return None
 
 
 def x_fp64(ctx, rp, n, nb):
@@ -180,16 +184,16 @@ def run_fp64(ctx, rp, args):
             stats.append(st)
 
     pivots = None
-    gold = os.path.join(ROOT, "tests", "golden", "hpl_golden.npz")
-    if args.check_pivots == 0 and os.path.exists(gold):
-        # the first 64 pivots against LAPACK dgetrf on the N x 64 column
-        # prefix of the same matrix (committed fixture)
-        key = "prefix_ipiv_%d_64" % n
-        fx = np.load(gold, allow_pickle=False)
-        if key in fx.files:
-            gpiv = ctx.download_vec(mat.ipiv, 64, np.int32)
-            pivots = {"prefix": 64, "equal": bool(np.array_equal(gpiv, fx[key])),
-                      "against": "LAPACK dgetrf on the N x 64 column prefix"}
+    if args.check_pivots == 0:
+        # the leading pivots against LAPACK dgetrf on the N x K column
+        # prefix of the same matrix (committed fixture, longest K available)
+        gold = gold_prefix(n, 0.0)
+        if gold is not None:
+            k = len(gold)
+            gpiv = ctx.download_vec(mat.ipiv, k, np.int32)
+            pivots = {"prefix": k, "equal": bool(np.array_equal(gpiv, gold)),
+                      "against": "LAPACK dgetrf on the N x %d column prefix "
+                                 "(committed fixture)" % k}
     if args.check_pivots > 0:
         import scipy.linalg.lapack as la
         k = min(args.check_pivots, n)

@@ -13,7 +13,8 @@ this script against an independent computation before it is written:
   * the first 64 pivots of the N=193280 matrix, from LAPACK dgetrf on the
     N x 64 column prefix (the first K pivots of LU(A) are those of
     LU(A[:, :K])), and likewise of the N=131072 matrix and of the N=229376
-    matrix with N/4 added to its diagonal (the mixed-precision forms).
+    matrix with N/4 added to its diagonal (the mixed-precision forms); the
+    same for the first 4096 pivots of all three (K = 4096).
 
 Run from the repository root: python tests/golden/make_golden.py
 """
@@ -103,6 +104,20 @@ def main():
         out[key] = lpiv[:k].astype(np.int32)
         if shift:  # column diagonal dominance keeps every pivot on the diagonal
             assert np.array_equal(out[key], np.arange(k))
+    # ... and the first 4096 pivots of each (the longer check the bench runs)
+    kl = 4096
+    for n, shift in ((193280, 0.0), (131072, 0.0), (229376, 229376 / 4.0)):
+        pref = np.asfortranarray(o.matgen_tile(n, 0, 0, n, kl))
+        for j in range(kl):
+            pref[j, j] += shift
+        _, lpiv, info = la.dgetrf(pref, overwrite_a=1)
+        del pref
+        assert info == 0
+        key = "prefix_ipiv_%d_%d" % (n, kl) + ("_shift%d" % shift if shift else "")
+        out[key] = lpiv[:kl].astype(np.int32)
+        short = "prefix_ipiv_%d_64" % n + ("_shift%d" % shift if shift else "")
+        assert np.array_equal(out[key][:64], out[short]), key
+        print("%s: %d pivots" % (key, kl), flush=True)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "hpl_golden.npz")
     np.savez_compressed(path, **out)

@@ -37,7 +37,19 @@ def test_generator_spot_values(gold, n):
     assert np.all(np.abs(vals) <= 0.5)
 
 
-def test_prefix_pivots_fixture_is_a_permutation_step(gold):
-    p = gold["prefix_ipiv_193280_64"]
-    assert p.shape == (64,)
-    assert np.all(p >= np.arange(64)) and np.all(p < 193280)
+@pytest.mark.parametrize("key,n,k", [
+    ("prefix_ipiv_193280_64", 193280, 64),
+    ("prefix_ipiv_193280_4096", 193280, 4096),
+    ("prefix_ipiv_131072_64", 131072, 64),
+    ("prefix_ipiv_131072_4096", 131072, 4096),
+    ("prefix_ipiv_229376_64_shift57344", 229376, 64),
+    ("prefix_ipiv_229376_4096_shift57344", 229376, 4096)])
+def test_prefix_pivots_fixture_is_a_permutation_step(gold, key, n, k):
+    p = gold[key]
+    assert p.shape == (k,)
+    assert np.all(p >= np.arange(k)) and np.all(p < n)
+    # the short prefix is the start of the long one
+    short = gold[key.replace("_%d" % k, "_64", 1)] if k != 64 else p
+    assert np.array_equal(p[:64], short)
+    if "shift" in key:  # column diagonal dominance: no interchange
+        assert np.array_equal(p, np.arange(k))
