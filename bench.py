@@ -366,8 +366,9 @@ def _pmc_per_dispatch(path, counter, match):
 def update_probe(m, n, jb):
     """The FP64 trailing update (HPL_pdupdateTT: swaps, U12 = L11^-1 A12,
     A22 -= L21 U12) alone on an m x (jb + n) block, three times: the shape
-    the HBM counters are read on (the whole solve does not run under counter
-    collection)."""
+    the HBM counters are read on (under rocprofv3 --pmc the whole solve ends
+    with a host-side crash in the profiler right after the matrix
+    allocation, at N = 193280 and at N = 163840)."""
     import numpy as np
     import rochpl_amd as rp
     ctx = rp.Context(0)
