@@ -498,7 +498,7 @@ def main():
         copy_probe(args.copy_probe_gib)
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU (got --gpus %d); "
+        print("bench.py: runs on exactly one GPU (got --gpus %d); "
               "multi-GPU runs are not part of it" % args.gpus, flush=True)
         sys.exit(2)
 

@@ -29,7 +29,7 @@ extern "C" {
 /* ---- status codes ------------------------------------------------------ */
 #define HPLX_SUCCESS 0
 #define HPLX_ERR_INVALID -1     /* bad argument                            */
-#define HPLX_ERR_UNSUPPORTED -2 /* valid HPL setting this build lacks      */
+#define HPLX_ERR_UNSUPPORTED -2 /* valid HPL setting the library lacks    */
 #define HPLX_ERR_HIP -3         /* HIP runtime error                       */
 #define HPLX_ERR_ALLOC -4       /* device or pinned allocation failed      */
 

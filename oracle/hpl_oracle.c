@@ -1,8 +1,8 @@
 /*
  * hpl_oracle.c -- CPU restatement of rocHPL's P=Q=1 HPL_pdgesv path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
- * rochpl_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY.  Tests compare the HIP library in rochpl_amd/
+ * against it: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  Nothing in the product path links it.
  *
  * What it restates (reference files, paths relative to the rocHPL tree):

@@ -1,4 +1,4 @@
-"""TEST INFRASTRUCTURE ONLY -- the checker of the mixed-precision update.
+"""TEST INFRASTRUCTURE ONLY -- CPU restatement of the mixed-precision update.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline may import
 this module; the library never calls it.

@@ -4,7 +4,7 @@
 // (command line -P -Q -p -q -N -NB -f -it -i --version -h, the 31-line
 // HPL.dat, parameter echo) and src/HPL_pdtest.cpp (matgen, timed
 // HPL_pdgesv, W-line, residual check).  One process drives one GPU, so only
-// the P=Q=1 grid runs; other grids, and panel variants this build does not
+// the P=Q=1 grid runs; other grids, and panel variants the library does not
 // implement, are counted as skipped with a message, as the reference counts
 // tests it cannot run.
 //
@@ -444,7 +444,7 @@ int main(int argc, char** argv) {
     }
   }
   if(P * Q > 1) {
-    warn("Need at least %d processes for these tests; this build runs one "
+    warn("Need at least %d processes for these tests; xhplx runs one "
          "process on one GPU (P=Q=1)", P * Q);
     return 1;
   }
