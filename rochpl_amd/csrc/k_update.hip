@@ -10,16 +10,20 @@
 //     store 16 consecutive rows of a C column;
 //   - K in 8-deep stages through a ring of 4 LDS slots (66.5 KB, two
 //     workgroups per CU); each stage is 16 GLOBAL_LOAD_LDS_DWORDX4 (one
-//     128-double LDS row each, rows padded to 130 doubles), issued three
-//     stages ahead of its use;
-//   - per stage one wait for this wave's loads of the stage and one
-//     workgroup barrier, after which the slot of the previous stage is
-//     refilled;
+//     128-double LDS row each, rows padded to 130 doubles), issued two
+//     stages before the barrier that publishes it;
+//   - per stage one wait for this wave's loads of the next stage and one
+//     workgroup barrier, placed between the stage's two 4-deep halves:
+//     the second half's fragments are read before it and the next stage's
+//     first fragments right after it, so no MFMA block waits for an LDS
+//     read that the barrier held back; after the barrier the slot of the
+//     previous stage is refilled;
 //   - rows of L21 / columns of U12^T beyond the matrix are read from a
 //     clamped (in-bounds) address: they only reach C entries outside the
 //     matrix, which are not stored, so no masking is needed; K must be a
 //     multiple of 8 (HPL's NB);
-//   - workgroup -> tile mapping XCD-aware (8 XCDs, groups of 8 tile rows).
+//   - workgroup -> tile mapping XCD-aware (8 XCDs, groups of 8 tile rows);
+//   - interior tiles update C with up to 32 loads per lane in flight.
 //
 // The stage loads come from lds_stream.hpp; the kernel waits explicitly, no
 // compiler-issued vector memory access is in flight inside the K loop, and
@@ -83,44 +87,79 @@ __global__ __launch_bounds__(256, 2) void update_kernel(
   for(int i = 0; i < 4; ++i)
 #pragma unroll
     for(int j = 0; j < 4; ++j) acc[i][j] = ud4{0.0, 0.0, 0.0, 0.0};
-  auto compute = [&](double (*a)[UROW], double (*b)[UROW]) {
+  // fragments of one 4-deep half stage, and the 16 MFMAs that consume them
+  auto frag = [&](double (*a)[UROW], double (*b)[UROW], int ks, double* fa,
+                  double* fb) {
 #pragma unroll
-    for(int ks = 0; ks < UBK / 4; ++ks) {
-      double fa[4], fb[4];
+    for(int i = 0; i < 4; ++i) fa[i] = a[ks * 4 + lk][wm0 + i * 16 + l16];
 #pragma unroll
-      for(int i = 0; i < 4; ++i) fa[i] = a[ks * 4 + lk][wm0 + i * 16 + l16];
+    for(int j = 0; j < 4; ++j) fb[j] = b[ks * 4 + lk][wn0 + j * 16 + l16];
+  };
+  auto mma = [&](const double* fa, const double* fb) {
 #pragma unroll
-      for(int j = 0; j < 4; ++j) fb[j] = b[ks * 4 + lk][wn0 + j * 16 + l16];
+    for(int i = 0; i < 4; ++i)
 #pragma unroll
-      for(int i = 0; i < 4; ++i)
-#pragma unroll
-        for(int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i],
-                                                           acc[i][j], 0, 0, 0);
-    }
+      for(int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i],
+                                                         acc[i][j], 0, 0, 0);
   };
 
   issue(0, sA0, sB0);
   issue(1, sA1, sB1);
   issue(2, sA2, sB2);
-  // stage s lives in slot s % 4; unrolled by 4 so every slot is static
+  double f0a[4], f0b[4], f1a[4], f1b[4];
+  wait_vm<4 * (UNS - 2)>();
+  __builtin_amdgcn_s_barrier();
+  frag(sA0, sB0, 0, f0a, f0b);
+  // stage s lives in slot s % 4; unrolled by 4 so every slot is static.
+  // Per stage: read the second half's fragments, multiply the first half,
+  // wait for this wave's loads of stage s + 1 and pass the barrier that
+  // publishes them (and frees the slot of stage s - 1), refill that slot,
+  // read stage s + 1's first fragments, multiply the second half.
   for(int s = 0; s < nst; s += UNS) {
-#define HPLX_USTEP(q, ca, cb, na, nb)                                        \
+#define HPLX_USTEP(q, ca, cb, na, nb, ra, rb)                                \
   if(s + q < nst) {                                                          \
-    wait_vm<4 * (UNS - 2)>(); /* this wave's loads of stage s+q are in */    \
-    __builtin_amdgcn_s_barrier(); /* ... and every wave's; slot of s+q-1 free */ \
-    issue(s + q + UNS - 1, na, nb);                                          \
-    compute(ca, cb);                                                         \
+    frag(ca, cb, 1, f1a, f1b);                                               \
+    mma(f0a, f0b);                                                           \
+    wait_vm<4 * (UNS - 3)>(); /* this wave's loads of stage s+q+1 are in */  \
+    __builtin_amdgcn_s_barrier(); /* every wave's; slot of s+q-1 free */     \
+    issue(s + q + UNS - 1, ra, rb);                                          \
+    if(s + q + 1 < nst) frag(na, nb, 0, f0a, f0b);                           \
+    mma(f1a, f1b);                                                           \
   }
-    HPLX_USTEP(0, sA0, sB0, sA3, sB3)
-    HPLX_USTEP(1, sA1, sB1, sA0, sB0)
-    HPLX_USTEP(2, sA2, sB2, sA1, sB1)
-    HPLX_USTEP(3, sA3, sB3, sA2, sB2)
+    HPLX_USTEP(0, sA0, sB0, sA1, sB1, sA3, sB3)
+    HPLX_USTEP(1, sA1, sB1, sA2, sB2, sA0, sB0)
+    HPLX_USTEP(2, sA2, sB2, sA3, sB3, sA1, sB1)
+    HPLX_USTEP(3, sA3, sB3, sA0, sB0, sA2, sB2)
 #undef HPLX_USTEP
   }
   wait_vm<0>();  // the trailing reloads
 
   // D[4r + l/16][l%16] of block (i, j) is C[row = l16][col = 4r + lk]
+  if(m0 + 128 <= M && n0 + 128 <= N) {
+    // interior tile: the 16 values of block row i + 1 are loaded before
+    // those of block row i are stored, so up to 32 loads per lane are in
+    // flight instead of one load-store round trip per element
+    double* cb = C + (m0 + wm0 + l16) + static_cast<size_t>(n0 + wn0 + lk) * ldc;
+    double cv[2][16];
+#pragma unroll
+    for(int e = 0; e < 16; ++e)
+      cv[0][e] = cb[static_cast<size_t>((e >> 2) * 16 + 4 * (e & 3)) * ldc];
+#pragma unroll
+    for(int i = 0; i < 4; ++i) {
+      if(i + 1 < 4) {
+#pragma unroll
+        for(int e = 0; e < 16; ++e)
+          cv[(i + 1) & 1][e] =
+              cb[(i + 1) * 16 + static_cast<size_t>((e >> 2) * 16 + 4 * (e & 3)) * ldc];
+      }
+#pragma unroll
+      for(int e = 0; e < 16; ++e)
+        cb[i * 16 + static_cast<size_t>((e >> 2) * 16 + 4 * (e & 3)) * ldc] =
+            cv[i & 1][e] - acc[i][e >> 2][e & 3];
+    }
+    return;
+  }
 #pragma unroll
   for(int i = 0; i < 4; ++i) {
     const int row = m0 + wm0 + i * 16 + l16;
