@@ -393,7 +393,9 @@ def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, n, nb,
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
     counter reads for the copy probe of known size.  probe = (m, n, k) of the
     update probe, or None for the FP64 bench at (n, nb)."""
-    upd = lambda k: "update_kernel" in k
+    # the trailing update's instance (update_kernel<true> runs the panel
+    # products)
+    upd = lambda k: "update_kernel<false>" in k
     cpy = lambda k: "copy" in k.lower()
     f, _ = _pmc_per_dispatch(fetch_csv, "FETCH_SIZE", upd)
     w, _ = _pmc_per_dispatch(write_csv, "WRITE_SIZE", upd)
@@ -424,7 +426,7 @@ def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, n, nb,
             launches += 1
     shape = ({"update_probe": {"m": probe[0], "n": probe[1], "k": probe[2]}}
              if probe is not None else {"N": n, "NB": nb})
-    d = dict(shape, **{"kernel": "update_kernel",
+    d = dict(shape, **{"kernel": "update_kernel<false>",
          "launches": len(f), "fetch_kib_per_launch_raw": sum(f.values()) / len(f),
          "write_kib_per_launch_raw": sum(w.values()) / len(w),
          "copy_probe_bytes": copy_bytes, "copy_fetch_kib_raw": cf_kib,

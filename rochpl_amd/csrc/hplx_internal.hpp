@@ -131,12 +131,14 @@ hipError_t launch_sgemm(hipStream_t s, int m, int n, int k, double alpha,
 // The trailing update C -= A * Bt^T with Bt = U12^T stored n-contiguous
 // (k_update.hip): K a multiple of 8, 16-byte aligned operands, even leading
 // dimensions; a_rows / b_cols bound the (clamped) reads of A's columns and
-// Bt's rows.  update_fast_ok says whether a call qualifies.
+// Bt's rows.  update_fast_ok says whether a call qualifies.  panel = true
+// launches the instance kept for the panel products (a separate kernel name
+// in traces, same code).
 bool       update_fast_ok(int m, int n, int k, const double* A, int lda,
                           const double* Bt, int ldb, int a_rows, int b_cols);
 hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,
                          int lda, const double* Bt, int ldb, double* C,
-                         int ldc, int a_rows, int b_cols);
+                         int ldc, int a_rows, int b_cols, bool panel = false);
 // B (cols x rows, ldb) = A^T for A (rows x cols, lda), column-major both.
 hipError_t launch_transpose(hipStream_t s, int rows, int cols, const double* A,
                             int lda, double* B, int ldb);

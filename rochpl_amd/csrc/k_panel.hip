@@ -279,7 +279,8 @@ inline hipError_t panel_gemm(Ctx& c, int m, int n, int k, const double* A,
      update_fast_ok(m, n, k, A, lda, c.Linv, ldt, m, n)) {
     hipError_t e = launch_transpose(c.stream, k, n, B, ldb, c.Linv, ldt);
     if(e != hipSuccess) return e;
-    return launch_update(c.stream, m, n, k, A, lda, c.Linv, ldt, C, ldc, m, n);
+    return launch_update(c.stream, m, n, k, A, lda, c.Linv, ldt, C, ldc, m, n,
+                         true);
   }
   return launch_dgemm(c.stream, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc,
                       false);

@@ -48,6 +48,9 @@ typedef double ud4 __attribute__((ext_vector_type(4)));
 // C (M x N) -= A (M x K, column-major) * Bt^T (Bt: K rows of N, row-major,
 // i.e. column-major N x K).  a_rows / b_cols: readable extent of A's columns
 // / Bt's rows from the given base (clamp bound for edge tiles).
+// kPanel marks the panel products (HPL_pdrpanrlT's GEMMs) so that kernel
+// traces tell them from the trailing update; the code is the same.
+template <bool kPanel>
 __global__ __launch_bounds__(256, 2) void update_kernel(
     int M, int N, int K, const double* __restrict__ A, int lda,
     const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
@@ -214,16 +217,21 @@ bool update_fast_ok(int m, int n, int k, const double* A, int lda,
 
 hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,
                          int lda, const double* Bt, int ldb, double* C, int ldc,
-                         int a_rows, int b_cols) {
+                         int a_rows, int b_cols, bool panel) {
   if(!update_fast_ok(m, n, k, A, lda, Bt, ldb, a_rows, b_cols))
     return hipErrorInvalidValue;
   const int  tiles_m = (m + 127) / 128, tiles_n = (n + 127) / 128;
   const long T       = static_cast<long>(tiles_m) * tiles_n;
   const long grid    = (T + kUXcds - 1) / kUXcds * kUXcds;
   if(grid > 0x7fffffffL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(update_kernel, dim3(static_cast<unsigned>(grid)),
-                     dim3(256), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
-                     tiles_m, tiles_n, a_rows, b_cols);
+  if(panel)
+    hipLaunchKernelGGL(update_kernel<true>, dim3(static_cast<unsigned>(grid)),
+                       dim3(256), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
+                       tiles_m, tiles_n, a_rows, b_cols);
+  else
+    hipLaunchKernelGGL(update_kernel<false>, dim3(static_cast<unsigned>(grid)),
+                       dim3(256), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
+                       tiles_m, tiles_n, a_rows, b_cols);
   return hipGetLastError();
 }
 

@@ -43,14 +43,17 @@ def test_traffic_summary_scales_by_the_copy_probe(tmp_path, monkeypatch):
     _counter_csv(tmp_path / "cw.csv",
                  [(2, "__amd_rocclr_copyBuffer", "WRITE_SIZE", gib / 2048),
                   (2, "__amd_rocclr_copyBuffer", "WRITE_SIZE", gib / 2048)])
-    # two update launches: 3 and 5 GiB fetched by the counter, 2 GiB written
+    # two trailing-update launches: 3 and 5 GiB fetched by the counter, 2 GiB
+    # written; the panel-product instance and other kernels are not counted
     _counter_csv(tmp_path / "f.csv",
-                 [(7, "update_kernel(int)", "FETCH_SIZE", 3 * gib / 1024),
-                  (9, "update_kernel(int)", "FETCH_SIZE", 5 * gib / 1024),
+                 [(7, "update_kernel<false>(int)", "FETCH_SIZE", 3 * gib / 1024),
+                  (9, "update_kernel<false>(int)", "FETCH_SIZE", 5 * gib / 1024),
+                  (6, "update_kernel<true>(int)", "FETCH_SIZE", 1e9),
                   (8, "leaf_kernel", "FETCH_SIZE", 1e9)])
     _counter_csv(tmp_path / "w.csv",
-                 [(7, "update_kernel(int)", "WRITE_SIZE", 2 * gib / 1024),
-                  (9, "update_kernel(int)", "WRITE_SIZE", 2 * gib / 1024)])
+                 [(7, "update_kernel<false>(int)", "WRITE_SIZE", 2 * gib / 1024),
+                  (9, "update_kernel<false>(int)", "WRITE_SIZE", 2 * gib / 1024),
+                  (6, "update_kernel<true>(int)", "WRITE_SIZE", 1e9)])
     bench.traffic_summary(str(tmp_path / "f.csv"), str(tmp_path / "w.csv"),
                           str(tmp_path / "cf.csv"), str(tmp_path / "cw.csv"),
                           0, 896, gib, probe=(1024, 2048, 896))
