@@ -15,6 +15,9 @@ mixed-precision forms, reported in the same JSON line:
     A = U(-0.5,0.5) + (N/4) I.
 Their timed region is the whole hplx_mxp_pdgesv (factor, first solve and
 every refinement step); HPL-equivalent TFLOP/s = (2/3 N^3 + 2 N^2) / t.
+The default run then times the adjoint solve ("adjoint": getrf, getrs and
+the adjoint of X = A^{-1} B at N=16384, A = U(-0.5,0.5) + 2 sqrt(N) I, 64
+right-hand sides; --no-adjoint skips it).
 --mode fp64|bf16|fp8 runs one form alone and makes it the line's value.
 Prints ONE JSON line.
 """
@@ -443,6 +446,109 @@ def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, n, nb,
     print(json.dumps(d), flush=True)
 
 
+def run_adjoint(ctx, rp, args):
+    """The adjoint solve: X = A^{-1} B and, for an upstream gradient G,
+    gradB = A^{-T} G and gradA = -gradB X^T, at N = 16384 with
+    A = U(-0.5,0.5) + 2 sqrt(N) I from the HPL generator and 64 right-hand
+    sides from a seeded normal generator.  getrf, getrs and the adjoint are
+    each the median of `reps` timed calls after one untimed call; the
+    adjoint is timed with and without gradA, whose difference is the
+    HBM-bound outer product that writes the N x N gradA.  Results are
+    checked on the host against the matrix as generated: the residuals of
+    both solves and sampled gradA columns against -gradB X^T."""
+    import numpy as np
+    n, nb, nrhs, reps = args.adj_n, args.adj_nb, 64, 5
+    shift = 2.0 * n ** 0.5
+    palg = rp.palg_default()
+    log("adjoint: N=%d NB=%d nrhs=%d" % (n, nb, nrhs))
+    mat = ctx.pmat_alloc(n, nb)
+    rng = np.random.default_rng(20240501)
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    G = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    dX, dG, dgB = (ctx.alloc(8 * n * nrhs) for _ in range(3))
+    dgA = ctx.alloc(8 * n * n)
+    dG.upload(G)
+
+    def timed(fn):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    def getrf():
+        info = ctx.pdgetrf(palg, mat)
+        if info != 0:
+            raise SystemExit("adjoint: exact zero pivot at column %d" % info)
+
+    try:
+        ctx.pdrandmat(n, n, nb, mat.A, mat.ld, rp.HPL_ISEED, shift)
+        A0 = np.array(ctx.download_matrix(mat, rows=n, cols=n), order="F")
+        t_lu = []
+        for r in range(reps + 1):
+            ctx.pdrandmat(n, n, nb, mat.A, mat.ld, rp.HPL_ISEED, shift)
+            t = timed(getrf)
+            if r:
+                t_lu.append(t)
+        t_rs = []
+        for r in range(reps + 1):
+            dX.upload(B)
+            t = timed(lambda: ctx.pdgetrs(mat, 0, nrhs, dX.ptr, n))
+            if r:
+                t_rs.append(t)
+        t_adj, t_adj_b = [], []
+        for r in range(reps + 1):
+            t = timed(lambda: ctx.pdgesv_adjoint(mat, nrhs, dX.ptr, n, dG.ptr,
+                                                 n, dgB.ptr, n, dgA.ptr, n))
+            if r:
+                t_adj.append(t)
+            t = timed(lambda: ctx.pdgesv_adjoint(mat, nrhs, dX.ptr, n, dG.ptr,
+                                                 n, dgB.ptr, n, None, n))
+            if r:
+                t_adj_b.append(t)
+        X = dX.download((n, nrhs), np.float64)
+        gB = dgB.download((n, nrhs), np.float64)
+        cols = np.linspace(0, n - 1, 16).astype(int)
+        gA = np.stack([dgA.download((n,), np.float64, byte_offset=8 * n * int(j))
+                       for j in cols], axis=1)
+    finally:
+        for d in (dX, dG, dgB, dgA):
+            d.free()
+        ctx.pmat_free(mat)
+
+    an = np.linalg.norm(A0)
+    res_x = float(np.linalg.norm(A0 @ X - B) / (an * np.linalg.norm(X)))
+    res_gb = float(np.linalg.norm(A0.T @ gB - G) / (an * np.linalg.norm(gB)))
+    ref = -(gB @ X[cols].T)
+    err_ga = float(np.abs(gA - ref).max() / np.abs(ref).max())
+    del A0
+    checks = {"solve_residual": res_x, "adjoint_residual": res_gb,
+              "gradA_vs_outer_product": err_ga}
+    if not (res_x < 1e-13 and res_gb < 1e-13 and err_ga < 1e-12):
+        raise SystemExit("adjoint check FAILED: %r" % checks)
+    med = lambda v: sorted(v)[len(v) // 2]
+    lu, rs, adj, adj_b = med(t_lu), med(t_rs), med(t_adj), med(t_adj_b)
+    ga = max(adj - adj_b, 1e-9)
+    out = {"config": {"workload": "hpl_getrf_getrs_adjoint_p1q1", "N": n,
+                      "NB": nb, "nrhs": nrhs, "diag_shift": "2*sqrt(N)",
+                      "reps": reps},
+           "getrf_ms": round(lu * 1e3, 3),
+           "getrf_tflops": round(2.0 / 3.0 * n ** 3 / lu / 1e12, 2),
+           "getrs_ms": round(rs * 1e3, 3),
+           "getrs_tflops": round(2.0 * n * n * nrhs / rs / 1e12, 3),
+           "adjoint_ms": round(adj * 1e3, 3),
+           "adjoint_without_gradA_ms": round(adj_b * 1e3, 3),
+           "gradA_ms": round(ga * 1e3, 3),
+           "gradA_gbs": round(8.0 * n * n / ga / 1e9, 1),
+           "gradA_hbm_frac": round(8.0 * n * n / ga / 1e9 / HBM_PEAK_GBS, 4),
+           "checks": checks}
+    log("adjoint: getrf %.2f ms (%.1f TF/s), getrs %.2f ms, adjoint %.2f ms "
+        "(gradA %.2f ms, %.0f GB/s)" % (lu * 1e3, out["getrf_tflops"],
+                                        rs * 1e3, adj * 1e3, ga * 1e3,
+                                        out["gradA_gbs"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,6 +567,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=24576)
     ap.add_argument("--cpu-nb", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
+                    help="skip the adjoint-solve leg of the default run")
+    ap.add_argument("--adj-n", type=int, default=16384)
+    ap.add_argument("--adj-nb", type=int, default=512)
     ap.add_argument("--prewarm", type=int, default=None,
                     help="N of an untimed warm-up solve (default: min(N,4096) "
                          "for N <= 32768, none above)")
@@ -523,6 +633,9 @@ def main():
                     raise
                 log("%s form failed: %s" % (form, e))
                 mixed[form] = {"error": str(e)}
+    adjoint = None
+    if args.mode == "all" and args.adjoint:
+        adjoint = run_adjoint(ctx, rp, args)
     ctx.close()
 
     if fp64 is not None:
@@ -564,6 +677,8 @@ def main():
         if fp64 is not None and "value" in m:
             m["vs_fp64"] = round(m["value"] / fp64["value"], 3)
         out[form] = m
+    if adjoint is not None:
+        out["adjoint"] = adjoint
     if not args.no_cpu:
         log("CPU baseline sample N=%d" % args.cpu_n)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_nb, _CORES)

@@ -417,9 +417,18 @@ int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
                                      B + j, ldb, 1.0, B + e, ldb, false, 0,
                                      true, false));
     }
+    // z_p -= L21^T z_below reduces over all N - e rows below the panel for
+    // a jb x nrhs result: split over K (partials in the idle L11^{-1} buffer)
+    // while they fit, else the tiled GEMM
+    const size_t work = static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB;
     for(int p = np - 1; p >= 0; --p) {
       const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb, e = j + jb;
-      if(e < N)
+      if(e < N && static_cast<size_t>(jb) * nrhs <= work / 2)
+        HIPCHK(c, hplx::launch_tn_sub(s, N - e, jb, nrhs,
+                                      A + e + static_cast<size_t>(j) * lda,
+                                      lda, B + e, ldb, B + j, ldb, c.Linv,
+                                      work));
+      else if(e < N)
         HIPCHK(c, hplx::launch_dgemm(s, jb, nrhs, N - e, -1.0,
                                      A + e + static_cast<size_t>(j) * lda, lda,
                                      B + e, ldb, 1.0, B + j, ldb, false, 0,

@@ -106,6 +106,72 @@ __global__ __launch_bounds__(1024) void inv_gemv_kernel(
   }
 }
 
+// Split-K partial products of C (m x n) -= A^T B for the transposed solve,
+// where A (K x m) and B (K x n) are column-major with K far larger than m and
+// n: split s sums k in [s * kc, min(K, (s + 1) * kc)) for one 64 x 64 tile of
+// C into P + s * m * n (column-major, leading dimension m).  Each of the 256
+// threads owns a 4 x 4 block of the tile; 32-deep K steps through LDS, each
+// column of A and B read as 256 contiguous bytes.
+__global__ __launch_bounds__(256) void tn_partial_kernel(
+    int K, int m, int n, int kc, const double* __restrict__ A, int lda,
+    const double* __restrict__ B, int ldb, double* __restrict__ P, int tiles_m,
+    int tiles_n) {
+  __shared__ double sa[32][65];
+  __shared__ double sb[32][65];
+  const int tiles = tiles_m * tiles_n, split = blockIdx.x / tiles;
+  const int tile = blockIdx.x - split * tiles;
+  const int m0 = (tile % tiles_m) * 64, n0 = (tile / tiles_m) * 64;
+  const int k0 = split * kc, k1 = min(K, k0 + kc);
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  double acc[4][4] = {};
+  for(int k = k0; k < k1; k += 32) {
+#pragma unroll
+    for(int r = 0; r < 8; ++r) {
+      const int e = t + 256 * r, kk = e & 31, c = e >> 5;
+      const bool kin = k + kk < k1;
+      sa[kk][c] = kin && m0 + c < m
+                      ? A[(k + kk) + static_cast<size_t>(m0 + c) * lda] : 0.0;
+      sb[kk][c] = kin && n0 + c < n
+                      ? B[(k + kk) + static_cast<size_t>(n0 + c) * ldb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for(int kk = 0; kk < 32; ++kk) {
+      double av[4], bv[4];
+#pragma unroll
+      for(int i = 0; i < 4; ++i) av[i] = sa[kk][ty * 4 + i];
+#pragma unroll
+      for(int j = 0; j < 4; ++j) bv[j] = sb[kk][tx * 4 + j];
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+#pragma unroll
+        for(int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  double* ps = P + static_cast<size_t>(split) * m * n;
+#pragma unroll
+  for(int i = 0; i < 4; ++i)
+#pragma unroll
+    for(int j = 0; j < 4; ++j) {
+      const int row = m0 + ty * 4 + i, col = n0 + tx * 4 + j;
+      if(row < m && col < n) ps[row + static_cast<size_t>(col) * m] = acc[i][j];
+    }
+}
+
+// C[i, j] -= sum over the splits of P, in split order (the result does not
+// depend on how the splits were scheduled).
+__global__ __launch_bounds__(256) void tn_reduce_kernel(
+    int m, int n, int splits, const double* __restrict__ P,
+    double* __restrict__ C, int ldc) {
+  const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  if(e >= static_cast<long>(m) * n) return;
+  const int i = static_cast<int>(e % m), j = static_cast<int>(e / m);
+  double sum = 0.0;
+  for(int sp = 0; sp < splits; ++sp) sum += P[static_cast<size_t>(sp) * m * n + e];
+  C[i + static_cast<size_t>(j) * ldc] -= sum;
+}
+
 template <typename MT>
 hipError_t gemv_sub_t(hipStream_t s, int m, int k, const MT* M, int ldm,
                       const double* x, double* y, double* xback) {
@@ -142,6 +208,32 @@ hipError_t launch_inv_gemv(hipStream_t s, int k, const double* Inv,
   if(k > kMaxNB) return hipErrorInvalidValue;
   hipLaunchKernelGGL(inv_gemv_kernel, dim3((k + 63) / 64), dim3(1024), 0, s, k,
                      Inv, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_tn_sub(hipStream_t s, int K, int m, int n, const double* A,
+                         int lda, const double* B, int ldb, double* C, int ldc,
+                         double* work, size_t work_doubles) {
+  if(K <= 0 || m <= 0 || n <= 0) return hipSuccess;
+  const size_t mn = static_cast<size_t>(m) * n;
+  if(mn > work_doubles) return hipErrorInvalidValue;
+  const int tiles_m = (m + 63) / 64, tiles_n = (n + 63) / 64;
+  // about 256 k per split, at most 64 splits and what the work buffer holds
+  long splits = (K + 255) / 256;
+  splits = splits < 64 ? splits : 64;
+  const long cap = static_cast<long>(work_doubles / mn);
+  splits = splits < cap ? splits : cap;
+  const int kc = static_cast<int>(((K + splits - 1) / splits + 31) / 32 * 32);
+  splits = (K + kc - 1) / kc;
+  const long grid = static_cast<long>(tiles_m) * tiles_n * splits;
+  if(grid > 0x7fffffffL || static_cast<long>(mn) > 0x7fffffffL * 256L)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tn_partial_kernel, dim3(static_cast<unsigned>(grid)),
+                     dim3(256), 0, s, K, m, n, kc, A, lda, B, ldb, work,
+                     tiles_m, tiles_n);
+  hipLaunchKernelGGL(tn_reduce_kernel,
+                     dim3(static_cast<unsigned>((mn + 255) / 256)), dim3(256),
+                     0, s, m, n, static_cast<int>(splits), work, C, ldc);
   return hipGetLastError();
 }
 
