@@ -18,7 +18,8 @@ every refinement step); HPL-equivalent TFLOP/s = (2/3 N^3 + 2 N^2) / t.
 The default run then times the adjoint solve ("adjoint": getrf, getrs and
 the adjoint of X = A^{-1} B at N=16384, A = U(-0.5,0.5) + 2 sqrt(N) I, 64
 right-hand sides; --no-adjoint skips it).
---mode fp64|bf16|fp8 runs one form alone and makes it the line's value.
+--mode fp64|bf16|fp8 runs one form alone and makes it the line's value;
+--mode adjoint runs the adjoint leg alone (value: its getrf's rate).
 Prints ONE JSON line.
 """
 import argparse
@@ -554,7 +555,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--mode", choices=["all", "fp64", "bf16", "fp8"],
+    ap.add_argument("--mode", choices=["all", "fp64", "bf16", "fp8", "adjoint"],
                     default="all")
     ap.add_argument("--n", type=int, default=193280)
     ap.add_argument("--nb", type=int, default=896)
@@ -634,9 +635,30 @@ def main():
                 log("%s form failed: %s" % (form, e))
                 mixed[form] = {"error": str(e)}
     adjoint = None
-    if args.mode == "all" and args.adjoint:
+    if args.mode == "adjoint" or (args.mode == "all" and args.adjoint):
         adjoint = run_adjoint(ctx, rp, args)
     ctx.close()
+
+    if args.mode == "adjoint":
+        # the adjoint leg alone: value = HPL-equivalent rate of its getrf
+        a = adjoint
+        out.update({
+            "value": round(hpl_flops(args.adj_n) / (a["getrf_ms"] * 1e-3) / 1e12, 3),
+            "unit": "TFLOP/s", "n_gpus": 1, "steps": a["config"]["reps"],
+            "warmup": 1, "ms_per_step": round(a["getrf_ms"] + a["getrs_ms"] +
+                                              a["adjoint_ms"], 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic: HPL 64-bit LCG matgen, seed 100, plus "
+                    "2 sqrt(N) I; B, G seeded normal",
+            "config": a["config"],
+            "roofline": {"bound": "hbm", "achieved": a["gradA_gbs"],
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": a["gradA_hbm_frac"], "traffic": None,
+                         "kernel": "gradA = -gradB X^T (writes N x N)"},
+            "adjoint": a, "cpu_baseline": None})
+        print(json.dumps(out), flush=True)
+        return
 
     if fp64 is not None:
         head = fp64
