@@ -145,7 +145,9 @@ int hplx_pdgetrf(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat);
  * solves A X = B, trans = 1 solves A^T X = B, for nrhs columns of B (device,
  * ldb >= n) in place.  The L solve applies each panel's interchanges the way
  * HPL_pdgesv carries b through the factorisation (HPL's L is left unpivoted
- * by later panels); the U solve is HPL_pdtrsv's. */
+ * by later panels); the U solve is HPL_pdtrsv's.  No allocation: the
+ * transposed solve keeps split-K partial sums in the context's own panel
+ * buffers, so calls on one context must not overlap. */
 int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
                  double* B, int ldb);
 

@@ -58,9 +58,13 @@ class Solver:
         self.ctx.pmat_free(self.mat)
 
 
+# the last case has jb * nrhs above the transposed solve's split-K scratch
+# (2^19 doubles) for its two 1024-wide panels, so those take the tiled GEMM,
+# and the 52-wide last panel the split-K product
 @pytest.mark.parametrize("n,nb,shift,nrhs", [(1000, 64, 0.0, 5),
                                              (1500, 256, 2 * 1500 ** 0.5, 7),
-                                             (777, 96, 0.0, 1)])
+                                             (777, 96, 0.0, 1),
+                                             (2100, 1024, 0.0, 513)])
 def test_getrs_matches_lapack(ctx, n, nb, shift, nrhs):
     say("getrs n=%d nb=%d" % (n, nb))
     rng = np.random.default_rng(n)
