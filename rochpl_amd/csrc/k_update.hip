@@ -5,13 +5,15 @@
 //
 // Both operands are then contiguous along the tile edge, which lets every
 // stage move from HBM straight into LDS without passing through registers:
-//   - 128 x 128 workgroup tile, 4 waves each 64 x 64 = 4 x 4 blocks of
-//     V_MFMA_F64_16X16X4_F64, the transposed block computed so that 16 lanes
-//     store 16 consecutive rows of a C column;
-//   - K in 8-deep stages through a ring of 4 LDS slots (66.5 KB, two
-//     workgroups per CU); each stage is 16 GLOBAL_LOAD_LDS_DWORDX4 (one
-//     128-double LDS row each, rows padded to 130 doubles), issued two
-//     stages before the barrier that publishes it;
+//   - 128 x 128 workgroup tile, 8 waves each 64 x 32 = 4 x 2 blocks of
+//     V_MFMA_F64_16X16X4_F64 (64 accumulator VGPRs, so two workgroups per
+//     CU give four waves per SIMD to cover each other's waits), the
+//     transposed block computed so that 16 lanes store 16 consecutive rows
+//     of a C column;
+//   - K in 8-deep stages through a ring of 4 LDS slots (66.5 KB per
+//     workgroup); each stage is 16 GLOBAL_LOAD_LDS_DWORDX4 (one 128-double
+//     LDS row each, rows padded to 130 doubles; wave w loads K row w of both
+//     operands), issued two stages before the barrier that publishes it;
 //   - per stage one wait for this wave's loads of the next stage and one
 //     workgroup barrier, placed between the stage's two 4-deep halves:
 //     the second half's fragments are read before it and the next stage's
@@ -23,7 +25,7 @@
 //     matrix, which are not stored, so no masking is needed; K must be a
 //     multiple of 8 (HPL's NB);
 //   - workgroup -> tile mapping XCD-aware (8 XCDs, groups of 8 tile rows);
-//   - interior tiles update C with up to 32 loads per lane in flight.
+//   - interior tiles update C with up to 16 loads per lane in flight.
 //
 // The stage loads come from lds_stream.hpp; the kernel waits explicitly, no
 // compiler-issued vector memory access is in flight inside the K loop, and
@@ -51,7 +53,7 @@ typedef double ud4 __attribute__((ext_vector_type(4)));
 // kPanel marks the panel products (HPL_pdrpanrlT's GEMMs) so that kernel
 // traces tell them from the trailing update; the code is the same.
 template <bool kPanel>
-__global__ __launch_bounds__(256, 2) void update_kernel(
+__global__ __launch_bounds__(512, 2) void update_kernel(
     int M, int N, int K, const double* __restrict__ A, int lda,
     const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
     int tiles_m, int tiles_n, int a_rows, int b_cols) {
@@ -65,14 +67,14 @@ __global__ __launch_bounds__(256, 2) void update_kernel(
   const int within = tile - group * width;
   const int m0 = (firstm + within % gsz) * 128, n0 = (within / gsz) * 128;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
+  const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 32;
   const int l16 = lane & 15, lk = lane >> 4;
 
-  // wave w loads k rows 2w and 2w+1 of every stage, lane l the pair 2l
+  // wave w loads k row w of every stage, lane l the pair 2l
   const int oa = min(m0 + 2 * lane, a_rows - 2);
   const int ob = min(n0 + 2 * lane, b_cols - 2);
-  const double* pa0 = A + oa + static_cast<size_t>(2 * wave) * lda;
-  const double* pb0 = Bt + ob + static_cast<size_t>(2 * wave) * ldb;
+  const double* pa0 = A + oa + static_cast<size_t>(wave) * lda;
+  const double* pb0 = Bt + ob + static_cast<size_t>(wave) * ldb;
   const int nst = K / UBK;
   // past the last stage the last one is loaded again, so every iteration
   // waits for the same number of outstanding loads
@@ -80,29 +82,27 @@ __global__ __launch_bounds__(256, 2) void update_kernel(
     const int s = st < nst ? st : nst - 1;
     const double* pa = pa0 + static_cast<size_t>(s) * UBK * lda;
     const double* pb = pb0 + static_cast<size_t>(s) * UBK * ldb;
-    lds_load16(pa, &a[2 * wave][0]);
-    lds_load16(pa + lda, &a[2 * wave + 1][0]);
-    lds_load16(pb, &b[2 * wave][0]);
-    lds_load16(pb + ldb, &b[2 * wave + 1][0]);
+    lds_load16(pa, &a[wave][0]);
+    lds_load16(pb, &b[wave][0]);
   };
-  ud4 acc[4][4];
+  ud4 acc[4][2];
 #pragma unroll
   for(int i = 0; i < 4; ++i)
 #pragma unroll
-    for(int j = 0; j < 4; ++j) acc[i][j] = ud4{0.0, 0.0, 0.0, 0.0};
+    for(int j = 0; j < 2; ++j) acc[i][j] = ud4{0.0, 0.0, 0.0, 0.0};
   // fragments of one 4-deep half stage, and the 16 MFMAs that consume them
   auto frag = [&](double (*a)[UROW], double (*b)[UROW], int ks, double* fa,
                   double* fb) {
 #pragma unroll
     for(int i = 0; i < 4; ++i) fa[i] = a[ks * 4 + lk][wm0 + i * 16 + l16];
 #pragma unroll
-    for(int j = 0; j < 4; ++j) fb[j] = b[ks * 4 + lk][wn0 + j * 16 + l16];
+    for(int j = 0; j < 2; ++j) fb[j] = b[ks * 4 + lk][wn0 + j * 16 + l16];
   };
   auto mma = [&](const double* fa, const double* fb) {
 #pragma unroll
     for(int i = 0; i < 4; ++i)
 #pragma unroll
-      for(int j = 0; j < 4; ++j)
+      for(int j = 0; j < 2; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i],
                                                          acc[i][j], 0, 0, 0);
   };
@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256, 2) void update_kernel(
   issue(0, sA0, sB0);
   issue(1, sA1, sB1);
   issue(2, sA2, sB2);
-  double f0a[4], f0b[4], f1a[4], f1b[4];
-  wait_vm<4 * (UNS - 2)>();
+  double f0a[4], f0b[2], f1a[4], f1b[2];
+  wait_vm<2 * (UNS - 2)>();
   __builtin_amdgcn_s_barrier();
   frag(sA0, sB0, 0, f0a, f0b);
   // stage s lives in slot s % 4; unrolled by 4 so every slot is static.
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256, 2) void update_kernel(
   if(s + q < nst) {                                                          \
     frag(ca, cb, 1, f1a, f1b);                                               \
     mma(f0a, f0b);                                                           \
-    wait_vm<4 * (UNS - 3)>(); /* this wave's loads of stage s+q+1 are in */  \
+    wait_vm<2 * (UNS - 3)>(); /* this wave's loads of stage s+q+1 are in */  \
     __builtin_amdgcn_s_barrier(); /* every wave's; slot of s+q-1 free */     \
     issue(s + q + UNS - 1, ra, rb);                                          \
     if(s + q + 1 < nst) frag(na, nb, 0, f0a, f0b);                           \
@@ -140,24 +140,24 @@ __global__ __launch_bounds__(256, 2) void update_kernel(
 
   // D[4r + l/16][l%16] of block (i, j) is C[row = l16][col = 4r + lk]
   if(m0 + 128 <= M && n0 + 128 <= N) {
-    // interior tile: the 16 values of block row i + 1 are loaded before
-    // those of block row i are stored, so up to 32 loads per lane are in
+    // interior tile: the 8 values of block row i + 1 are loaded before
+    // those of block row i are stored, so up to 16 loads per lane are in
     // flight instead of one load-store round trip per element
     double* cb = C + (m0 + wm0 + l16) + static_cast<size_t>(n0 + wn0 + lk) * ldc;
-    double cv[2][16];
+    double cv[2][8];
 #pragma unroll
-    for(int e = 0; e < 16; ++e)
+    for(int e = 0; e < 8; ++e)
       cv[0][e] = cb[static_cast<size_t>((e >> 2) * 16 + 4 * (e & 3)) * ldc];
 #pragma unroll
     for(int i = 0; i < 4; ++i) {
       if(i + 1 < 4) {
 #pragma unroll
-        for(int e = 0; e < 16; ++e)
+        for(int e = 0; e < 8; ++e)
           cv[(i + 1) & 1][e] =
               cb[(i + 1) * 16 + static_cast<size_t>((e >> 2) * 16 + 4 * (e & 3)) * ldc];
       }
 #pragma unroll
-      for(int e = 0; e < 16; ++e)
+      for(int e = 0; e < 8; ++e)
         cb[i * 16 + static_cast<size_t>((e >> 2) * 16 + 4 * (e & 3)) * ldc] =
             cv[i & 1][e] - acc[i][e >> 2][e & 3];
     }
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256, 2) void update_kernel(
     const int row = m0 + wm0 + i * 16 + l16;
     if(row >= M) continue;
 #pragma unroll
-    for(int j = 0; j < 4; ++j)
+    for(int j = 0; j < 2; ++j)
 #pragma unroll
       for(int r = 0; r < 4; ++r) {
         const int col = n0 + wn0 + j * 16 + 4 * r + lk;
@@ -226,11 +226,11 @@ hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,
   if(grid > 0x7fffffffL) return hipErrorInvalidValue;
   if(panel)
     hipLaunchKernelGGL(update_kernel<true>, dim3(static_cast<unsigned>(grid)),
-                       dim3(256), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
+                       dim3(512), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
                        tiles_m, tiles_n, a_rows, b_cols);
   else
     hipLaunchKernelGGL(update_kernel<false>, dim3(static_cast<unsigned>(grid)),
-                       dim3(256), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
+                       dim3(512), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
                        tiles_m, tiles_n, a_rows, b_cols);
   return hipGetLastError();
 }
