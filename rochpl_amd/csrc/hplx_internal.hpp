@@ -132,8 +132,9 @@ hipError_t launch_sgemm(hipStream_t s, int m, int n, int k, double alpha,
 // (k_update.hip): K a multiple of 8, 16-byte aligned operands, even leading
 // dimensions; a_rows / b_cols bound the (clamped) reads of A's columns and
 // Bt's rows.  update_fast_ok says whether a call qualifies.  panel = true
-// launches the instance kept for the panel products (a separate kernel name
-// in traces, same code).
+// launches the instance kept for the other products (panel recursion,
+// solves with many right-hand sides): a separate kernel name in traces, so
+// that the trailing update's launches stand alone; same code.
 bool       update_fast_ok(int m, int n, int k, const double* A, int lda,
                           const double* Bt, int ldb, int a_rows, int b_cols);
 hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,

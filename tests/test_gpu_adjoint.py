@@ -58,13 +58,16 @@ class Solver:
         self.ctx.pmat_free(self.mat)
 
 
-# the last case has jb * nrhs above the transposed solve's split-K scratch
-# (2^19 doubles) for its two 1024-wide panels, so those take the tiled GEMM,
-# and the 52-wide last panel the split-K product
+# (2100, 1024, 513): jb * nrhs is above the transposed solve's split-K
+# scratch (2^19 doubles) for the two 1024-wide panels, so those take the
+# tiled GEMM, and the 52-wide last panel the split-K product.  (1300, 256,
+# 40): the forward solve's products take the streamed MFMA kernel, except the
+# 20-wide last panel's (K not a multiple of 8), which takes the tiled GEMM.
 @pytest.mark.parametrize("n,nb,shift,nrhs", [(1000, 64, 0.0, 5),
                                              (1500, 256, 2 * 1500 ** 0.5, 7),
                                              (777, 96, 0.0, 1),
-                                             (2100, 1024, 0.0, 513)])
+                                             (2100, 1024, 0.0, 513),
+                                             (1300, 256, 0.0, 40)])
 def test_getrs_matches_lapack(ctx, n, nb, shift, nrhs):
     say("getrs n=%d nb=%d" % (n, nb))
     rng = np.random.default_rng(n)
