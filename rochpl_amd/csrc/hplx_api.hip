@@ -381,11 +381,10 @@ int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
   // the shapes qualify (as the panel products do), else the tiled GEMM.
   // a_rows: rows of A readable from Ablk (its clamped edge reads).
   const int ldt = hplx::ut_ld(nrhs);
+  const size_t work = static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB;
   auto sub_product = [&](int m, int jb, const double* Ablk, int a_rows,
                          const double* Y, double* Cblk) -> hipError_t {
-    if(m >= 128 && nrhs >= 16 &&
-       static_cast<size_t>(jb) * ldt <=
-           static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB &&
+    if(m >= 128 && nrhs >= 16 && static_cast<size_t>(jb) * ldt <= work &&
        hplx::update_fast_ok(m, nrhs, jb, Ablk, lda, c.Linv, ldt, a_rows, ldt)) {
       hipError_t e = hplx::launch_transpose(s, jb, nrhs, Y, ldb, c.Linv, ldt);
       if(e != hipSuccess) return e;
@@ -428,28 +427,24 @@ int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
       HIPCHK(c, hplx::launch_tri_solve(s, true, true, false, jb, nrhs,
                                        A + j + static_cast<size_t>(j) * lda,
                                        lda, B + j, ldb));
+      // z_below -= U12^T z_p (U12 = rows j..e of the columns right of the
+      // panel)
       if(e < N)
-        HIPCHK(c, hplx::launch_dgemm(s, N - e, nrhs, jb, -1.0,
-                                     A + j + static_cast<size_t>(e) * lda, lda,
-                                     B + j, ldb, 1.0, B + e, ldb, false, 0,
-                                     true, false));
+        HIPCHK(c, hplx::launch_tn_sub(s, jb, N - e, nrhs,
+                                      A + j + static_cast<size_t>(e) * lda,
+                                      lda, B + j, ldb, B + e, ldb, c.Linv,
+                                      work));
     }
     // z_p -= L21^T z_below reduces over all N - e rows below the panel for
-    // a jb x nrhs result: split over K (partials in the idle L11^{-1} buffer)
-    // while they fit, else the tiled GEMM
-    const size_t work = static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB;
+    // a jb x nrhs result: split over K, the partials in the idle L11^{-1}
+    // buffer, while they fit
     for(int p = np - 1; p >= 0; --p) {
       const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb, e = j + jb;
-      if(e < N && static_cast<size_t>(jb) * nrhs <= work / 2)
+      if(e < N)
         HIPCHK(c, hplx::launch_tn_sub(s, N - e, jb, nrhs,
                                       A + e + static_cast<size_t>(j) * lda,
                                       lda, B + e, ldb, B + j, ldb, c.Linv,
                                       work));
-      else if(e < N)
-        HIPCHK(c, hplx::launch_dgemm(s, jb, nrhs, N - e, -1.0,
-                                     A + e + static_cast<size_t>(j) * lda, lda,
-                                     B + e, ldb, 1.0, B + j, ldb, false, 0,
-                                     true, false));
       HIPCHK(c, hplx::launch_tri_solve(s, false, true, true, jb, nrhs,
                                        A + j + static_cast<size_t>(j) * lda,
                                        lda, B + j, ldb));

@@ -147,9 +147,11 @@ hipError_t launch_transpose(hipStream_t s, int rows, int cols, const double* A,
 // the clamped pair reads).
 inline int ut_ld(int n) { return (n + 3) & ~1; }
 
-// C (m x n, ldc) -= A^T B for A (K x m, lda) and B (K x n, ldb) with K much
-// larger than m and n (the transposed solve's L21^T z step): split-K partial
-// products in work (at least m * n doubles), summed in a fixed order.
+// C (m x n, ldc) -= A^T B for A (K x m, lda) and B (K x n, ldb) (the
+// transposed solve's U12^T z and L21^T z steps).  When the 64 x 64 tiles of
+// C are too few to fill the GPU, K is split and the partial products go to
+// work (work_doubles of them; none needed otherwise), summed in a fixed
+// order.
 hipError_t launch_tn_sub(hipStream_t s, int K, int m, int n, const double* A,
                          int lda, const double* B, int ldb, double* C, int ldc,
                          double* work, size_t work_doubles);

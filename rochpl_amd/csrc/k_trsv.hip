@@ -13,6 +13,8 @@
 namespace hplx {
 namespace {
 
+typedef double ud4 __attribute__((ext_vector_type(4)));
+
 template <typename MT>
 __device__ inline double to_d(MT x) {
   if constexpr(sizeof(MT) == 2)
@@ -106,57 +108,89 @@ __global__ __launch_bounds__(1024) void inv_gemv_kernel(
   }
 }
 
-// Split-K partial products of C (m x n) -= A^T B for the transposed solve,
-// where A (K x m) and B (K x n) are column-major with K far larger than m and
-// n: split s sums k in [s * kc, min(K, (s + 1) * kc)) for one 64 x 64 tile of
-// C into P + s * m * n (column-major, leading dimension m).  Each of the 256
-// threads owns a 4 x 4 block of the tile; 32-deep K steps through LDS, each
-// column of A and B read as 256 contiguous bytes.
-__global__ __launch_bounds__(256) void tn_partial_kernel(
+// C (m x n) -= A^T B for the transposed solves, A (K x m) and B (K x n)
+// column-major: one 64 x 64 tile of the result per workgroup over
+// k in [s * kc, min(K, (s + 1) * kc)).  DIRECT (one split): C is updated in
+// place; otherwise split s writes its partial tile to P + s * m * n
+// (column-major, leading dimension m) for tn_reduce_kernel.  The tile is
+// 4 waves of 32 x 32 (2 x 2 blocks of V_MFMA_F64_16X16X4_F64); 32-deep K
+// steps go global -> registers -> LDS (k-major, as the trailing update's
+// stages), the next step's loads in flight during the current step's MFMAs.
+template <bool DIRECT>
+__global__ __launch_bounds__(256) void tn_kernel(
     int K, int m, int n, int kc, const double* __restrict__ A, int lda,
-    const double* __restrict__ B, int ldb, double* __restrict__ P, int tiles_m,
-    int tiles_n) {
+    const double* __restrict__ B, int ldb, double* __restrict__ out, int ldo,
+    int tiles_m, int tiles_n) {
   __shared__ double sa[32][65];
   __shared__ double sb[32][65];
   const int tiles = tiles_m * tiles_n, split = blockIdx.x / tiles;
   const int tile = blockIdx.x - split * tiles;
   const int m0 = (tile % tiles_m) * 64, n0 = (tile / tiles_m) * 64;
   const int k0 = split * kc, k1 = min(K, k0 + kc);
-  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
-  double acc[4][4] = {};
-  for(int k = k0; k < k1; k += 32) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
+  const int l16 = lane & 15, lk = lane >> 4;
+  double ra[8], rb[8];
+  auto load = [&](int k) {
 #pragma unroll
     for(int r = 0; r < 8; ++r) {
       const int e = t + 256 * r, kk = e & 31, c = e >> 5;
       const bool kin = k + kk < k1;
-      sa[kk][c] = kin && m0 + c < m
-                      ? A[(k + kk) + static_cast<size_t>(m0 + c) * lda] : 0.0;
-      sb[kk][c] = kin && n0 + c < n
-                      ? B[(k + kk) + static_cast<size_t>(n0 + c) * ldb] : 0.0;
+      ra[r] = kin && m0 + c < m
+                  ? A[(k + kk) + static_cast<size_t>(m0 + c) * lda] : 0.0;
+      rb[r] = kin && n0 + c < n
+                  ? B[(k + kk) + static_cast<size_t>(n0 + c) * ldb] : 0.0;
+    }
+  };
+  ud4 acc[2][2];
+#pragma unroll
+  for(int i = 0; i < 2; ++i)
+#pragma unroll
+    for(int j = 0; j < 2; ++j) acc[i][j] = ud4{0.0, 0.0, 0.0, 0.0};
+  if(k0 < k1) load(k0);
+  for(int k = k0; k < k1; k += 32) {
+#pragma unroll
+    for(int r = 0; r < 8; ++r) {
+      const int e = t + 256 * r, kk = e & 31, c = e >> 5;
+      sa[kk][c] = ra[r];
+      sb[kk][c] = rb[r];
     }
     __syncthreads();
-#pragma unroll 8
-    for(int kk = 0; kk < 32; ++kk) {
-      double av[4], bv[4];
+    if(k + 32 < k1) load(k + 32);
 #pragma unroll
-      for(int i = 0; i < 4; ++i) av[i] = sa[kk][ty * 4 + i];
+    for(int ks = 0; ks < 8; ++ks) {
+      double fa[2], fb[2];
 #pragma unroll
-      for(int j = 0; j < 4; ++j) bv[j] = sb[kk][tx * 4 + j];
+      for(int i = 0; i < 2; ++i) fa[i] = sa[ks * 4 + lk][wm0 + i * 16 + l16];
 #pragma unroll
-      for(int i = 0; i < 4; ++i)
+      for(int j = 0; j < 2; ++j) fb[j] = sb[ks * 4 + lk][wn0 + j * 16 + l16];
 #pragma unroll
-        for(int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+      for(int i = 0; i < 2; ++i)
+#pragma unroll
+        for(int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i],
+                                                           acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
-  double* ps = P + static_cast<size_t>(split) * m * n;
+  // acc[i][j][r]: row wm0 + 16 i + l16, column wn0 + 16 j + 4 r + lk
+  double* po = DIRECT ? out : out + static_cast<size_t>(split) * m * n;
 #pragma unroll
-  for(int i = 0; i < 4; ++i)
+  for(int i = 0; i < 2; ++i) {
+    const int row = m0 + wm0 + i * 16 + l16;
+    if(row >= m) continue;
 #pragma unroll
-    for(int j = 0; j < 4; ++j) {
-      const int row = m0 + ty * 4 + i, col = n0 + tx * 4 + j;
-      if(row < m && col < n) ps[row + static_cast<size_t>(col) * m] = acc[i][j];
-    }
+    for(int j = 0; j < 2; ++j)
+#pragma unroll
+      for(int r = 0; r < 4; ++r) {
+        const int col = n0 + wn0 + j * 16 + 4 * r + lk;
+        if(col >= n) continue;
+        if(DIRECT)
+          po[row + static_cast<size_t>(col) * ldo] -= acc[i][j][r];
+        else
+          po[row + static_cast<size_t>(col) * m] = acc[i][j][r];
+      }
+  }
 }
 
 // C[i, j] -= sum over the splits of P, in split order (the result does not
@@ -216,21 +250,25 @@ hipError_t launch_tn_sub(hipStream_t s, int K, int m, int n, const double* A,
                          double* work, size_t work_doubles) {
   if(K <= 0 || m <= 0 || n <= 0) return hipSuccess;
   const size_t mn = static_cast<size_t>(m) * n;
-  if(mn > work_doubles) return hipErrorInvalidValue;
   const int tiles_m = (m + 63) / 64, tiles_n = (n + 63) / 64;
-  // about 256 k per split, at most 64 splits and what the work buffer holds
-  long splits = (K + 255) / 256;
+  const long tiles = static_cast<long>(tiles_m) * tiles_n;
+  // split K when the tiles alone leave the GPU short of work: about 256 k
+  // per split, at most 64 splits and what the work buffer holds
+  long splits = tiles >= 256 ? 1 : (K + 255) / 256;
   splits = splits < 64 ? splits : 64;
-  const long cap = static_cast<long>(work_doubles / mn);
-  splits = splits < cap ? splits : cap;
+  const long cap = work ? static_cast<long>(work_doubles / mn) : 0;
+  splits = splits < cap ? splits : (cap < 1 ? 1 : cap);
   const int kc = static_cast<int>(((K + splits - 1) / splits + 31) / 32 * 32);
   splits = (K + kc - 1) / kc;
-  const long grid = static_cast<long>(tiles_m) * tiles_n * splits;
-  if(grid > 0x7fffffffL || static_cast<long>(mn) > 0x7fffffffL * 256L)
-    return hipErrorInvalidValue;
-  hipLaunchKernelGGL(tn_partial_kernel, dim3(static_cast<unsigned>(grid)),
-                     dim3(256), 0, s, K, m, n, kc, A, lda, B, ldb, work,
-                     tiles_m, tiles_n);
+  if(tiles * splits > 0x7fffffffL) return hipErrorInvalidValue;
+  const dim3 grid(static_cast<unsigned>(tiles * splits));
+  if(splits == 1) {
+    hipLaunchKernelGGL(tn_kernel<true>, grid, dim3(256), 0, s, K, m, n, kc, A,
+                       lda, B, ldb, C, ldc, tiles_m, tiles_n);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(tn_kernel<false>, grid, dim3(256), 0, s, K, m, n, kc, A,
+                     lda, B, ldb, work, m, tiles_m, tiles_n);
   hipLaunchKernelGGL(tn_reduce_kernel,
                      dim3(static_cast<unsigned>((mn + 255) / 256)), dim3(256),
                      0, s, m, n, static_cast<int>(splits), work, C, ldc);

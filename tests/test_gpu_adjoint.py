@@ -58,9 +58,10 @@ class Solver:
         self.ctx.pmat_free(self.mat)
 
 
-# (2100, 1024, 513): jb * nrhs is above the transposed solve's split-K
-# scratch (2^19 doubles) for the two 1024-wide panels, so those take the
-# tiled GEMM, and the 52-wide last panel the split-K product.  (1300, 256,
+# (2100, 1024, 513): the transposed solve's scratch (2^20 doubles) holds
+# one 1024 x 513 partial product only, so the L21^T z steps of the two
+# 1024-wide panels run unsplit and the 52-wide last panel's split over K;
+# the other cases split over K throughout.  (1300, 256,
 # 40): the forward solve's products take the streamed MFMA kernel, except the
 # 20-wide last panel's (K not a multiple of 8), which takes the tiled GEMM.
 @pytest.mark.parametrize("n,nb,shift,nrhs", [(1000, 64, 0.0, 5),
