@@ -376,23 +376,11 @@ int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
   const double* A = mat->A;
   hipStream_t  s = c.stream;
   const int    np = (N + nb - 1) / nb;
-  // C (m x nrhs, ldb) -= Ablk (m x jb, lda) * Y (jb x nrhs, ldb): the
-  // streamed MFMA kernel with Y transposed into the idle L11^{-1} buffer when
-  // the shapes qualify (as the panel products do), else the tiled GEMM.
-  // a_rows: rows of A readable from Ablk (its clamped edge reads).
-  const int ldt = hplx::ut_ld(nrhs);
   const size_t work = static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB;
-  auto sub_product = [&](int m, int jb, const double* Ablk, int a_rows,
-                         const double* Y, double* Cblk) -> hipError_t {
-    if(m >= 128 && nrhs >= 16 && static_cast<size_t>(jb) * ldt <= work &&
-       hplx::update_fast_ok(m, nrhs, jb, Ablk, lda, c.Linv, ldt, a_rows, ldt)) {
-      hipError_t e = hplx::launch_transpose(s, jb, nrhs, Y, ldb, c.Linv, ldt);
-      if(e != hipSuccess) return e;
-      return hplx::launch_update(s, m, nrhs, jb, Ablk, lda, c.Linv, ldt, Cblk,
-                                 ldb, a_rows, ldt, true);
-    }
-    return hplx::launch_dgemm(s, m, nrhs, jb, -1.0, Ablk, lda, Y, ldb, 1.0,
-                              Cblk, ldb, false);
+  // C (m x nrhs, ldb) -= Ablk (m x jb, lda) * Y (jb x nrhs, ldb) on FP64 MFMA
+  auto sub_product = [&](int m, int jb, const double* Ablk, const double* Y,
+                         double* Cblk) -> hipError_t {
+    return hplx::launch_nn_sub(s, jb, m, nrhs, Ablk, lda, Y, ldb, Cblk, ldb);
   };
   if(!trans) {
     // L solve with the panel-by-panel interchanges, as b is carried through
@@ -406,8 +394,8 @@ int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
                                        lda, B + j, ldb));
       if(N - j - jb > 0)
         HIPCHK(c, sub_product(N - j - jb, jb,
-                              A + j + jb + static_cast<size_t>(j) * lda,
-                              lda - j - jb, B + j, B + j + jb));
+                              A + j + jb + static_cast<size_t>(j) * lda, B + j,
+                              B + j + jb));
     }
     for(int p = np - 1; p >= 0; --p) {
       const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb;
@@ -415,8 +403,8 @@ int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
                                        A + j + static_cast<size_t>(j) * lda,
                                        lda, B + j, ldb));
       if(j > 0)
-        HIPCHK(c, sub_product(j, jb, A + static_cast<size_t>(j) * lda, lda,
-                              B + j, B));
+        HIPCHK(c, sub_product(j, jb, A + static_cast<size_t>(j) * lda, B + j,
+                              B));
     }
   } else {
     // A = P0^T L0 P1^T L1 ... U  =>  A^T z = g  is solved as

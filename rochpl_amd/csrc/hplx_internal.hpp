@@ -132,9 +132,8 @@ hipError_t launch_sgemm(hipStream_t s, int m, int n, int k, double alpha,
 // (k_update.hip): K a multiple of 8, 16-byte aligned operands, even leading
 // dimensions; a_rows / b_cols bound the (clamped) reads of A's columns and
 // Bt's rows.  update_fast_ok says whether a call qualifies.  panel = true
-// launches the instance kept for the other products (panel recursion,
-// solves with many right-hand sides): a separate kernel name in traces, so
-// that the trailing update's launches stand alone; same code.
+// launches the instance kept for the panel products: a separate kernel name
+// in traces, so that the trailing update's launches stand alone; same code.
 bool       update_fast_ok(int m, int n, int k, const double* A, int lda,
                           const double* Bt, int ldb, int a_rows, int b_cols);
 hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,
@@ -155,6 +154,12 @@ inline int ut_ld(int n) { return (n + 3) & ~1; }
 hipError_t launch_tn_sub(hipStream_t s, int K, int m, int n, const double* A,
                          int lda, const double* B, int ldb, double* C, int ldc,
                          double* work, size_t work_doubles);
+// C (m x n, ldc) -= A B for A (m x K, lda) and B (K x n, ldb): the solves'
+// products with many right-hand sides (same kernel as launch_tn_sub, A read
+// m-contiguous, never split).
+hipError_t launch_nn_sub(hipStream_t s, int K, int m, int n, const double* A,
+                         int lda, const double* B, int ldb, double* C,
+                         int ldc);
 hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                        double* x);
 

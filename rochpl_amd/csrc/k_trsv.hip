@@ -108,15 +108,16 @@ __global__ __launch_bounds__(1024) void inv_gemv_kernel(
   }
 }
 
-// C (m x n) -= A^T B for the transposed solves, A (K x m) and B (K x n)
-// column-major: one 64 x 64 tile of the result per workgroup over
+// C (m x n) -= op(A) B for the solves with many right-hand sides: op(A) =
+// A^T with A (K x m) when TA, else A (m x K); B (K x n); all column-major.
+// One 64 x 64 tile of the result per workgroup over
 // k in [s * kc, min(K, (s + 1) * kc)).  DIRECT (one split): C is updated in
 // place; otherwise split s writes its partial tile to P + s * m * n
 // (column-major, leading dimension m) for tn_reduce_kernel.  The tile is
 // 4 waves of 32 x 32 (2 x 2 blocks of V_MFMA_F64_16X16X4_F64); 32-deep K
 // steps go global -> registers -> LDS (k-major, as the trailing update's
 // stages), the next step's loads in flight during the current step's MFMAs.
-template <bool DIRECT>
+template <bool DIRECT, bool TA>
 __global__ __launch_bounds__(256) void tn_kernel(
     int K, int m, int n, int kc, const double* __restrict__ A, int lda,
     const double* __restrict__ B, int ldb, double* __restrict__ out, int ldo,
@@ -131,15 +132,21 @@ __global__ __launch_bounds__(256) void tn_kernel(
   const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
   const int l16 = lane & 15, lk = lane >> 4;
   double ra[8], rb[8];
+  // element r of a thread's share: k row kk, column c of the 32 x 64 step
+  // (k-contiguous for A^T and B, m-contiguous for A)
+  auto akk = [&](int e) { return TA ? (e & 31) : (e >> 6); };
+  auto ac  = [&](int e) { return TA ? (e >> 5) : (e & 63); };
   auto load = [&](int k) {
 #pragma unroll
     for(int r = 0; r < 8; ++r) {
-      const int e = t + 256 * r, kk = e & 31, c = e >> 5;
-      const bool kin = k + kk < k1;
-      ra[r] = kin && m0 + c < m
-                  ? A[(k + kk) + static_cast<size_t>(m0 + c) * lda] : 0.0;
-      rb[r] = kin && n0 + c < n
-                  ? B[(k + kk) + static_cast<size_t>(n0 + c) * ldb] : 0.0;
+      const int e = t + 256 * r;
+      const int ka = akk(e), ca = ac(e), kb = e & 31, cb = e >> 5;
+      ra[r] = k + ka < k1 && m0 + ca < m
+                  ? (TA ? A[(k + ka) + static_cast<size_t>(m0 + ca) * lda]
+                        : A[(m0 + ca) + static_cast<size_t>(k + ka) * lda])
+                  : 0.0;
+      rb[r] = k + kb < k1 && n0 + cb < n
+                  ? B[(k + kb) + static_cast<size_t>(n0 + cb) * ldb] : 0.0;
     }
   };
   ud4 acc[2][2];
@@ -151,9 +158,9 @@ __global__ __launch_bounds__(256) void tn_kernel(
   for(int k = k0; k < k1; k += 32) {
 #pragma unroll
     for(int r = 0; r < 8; ++r) {
-      const int e = t + 256 * r, kk = e & 31, c = e >> 5;
-      sa[kk][c] = ra[r];
-      sb[kk][c] = rb[r];
+      const int e = t + 256 * r;
+      sa[akk(e)][ac(e)] = ra[r];
+      sb[e & 31][e >> 5] = rb[r];
     }
     __syncthreads();
     if(k + 32 < k1) load(k + 32);
@@ -263,15 +270,28 @@ hipError_t launch_tn_sub(hipStream_t s, int K, int m, int n, const double* A,
   if(tiles * splits > 0x7fffffffL) return hipErrorInvalidValue;
   const dim3 grid(static_cast<unsigned>(tiles * splits));
   if(splits == 1) {
-    hipLaunchKernelGGL(tn_kernel<true>, grid, dim3(256), 0, s, K, m, n, kc, A,
+    hipLaunchKernelGGL((tn_kernel<true, true>), grid, dim3(256), 0, s, K, m, n, kc, A,
                        lda, B, ldb, C, ldc, tiles_m, tiles_n);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(tn_kernel<false>, grid, dim3(256), 0, s, K, m, n, kc, A,
+  hipLaunchKernelGGL((tn_kernel<false, true>), grid, dim3(256), 0, s, K, m, n, kc, A,
                      lda, B, ldb, work, m, tiles_m, tiles_n);
   hipLaunchKernelGGL(tn_reduce_kernel,
                      dim3(static_cast<unsigned>((mn + 255) / 256)), dim3(256),
                      0, s, m, n, static_cast<int>(splits), work, C, ldc);
+  return hipGetLastError();
+}
+
+hipError_t launch_nn_sub(hipStream_t s, int K, int m, int n, const double* A,
+                         int lda, const double* B, int ldb, double* C,
+                         int ldc) {
+  if(K <= 0 || m <= 0 || n <= 0) return hipSuccess;
+  const int  tiles_m = (m + 63) / 64, tiles_n = (n + 63) / 64;
+  const long tiles   = static_cast<long>(tiles_m) * tiles_n;
+  if(tiles > 0x7fffffffL || K > 0x7fffffff - 32) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((tn_kernel<true, false>), dim3(static_cast<unsigned>(tiles)),
+                     dim3(256), 0, s, K, m, n, K, A, lda, B, ldb, C, ldc,
+                     tiles_m, tiles_n);
   return hipGetLastError();
 }
 

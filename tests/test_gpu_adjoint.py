@@ -62,8 +62,8 @@ class Solver:
 # one 1024 x 513 partial product only, so the L21^T z steps of the two
 # 1024-wide panels run unsplit and the 52-wide last panel's split over K;
 # the other cases split over K throughout.  (1300, 256,
-# 40): the forward solve's products take the streamed MFMA kernel, except the
-# 20-wide last panel's (K not a multiple of 8), which takes the tiled GEMM.
+# 40): 40 right-hand sides (a partial 64-wide tile) and a 20-wide last panel
+# (K = 20 in its products).
 @pytest.mark.parametrize("n,nb,shift,nrhs", [(1000, 64, 0.0, 5),
                                              (1500, 256, 2 * 1500 ** 0.5, 7),
                                              (777, 96, 0.0, 1),
