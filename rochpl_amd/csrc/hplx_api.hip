@@ -461,9 +461,8 @@ int hplx_pdgesv_adjoint(hplx_ctx* ctx, const hplx_pmat* mat, int nrhs,
   if((r = hplx_pdgetrs(ctx, mat, 1, nrhs, gradB, ldgb))) return r;
   if(gradA) {
     // grad_A = -grad_B X^T: N x N from rank nrhs (HBM-bound store)
-    HIPCHK(c, hplx::launch_dgemm(c.stream, N, N, nrhs, -1.0, gradB, ldgb, X,
-                                 ldx, 0.0, gradA, ldga, false, 0, false,
-                                 true));
+    HIPCHK(c, hplx::launch_outer_neg(c.stream, nrhs, N, N, gradB, ldgb, X,
+                                     ldx, gradA, ldga));
     HIPCHK(c, hipStreamSynchronize(c.stream));
   }
   return HPLX_SUCCESS;

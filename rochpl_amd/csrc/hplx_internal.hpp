@@ -160,6 +160,11 @@ hipError_t launch_tn_sub(hipStream_t s, int K, int m, int n, const double* A,
 hipError_t launch_nn_sub(hipStream_t s, int K, int m, int n, const double* A,
                          int lda, const double* B, int ldb, double* C,
                          int ldc);
+// C (m x n, ldc) = -A B^T for A (m x K, lda) and B (n x K, ldb): the
+// adjoint's gradA = -gradB X^T (write-bound; same kernel as launch_tn_sub).
+hipError_t launch_outer_neg(hipStream_t s, int K, int m, int n,
+                            const double* A, int lda, const double* B, int ldb,
+                            double* C, int ldc);
 hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                        double* x);
 

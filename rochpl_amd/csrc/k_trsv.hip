@@ -108,8 +108,10 @@ __global__ __launch_bounds__(1024) void inv_gemv_kernel(
   }
 }
 
-// C (m x n) -= op(A) B for the solves with many right-hand sides: op(A) =
-// A^T with A (K x m) when TA, else A (m x K); B (K x n); all column-major.
+// C (m x n) -= op(A) op(B) for the solves with many right-hand sides:
+// op(A) = A^T with A (K x m) when TA, else A (m x K); op(B) = B^T with
+// B (n x K) when TB, else B (K x n); all column-major.  BETA0: C = -op(A)
+// op(B) instead (the adjoint's rank-nrhs gradA).
 // One 64 x 64 tile of the result per workgroup over
 // k in [s * kc, min(K, (s + 1) * kc)).  DIRECT (one split): C is updated in
 // place; otherwise split s writes its partial tile to P + s * m * n
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(1024) void inv_gemv_kernel(
 // 4 waves of 32 x 32 (2 x 2 blocks of V_MFMA_F64_16X16X4_F64); 32-deep K
 // steps go global -> registers -> LDS (k-major, as the trailing update's
 // stages), the next step's loads in flight during the current step's MFMAs.
-template <bool DIRECT, bool TA>
+template <bool DIRECT, bool TA, bool TB = false, bool BETA0 = false>
 __global__ __launch_bounds__(256) void tn_kernel(
     int K, int m, int n, int kc, const double* __restrict__ A, int lda,
     const double* __restrict__ B, int ldb, double* __restrict__ out, int ldo,
@@ -140,13 +142,16 @@ __global__ __launch_bounds__(256) void tn_kernel(
 #pragma unroll
     for(int r = 0; r < 8; ++r) {
       const int e = t + 256 * r;
-      const int ka = akk(e), ca = ac(e), kb = e & 31, cb = e >> 5;
+      const int ka = akk(e), ca = ac(e);
+      const int kb = TB ? (e >> 6) : (e & 31), cb = TB ? (e & 63) : (e >> 5);
       ra[r] = k + ka < k1 && m0 + ca < m
                   ? (TA ? A[(k + ka) + static_cast<size_t>(m0 + ca) * lda]
                         : A[(m0 + ca) + static_cast<size_t>(k + ka) * lda])
                   : 0.0;
       rb[r] = k + kb < k1 && n0 + cb < n
-                  ? B[(k + kb) + static_cast<size_t>(n0 + cb) * ldb] : 0.0;
+                  ? (TB ? B[(n0 + cb) + static_cast<size_t>(k + kb) * ldb]
+                        : B[(k + kb) + static_cast<size_t>(n0 + cb) * ldb])
+                  : 0.0;
     }
   };
   ud4 acc[2][2];
@@ -160,7 +165,10 @@ __global__ __launch_bounds__(256) void tn_kernel(
     for(int r = 0; r < 8; ++r) {
       const int e = t + 256 * r;
       sa[akk(e)][ac(e)] = ra[r];
-      sb[e & 31][e >> 5] = rb[r];
+      if(TB)
+        sb[e >> 6][e & 63] = rb[r];
+      else
+        sb[e & 31][e >> 5] = rb[r];
     }
     __syncthreads();
     if(k + 32 < k1) load(k + 32);
@@ -192,7 +200,9 @@ __global__ __launch_bounds__(256) void tn_kernel(
       for(int r = 0; r < 4; ++r) {
         const int col = n0 + wn0 + j * 16 + 4 * r + lk;
         if(col >= n) continue;
-        if(DIRECT)
+        if(DIRECT && BETA0)
+          po[row + static_cast<size_t>(col) * ldo] = -acc[i][j][r];
+        else if(DIRECT)
           po[row + static_cast<size_t>(col) * ldo] -= acc[i][j][r];
         else
           po[row + static_cast<size_t>(col) * m] = acc[i][j][r];
@@ -292,6 +302,20 @@ hipError_t launch_nn_sub(hipStream_t s, int K, int m, int n, const double* A,
   hipLaunchKernelGGL((tn_kernel<true, false>), dim3(static_cast<unsigned>(tiles)),
                      dim3(256), 0, s, K, m, n, K, A, lda, B, ldb, C, ldc,
                      tiles_m, tiles_n);
+  return hipGetLastError();
+}
+
+hipError_t launch_outer_neg(hipStream_t s, int K, int m, int n,
+                            const double* A, int lda, const double* B, int ldb,
+                            double* C, int ldc) {
+  if(m <= 0 || n <= 0) return hipSuccess;
+  if(K <= 0) return hipErrorInvalidValue;
+  const int  tiles_m = (m + 63) / 64, tiles_n = (n + 63) / 64;
+  const long tiles   = static_cast<long>(tiles_m) * tiles_n;
+  if(tiles > 0x7fffffffL || K > 0x7fffffff - 32) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((tn_kernel<true, false, true, true>),
+                     dim3(static_cast<unsigned>(tiles)), dim3(256), 0, s, K, m,
+                     n, K, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
   return hipGetLastError();
 }
 
