@@ -280,12 +280,12 @@ hipError_t launch_tn_sub(hipStream_t s, int K, int m, int n, const double* A,
   if(tiles * splits > 0x7fffffffL) return hipErrorInvalidValue;
   const dim3 grid(static_cast<unsigned>(tiles * splits));
   if(splits == 1) {
-    hipLaunchKernelGGL((tn_kernel<true, true>), grid, dim3(256), 0, s, K, m, n, kc, A,
-                       lda, B, ldb, C, ldc, tiles_m, tiles_n);
+    hipLaunchKernelGGL((tn_kernel<true, true>), grid, dim3(256), 0, s, K, m,
+                       n, kc, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL((tn_kernel<false, true>), grid, dim3(256), 0, s, K, m, n, kc, A,
-                     lda, B, ldb, work, m, tiles_m, tiles_n);
+  hipLaunchKernelGGL((tn_kernel<false, true>), grid, dim3(256), 0, s, K, m,
+                     n, kc, A, lda, B, ldb, work, m, tiles_m, tiles_n);
   hipLaunchKernelGGL(tn_reduce_kernel,
                      dim3(static_cast<unsigned>((mn + 255) / 256)), dim3(256),
                      0, s, m, n, static_cast<int>(splits), work, C, ldc);
@@ -299,9 +299,9 @@ hipError_t launch_nn_sub(hipStream_t s, int K, int m, int n, const double* A,
   const int  tiles_m = (m + 63) / 64, tiles_n = (n + 63) / 64;
   const long tiles   = static_cast<long>(tiles_m) * tiles_n;
   if(tiles > 0x7fffffffL || K > 0x7fffffff - 32) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((tn_kernel<true, false>), dim3(static_cast<unsigned>(tiles)),
-                     dim3(256), 0, s, K, m, n, K, A, lda, B, ldb, C, ldc,
-                     tiles_m, tiles_n);
+  hipLaunchKernelGGL((tn_kernel<true, false>),
+                     dim3(static_cast<unsigned>(tiles)), dim3(256), 0, s, K, m,
+                     n, K, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
   return hipGetLastError();
 }
 
