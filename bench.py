@@ -636,7 +636,14 @@ def main():
                 mixed[form] = {"error": str(e)}
     adjoint = None
     if args.mode == "adjoint" or (args.mode == "all" and args.adjoint):
-        adjoint = run_adjoint(ctx, rp, args)
+        try:
+            adjoint = run_adjoint(ctx, rp, args)
+        except (SystemExit, rp.HplxError) as e:
+            # reported in the line like a failing mixed form
+            if args.mode == "adjoint":
+                raise
+            log("adjoint leg failed: %s" % e)
+            adjoint = {"error": str(e)}
     ctx.close()
 
     if args.mode == "adjoint":
