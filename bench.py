@@ -28,9 +28,30 @@ import os
 import sys
 import time
 
+# wall clock of the whole run starts here: every leg checks what is left
+_T0 = time.monotonic()
+
+
+def _host_threads():
+    """(threads the CPU baseline uses, CPUs this process may run on).  All
+    CPUs in the affinity mask, capped by OMP_NUM_THREADS when the host sets
+    it (the CPU share a shared host grants one GPU job)."""
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cpus = os.cpu_count() or 1
+    use = cpus
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        use = min(use, int(cap))
+    if os.environ.get("HPLX_CPU_CORES", "").isdigit():
+        use = int(os.environ["HPLX_CPU_CORES"])
+    return max(use, 1), cpus
+
+
 # host BLAS threads for the CPU baseline; set before numpy/scipy load it
-_CORES = int(os.environ.get("HPLX_CPU_CORES", "16"))
-os.environ.setdefault("OPENBLAS_NUM_THREADS", str(_CORES))
+_CORES, _HOST_CPUS = _host_threads()
+os.environ["OPENBLAS_NUM_THREADS"] = str(_CORES)
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -51,7 +72,19 @@ FORMS = {
 
 
 def log(msg):
-    print("[bench] " + msg, file=sys.stderr, flush=True)
+    print("[bench] %6.1f s  %s" % (time.monotonic() - _T0, msg),
+          file=sys.stderr, flush=True)
+
+
+def remaining(args):
+    """Seconds left of the run's wall-clock budget."""
+    return args.budget_s - (time.monotonic() - _T0)
+
+
+# Expected wall time of each optional leg on one MI355X (allocation,
+# generation, the timed run and its checks), what must be left of the budget
+# for the leg to start.
+LEG_SECONDS = {"bf16": 60.0, "fp8": 45.0, "adjoint": 30.0, "cpu": 20.0}
 
 
 def hpl_flops(n):
@@ -62,22 +95,35 @@ def ref_flops(n):
     return (2.0 / 3.0) * n ** 3 + 1.5 * n ** 2
 
 
-def cpu_baseline(n_sample, nb_sample, cores):
+def cpu_baseline(n_sample, nb_sample, cores, host_cpus, min_s, max_solves):
     """The oracle (a C restatement of the same P=Q=1 algorithm) on host
-    OpenBLAS, on a bounded sample of the workload."""
+    OpenBLAS: the CPU configuration (N=4096, NB=256 by default) solved
+    repeatedly until min_s seconds of solve time have accumulated (at most
+    max_solves solves), each checked with the HPL residual.  value = the
+    mean rate over the timed solves."""
     from oracle import oracle as o
     desc = o.use_host_blas(True)
-    t0 = time.perf_counter()
-    r = o.pdgesv(n_sample, nb_sample)
-    t = time.perf_counter() - t0
-    res = o.residual(n_sample, r["x"])
-    o.use_host_blas(False)
-    return {"value": hpl_flops(n_sample) / t / 1e12, "unit": "TFLOP/s",
-            "cores": cores, "kind": "port",
-            "sample": "one P=Q=1 solve N=%d NB=%d NBMIN=32 NDIV=2 (oracle "
+    times, resids = [], []
+    try:
+        while len(times) < max_solves and (not times or sum(times) < min_s):
+            t0 = time.perf_counter()
+            r = o.pdgesv(n_sample, nb_sample)
+            times.append(time.perf_counter() - t0)
+            resids.append(o.residual(n_sample, r["x"])["resid"])
+            if not resids[-1] < 16.0:
+                raise SystemExit("CPU baseline residual check FAILED: %r"
+                                 % resids[-1])
+    finally:
+        o.use_host_blas(False)
+    t = sum(times) / len(times)
+    return {"value": round(hpl_flops(n_sample) / t / 1e12, 4),
+            "unit": "TFLOP/s", "cores": cores, "host_cpus": host_cpus,
+            "kind": "port",
+            "sample": "%d P=Q=1 solves of N=%d NB=%d NBMIN=32 NDIV=2 (oracle "
                       "hpl_oracle.c, dgemm/dtrsm from %s, %d threads), "
-                      "%.2f s, resid %.3f" % (n_sample, nb_sample, desc, cores,
-                                              t, res["resid"])}
+                      "%.3f s per solve, %.1f s in all, max resid %.4f"
+                      % (len(times), n_sample, nb_sample, desc, cores, t,
+                         sum(times), max(resids))}
 
 
 def load_traffic(n, nb):
@@ -164,7 +210,23 @@ def run_fp64(ctx, rp, args):
         n, nb, rp.hpl_ld(n) * (n + 1) * 8 / 2 ** 30))
     mat = ctx.pmat_alloc(n, nb)
     times, checks, stats = [], [], []
-    for step in range(args.warmup + args.steps):
+    # Warm-up solves only where they are short: above N = 32768 one solve
+    # takes tens of seconds and the code objects are loaded by the first
+    # panel, so the requested warm-up steps are not run there.
+    warmup = args.warmup if n <= 32768 else 0
+    step, warm_done, last = 0, 0, 0.0
+    while True:
+        timed = warm_done >= warmup
+        if timed and len(times) >= max(args.steps, 1):
+            break
+        if step > 0:
+            # the next solve (matgen + solve + check, as long as the last
+            # one) must fit the FP64 share and leave the other legs theirs
+            if timed and times and sum(times) + last > args.max_timed_s:
+                break
+            if remaining(args) - args.reserve_s < 1.1 * last:
+                break
+        s0 = time.monotonic()
         ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -174,7 +236,7 @@ def run_fp64(ctx, rp, args):
             raise SystemExit("pdgesv: exact zero pivot at column %d" % info)
         st = ctx.update_stats()
         chk = ctx.pdtest_check(mat)
-        timed = step >= args.warmup
+        last = time.monotonic() - s0
         log("fp64 %s step %d: %.3f s  %.2f TFLOP/s  resid %.4f  trailing GEMM "
             "%.2f TFLOP/s over %d launches" % (
                 "timed" if timed else "warmup", step, t,
@@ -182,10 +244,13 @@ def run_fp64(ctx, rp, args):
                 st[2] / (st[0] * 1e-3) / 1e12 if st[0] > 0 else 0.0, st[1]))
         if not chk["resid"] < 16.0:
             raise SystemExit("residual check FAILED: %r" % chk)
+        step += 1
         if timed:
             times.append(t)
             checks.append(chk)
             stats.append(st)
+        else:
+            warm_done += 1
 
     pivots = None
     if args.check_pivots == 0:
@@ -235,6 +300,8 @@ def run_fp64(ctx, rp, args):
         "residual_r0_form": round(max(c["resid_r0_form"] for c in checks), 6),
         "tflops_ref_formula": round(ref_flops(n) / t_mean / 1e12, 3),
         "pivot_prefix_check": pivots,
+        "steps_run": len(times), "warmup_run": warm_done,
+        "step_seconds": [round(t, 3) for t in times],
     }
 
 
@@ -565,9 +632,21 @@ def main():
     ap.add_argument("--bf16-n", type=int, default=0)
     ap.add_argument("--fp8-n", type=int, default=0)
     ap.add_argument("--mixed-warmup", type=int, default=0)
-    ap.add_argument("--cpu-n", type=int, default=24576)
+    ap.add_argument("--cpu-n", type=int, default=4096,
+                    help="N of the CPU baseline's solves (the CPU "
+                         "configuration: N=4096, NB=256)")
     ap.add_argument("--cpu-nb", type=int, default=256)
+    ap.add_argument("--cpu-min-s", type=float, default=10.0,
+                    help="repeat the CPU baseline's solve until this much "
+                         "solve time has accumulated")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-clock budget of the whole run from process "
+                         "start: FP64 steps beyond the first and the other "
+                         "legs run only while it allows")
+    ap.add_argument("--max-timed-s", type=float, default=120.0,
+                    help="timed FP64 solve seconds after which no further "
+                         "step starts")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
                     help="skip the adjoint-solve leg of the default run")
     ap.add_argument("--adj-n", type=int, default=16384)
@@ -619,12 +698,30 @@ def main():
 
     ctx = rp.Context(0)
     out = {"metric": METRIC}
+    # legs after the first run only while the wall-clock budget leaves them
+    # their expected time; a skipped leg is named in the line
+    legs = [f for f in ("bf16", "fp8") if args.mode in ("all", f)]
+    if args.mode == "adjoint" or (args.mode == "all" and args.adjoint):
+        legs.append("adjoint")
+    if args.mode != "adjoint" and not args.no_cpu:
+        legs.append("cpu")
+    skipped = []
+
+    def leg_fits(leg):
+        if remaining(args) >= LEG_SECONDS[leg]:
+            return True
+        log("%s leg skipped: %.0f s of the %.0f s budget left" % (
+            leg, remaining(args), args.budget_s))
+        skipped.append(leg)
+        return False
+
     fp64 = None
     if args.mode in ("all", "fp64"):
+        args.reserve_s = sum(LEG_SECONDS[leg] for leg in legs)
         fp64 = run_fp64(ctx, rp, args)
     mixed = {}
     for form in ("bf16", "fp8"):
-        if args.mode in ("all", form):
+        if form in legs and (args.mode == form or leg_fits(form)):
             try:
                 mixed[form] = run_mixed(ctx, rp, form, args)
             except (SystemExit, rp.HplxError) as e:
@@ -635,7 +732,7 @@ def main():
                 log("%s form failed: %s" % (form, e))
                 mixed[form] = {"error": str(e)}
     adjoint = None
-    if args.mode == "adjoint" or (args.mode == "all" and args.adjoint):
+    if "adjoint" in legs and (args.mode == "adjoint" or leg_fits("adjoint")):
         try:
             adjoint = run_adjoint(ctx, rp, args)
         except (SystemExit, rp.HplxError) as e:
@@ -671,7 +768,10 @@ def main():
         head = fp64
         out.update({
             "value": round(fp64["value"], 3), "unit": "TFLOP/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "n_gpus": 1, "steps": fp64["steps_run"],
+            "warmup": fp64["warmup_run"],
+            "steps_requested": args.steps, "warmup_requested": args.warmup,
+            "step_seconds": fp64["step_seconds"],
             "ms_per_step": round(fp64["t_mean"] * 1e3, 2),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64",
@@ -708,11 +808,16 @@ def main():
         out[form] = m
     if adjoint is not None:
         out["adjoint"] = adjoint
-    if not args.no_cpu:
-        log("CPU baseline sample N=%d" % args.cpu_n)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_nb, _CORES)
-    else:
-        out["cpu_baseline"] = None
+    out["cpu_baseline"] = None
+    if "cpu" in legs and leg_fits("cpu"):
+        log("CPU baseline: N=%d NB=%d on %d threads (%d CPUs in the affinity "
+            "mask)" % (args.cpu_n, args.cpu_nb, _CORES, _HOST_CPUS))
+        out["cpu_baseline"] = cpu_baseline(
+            args.cpu_n, args.cpu_nb, _CORES, _HOST_CPUS,
+            min(args.cpu_min_s, max(remaining(args) - 5.0, 1.0)), 200)
+    out["legs_skipped"] = skipped
+    out["wall_s"] = round(time.monotonic() - _T0, 1)
+    out["budget_s"] = args.budget_s
     print(json.dumps(out), flush=True)
 
 

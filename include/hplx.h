@@ -56,7 +56,11 @@ extern "C" {
  * pfact/rfact/L1notran/Unotran values; only the reference's P=Q=1
  * configuration (pfact=rfact=HPLX_RIGHT_LOOKING, L1 and U transposed,
  * depth 1, long swap, equil 0) is implemented, anything else returns
- * HPLX_ERR_UNSUPPORTED from hplx_pdgesv. */
+ * HPLX_ERR_UNSUPPORTED from hplx_pdgesv.
+ * frac is accepted and not read: at P=1 the reference itself ignores it
+ * (HPL_pdpanel_init sets nu0 = JB look-ahead columns, nu1 = 0 and
+ * nu2 = the rest when nprow == 1, src/panel/HPL_pdpanel_init.cpp:176-184;
+ * only its nprow > 1 branch, :185-197, splits by frac). */
 typedef struct hplx_palg {
   int    btopo;    /* row broadcast topology (no-op at Q=1)        */
   int    depth;    /* look-ahead depth (must be 1, src/HPL_pdinfo.cpp:930-935) */
@@ -70,13 +74,19 @@ typedef struct hplx_palg {
   int    L1notran; /* 0: L1 transposed (required)                 */
   int    Unotran;  /* 0: U transposed (required)                  */
   int    align;    /* memory alignment in doubles                 */
-  double frac;     /* update split fraction (kept; P=1 ignores it) */
+  double frac;     /* update split fraction (accepted, not read at P=1) */
   int    its;      /* iterations of the timed solve               */
 } hplx_palg;
 
 /* Fill *palg with the reference defaults (src/HPL_pdinfo.cpp:487-562):
  * right/right, nbmin 32, nbdiv 2, depth 1, long swap, thresh 64, frac 0.3. */
 void hplx_palg_default(hplx_palg* palg);
+
+/* The checks HPL_pdinfo applies to these settings (src/HPL_pdinfo.cpp:
+ * 925-1004: depth 1, long swap, U transposed, no equilibration) plus the
+ * variants this library builds, without a device: HPLX_SUCCESS, or the code
+ * hplx_pdgesv / hplx_pdgetrf would return for *palg. */
+int hplx_palg_check(const hplx_palg* palg);
 
 /* ---- distributed matrix: mirrors HPL_T_pmat (include/hpl_pgesv.hpp:83-109)
  * A is the augmented [A|b], n x (n+1), leading dimension ld, b = column n.

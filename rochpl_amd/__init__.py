@@ -36,7 +36,7 @@ HPLX_UPDATE_FP32, HPLX_UPDATE_BF16, HPLX_UPDATE_FP8 = 0, 1, 2
 
 # Every symbol include/hplx.h declares (checked by tests/test_abi.py).
 EXPORTS = [
-    "hplx_palg_default", "hplx_ctx_create", "hplx_ctx_destroy",
+    "hplx_palg_default", "hplx_palg_check", "hplx_ctx_create", "hplx_ctx_destroy",
     "hplx_last_error", "hplx_set_stream", "hplx_synchronize", "hplx_num_cus",
     "hplx_malloc", "hplx_free", "hplx_memcpy_h2d", "hplx_memcpy_d2h",
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
@@ -135,6 +135,7 @@ def lib():
     P = ctypes.POINTER
     sig = {
         "hplx_palg_default": (None, [P(PAlg)]),
+        "hplx_palg_check": (i32, [P(PAlg)]),
         "hplx_ctx_create": (i32, [i32, P(vp)]),
         "hplx_ctx_destroy": (i32, [vp]),
         "hplx_last_error": (ctypes.c_char_p, [vp]),

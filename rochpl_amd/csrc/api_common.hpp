@@ -43,21 +43,50 @@ inline int use(hplx_ctx* ctx) {
   return HPLX_SUCCESS;
 }
 
-// The P=Q=1 configurations this library builds (HPL_T_palg values).
-inline int check_palg(Ctx& c, const hplx_palg* p) {
-  if(!p) return fail(c, HPLX_ERR_INVALID, "null palg");
-  if(p->pfact != HPLX_RIGHT_LOOKING || p->rfact != HPLX_RIGHT_LOOKING)
-    return fail(c, HPLX_ERR_UNSUPPORTED,
-                "only right-looking panel/recursive factorisation is built "
-                "(pfact=%d rfact=%d)",
-                p->pfact, p->rfact);
-  if(p->L1notran != 0 || p->Unotran != 0)
-    return fail(c, HPLX_ERR_UNSUPPORTED, "L1 and U must be in transposed form");
-  if(p->equil != 0)
-    return fail(c, HPLX_ERR_UNSUPPORTED, "equilibration is not supported");
-  if(p->nbmin < 1 || p->nbdiv < 2)
-    return fail(c, HPLX_ERR_INVALID, "nbmin=%d nbdiv=%d", p->nbmin, p->nbdiv);
+// The P=Q=1 configurations this library builds (HPL_T_palg values):
+// HPLX_SUCCESS, or an error code with its message in msg.  No device needed.
+inline int palg_status(const hplx_palg* p, char* msg, size_t len) {
+  if(!p) {
+    std::snprintf(msg, len, "null palg");
+    return HPLX_ERR_INVALID;
+  }
+  if(p->pfact != HPLX_RIGHT_LOOKING || p->rfact != HPLX_RIGHT_LOOKING) {
+    std::snprintf(msg, len,
+                  "only right-looking panel/recursive factorisation is built "
+                  "(pfact=%d rfact=%d)", p->pfact, p->rfact);
+    return HPLX_ERR_UNSUPPORTED;
+  }
+  if(p->L1notran != 0 || p->Unotran != 0) {
+    std::snprintf(msg, len, "L1 and U must be in transposed form");
+    return HPLX_ERR_UNSUPPORTED;
+  }
+  if(p->equil != 0) {
+    std::snprintf(msg, len, "equilibration is not supported");
+    return HPLX_ERR_UNSUPPORTED;
+  }
+  if(p->nbmin < 1 || p->nbdiv < 2) {
+    std::snprintf(msg, len, "nbmin=%d nbdiv=%d", p->nbmin, p->nbdiv);
+    return HPLX_ERR_INVALID;
+  }
+  // the reference refuses both in HPL_pdinfo (src/HPL_pdinfo.cpp:930-935
+  // "Value of DEPTH must be 1", :950-955 "Value of SWAP must be 1")
+  if(p->depth != 1) {
+    std::snprintf(msg, len, "look-ahead depth must be 1 (got %d)", p->depth);
+    return HPLX_ERR_UNSUPPORTED;
+  }
+  if(p->fswap != HPLX_SWAP01) {
+    std::snprintf(msg, len,
+                  "only long swapping (HPLX_SWAP01) is implemented (got %d)",
+                  p->fswap);
+    return HPLX_ERR_UNSUPPORTED;
+  }
   return HPLX_SUCCESS;
+}
+
+inline int check_palg(Ctx& c, const hplx_palg* p) {
+  char      msg[256];
+  const int r = palg_status(p, msg, sizeof(msg));
+  return r ? fail(c, r, "%s", msg) : HPLX_SUCCESS;
 }
 
 // Progress lines of a panel loop (HPLX_PROGRESS=<panels>, the counterpart of

@@ -56,6 +56,11 @@ void hplx_palg_default(hplx_palg* p) {
   p->its      = 1;
 }
 
+int hplx_palg_check(const hplx_palg* palg) {
+  char msg[256];
+  return hplx::palg_status(palg, msg, sizeof(msg));
+}
+
 int hplx_ctx_create(int device, hplx_ctx** out) {
   if(!out) return HPLX_ERR_INVALID;
   *out          = nullptr;

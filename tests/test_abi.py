@@ -95,3 +95,34 @@ def test_errors_without_device_are_codes_not_exits():
     rc = rp.lib().hplx_ctx_create(10 ** 6, ctypes.byref(h))
     assert rc < 0 and not h.value
     assert rp.lib().hplx_last_error(None) == b"null context"
+
+
+@pytest.mark.parametrize("field,value", [
+    ("depth", 0), ("depth", 2),                      # src/HPL_pdinfo.cpp:930-935
+    ("fswap", rp.HPLX_SWAP00), ("fswap", rp.HPLX_SW_MIX),  # :948-955
+    ("Unotran", 1),                                  # :979-986
+    ("equil", 1),                                    # :996-1003
+])
+def test_palg_settings_the_reference_refuses_are_unsupported(field, value):
+    """HPL_pdinfo refuses these HPL.dat settings; hplx_palg_check (the check
+    hplx_pdgesv / hplx_pdgetrf run first) reports them without a device."""
+    if not os.path.exists(rp.LIB_PATH):
+        pytest.skip("libhplx.so not built")
+    p = rp.palg_default()
+    assert rp.lib().hplx_palg_check(ctypes.byref(p)) == rp.HPLX_SUCCESS
+    setattr(p, field, value)
+    assert rp.lib().hplx_palg_check(ctypes.byref(p)) == rp.HPLX_ERR_UNSUPPORTED
+
+
+def test_palg_invalid_recursion_and_null():
+    if not os.path.exists(rp.LIB_PATH):
+        pytest.skip("libhplx.so not built")
+    for field, value in (("nbmin", 0), ("nbdiv", 1)):
+        p = rp.palg_default()
+        setattr(p, field, value)
+        assert rp.lib().hplx_palg_check(ctypes.byref(p)) == rp.HPLX_ERR_INVALID
+    assert rp.lib().hplx_palg_check(None) == rp.HPLX_ERR_INVALID
+    # frac is not read at P=1 (src/panel/HPL_pdpanel_init.cpp:176-184)
+    p = rp.palg_default()
+    p.frac = 0.6
+    assert rp.lib().hplx_palg_check(ctypes.byref(p)) == rp.HPLX_SUCCESS
