@@ -20,7 +20,10 @@ that a test can compute in FP64 exactly the products the MFMA units are given:
 Pins (tests/test_mxp_oracle.py): the BF16 and e4m3 rounding against torch's
 own CPU conversions (torch.bfloat16, torch.float8_e4m3fn) over random and
 edge-case values, bit for bit; the split identities (terms are exact
-differences); the scale rule against its definition by brute force.
+differences); the scale rule against its definition by brute force; the
+tile-scale placement by operands whose FP8 product is exact
+(scaled_e4m3_operands: update_product must equal L @ U, and a transposed or
+shifted exponent grid must not).
 """
 from __future__ import annotations
 
@@ -136,3 +139,35 @@ def update_abs_product(update: int, terms: int, L, U) -> np.ndarray:
     L = np.abs(np.asarray(L, dtype=np.float64))
     U = np.abs(np.asarray(U, dtype=np.float64))
     return L @ U
+
+
+def scaled_e4m3_operands(seed=6):
+    """L (300 x 384) and U (384 x 260) built tile by tile (128 x 128) from
+    small e4m3 integers times a power of two that differs for every tile:
+    L tile (row tile bi, K tile kt) carries 2^-(bi + 3 kt), U tile (K tile
+    kt, column tile bj) 2^-(3 bj - kt).  Each tile holds one entry of
+    magnitude 448 (so its scale exponent is exactly that tile's power) and
+    otherwise integers in [-15, 15]: quantisation is exact, and every
+    product, partial sum and output of the FP8 update is exactly
+    representable in FP32 (for one output element the K tiles' powers span
+    2^4 and each 128-deep integer sum stays below 2^18)."""
+    rng = np.random.default_rng(seed)
+    R, K, C = 300, 384, 260
+    sL = np.array([[bi + 3 * kt for kt in range(3)] for bi in range(3)])
+    sU = np.array([[3 * bj - kt for bj in range(3)] for kt in range(3)])
+
+    def build(rows, cols, s_of):
+        M = np.zeros((rows, cols), np.float64)
+        for ti in range((rows + 127) // 128):
+            for tj in range((cols + 127) // 128):
+                r0, c0 = ti * 128, tj * 128
+                r1, c1 = min(rows, r0 + 128), min(cols, c0 + 128)
+                q = rng.integers(-15, 16, (r1 - r0, c1 - c0)).astype(np.float64)
+                q[rng.integers(r1 - r0), rng.integers(c1 - c0)] = \
+                    448.0 * rng.choice([-1.0, 1.0])
+                M[r0:r1, c0:c1] = np.ldexp(q, -int(s_of(ti, tj)))
+        return M.astype(np.float32)
+
+    L = build(R, K, lambda bi, kt: sL[bi, kt])
+    U = build(K, C, lambda kt, bj: sU[kt, bj])
+    return L, U, sL, sU

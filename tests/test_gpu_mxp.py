@@ -109,6 +109,51 @@ def test_mxp_update_matches_oracle(ctx, update, terms, storage, m, n, k):
     assert np.array_equal(out[m:], C[m:])
 
 
+def test_mxp_fp8_update_exact_on_scaled_tiles(ctx):
+    """Every 128 x 128 tile of L (row tile x K tile) and U (K tile x column
+    tile) is an e4m3 matrix times its own power of two, all different
+    (oracle scaled_e4m3_operands): the FP8 update is then exact in FP32, so
+    C - L U must come back bit for bit.  A transposed, shifted or mixed-up
+    exponent index in the packing or the GEMM scales some tile by a wrong
+    power of two and breaks the equality."""
+    L, U, sL, sU = mo.scaled_e4m3_operands()
+    m, k = L.shape
+    n = U.shape[1]
+    say("mxp_update fp8 exact scaled tiles %dx%dx%d" % (m, n, k))
+    rng = np.random.default_rng(7)
+    ldl, ldu, ldc = m + 4, k + 2, m + 6
+    Lp = np.zeros((ldl, k), np.float32, order="F")
+    Up = np.zeros((ldu, n), np.float32, order="F")
+    Lp[:m], Up[:k] = L, U
+    # C: integers times 2^-(bi + 3 bj + 4) in each output tile, so that
+    # C - L U is exact in FP32 as well
+    C = np.zeros((ldc, n), np.float32, order="F")
+    for bi in range((m + 127) // 128):
+        for bj in range((n + 127) // 128):
+            rs, cs = slice(bi * 128, min(m, bi * 128 + 128)), \
+                slice(bj * 128, min(n, bj * 128 + 128))
+            C[rs, cs] = np.ldexp(rng.integers(-1000, 1001, (rs.stop - rs.start,
+                                                          cs.stop - cs.start))
+                                 .astype(np.float64), -(bi + 3 * bj + 4))
+    exact = C[:m].astype(np.float64) - L.astype(np.float64) @ \
+        U.astype(np.float64)
+    assert np.array_equal(exact.astype(np.float32).astype(np.float64), exact)
+    dL, dU, dC = upload(ctx, Lp), upload(ctx, Up), upload(ctx, C)
+    try:
+        ctx.mxp_update(rp.HPLX_UPDATE_FP8, 1, rp.HPLX_STORE_FP32, m, n, k,
+                       dL.ptr, ldl, dU.ptr, ldu, dC.ptr, ldc)
+        out = dC.download((ldc, n), np.float32)
+    finally:
+        for d in (dL, dU, dC):
+            d.free()
+    bad = np.argwhere(out[:m].astype(np.float64) != exact)
+    print("[mxp_update fp8 exact] %d of %d entries differ" % (len(bad),
+                                                              m * n),
+          flush=True)
+    assert len(bad) == 0, bad[:10]
+    assert np.array_equal(out[m:], C[m:])
+
+
 # ---------------------------------------------------------------------------
 # FP32 panel factorisation
 # ---------------------------------------------------------------------------

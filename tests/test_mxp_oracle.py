@@ -100,3 +100,36 @@ def test_update_product_accuracy_order():
     assert errs[(1, 3)] < 1e-7 < errs[(1, 2)] < 1e-4 < errs[(1, 1)] < 1e-2
     assert errs[(2, 1)] < 0.1
     assert np.array_equal(mo.update_product(0, 1, L, U), exact)
+
+
+def test_fp8_tile_scales_exact_on_scaled_e4m3_tiles():
+    """update_product(FP8) equals L @ U exactly when every tile is an e4m3
+    matrix times its own power of two, and the exponent grids come out as
+    built; a transposed or shifted exponent grid (planted here in the
+    oracle's own formula) changes the product."""
+    L, U, sL, sU = mo.scaled_e4m3_operands()
+    exact = L.astype(np.float64) @ U.astype(np.float64)
+    ql, el = mo.fp8_quantize(L, True)
+    qu, eu = mo.fp8_quantize(U, False)
+    assert np.array_equal(el, sL) and np.array_equal(eu, sU)
+    assert np.array_equal(np.ldexp(ql.astype(np.float64),
+                                   -np.repeat(np.repeat(el, 128, 0), 128, 1)
+                                   [:L.shape[0], :L.shape[1]]), L)
+    assert np.array_equal(mo.update_product(2, 1, L, U), exact)
+    # the same product with a mis-indexed exponent grid
+    K = L.shape[1]
+
+    def with_grids(gl, gu):
+        out = np.zeros(exact.shape)
+        for kt in range(K // 128):
+            ks = slice(kt * 128, (kt + 1) * 128)
+            s_l = np.repeat(np.ldexp(1.0, -gl[:, kt]), 128)[:L.shape[0]]
+            s_u = np.repeat(np.ldexp(1.0, -gu[kt, :]), 128)[:U.shape[1]]
+            out += (ql[:, ks].astype(np.float64) * s_l[:, None]) @ \
+                (qu[ks, :].astype(np.float64) * s_u[None, :])
+        return out
+
+    assert np.array_equal(with_grids(el, eu), exact)
+    for gl, gu in [(el.T, eu), (el, eu.T), (np.roll(el, 1, 0), eu),
+                   (el, np.roll(eu, 1, 1)), (np.roll(el, 1, 1), eu)]:
+        assert not np.array_equal(with_grids(gl, gu), exact)
