@@ -117,6 +117,19 @@ int         hplx_synchronize(hplx_ctx* ctx);
 /* Number of compute units of the context's device. */
 int         hplx_num_cus(hplx_ctx* ctx);
 
+/* Scheduling options of a context.  HPLX_OPT_LOOKAHEAD: the reference's
+ * DEPTH=1 look-ahead (src/pgesv/HPL_pdgesv.cpp:168-245), which at P=Q=1 it
+ * runs on one in-order stream (src/HPL_InitGPU.cpp:67,93).  1 (default):
+ * hplx_pdgesv / hplx_pdgetrf factor panel k+1 on a second stream while the
+ * update of step k runs, on CUs the update leaves free for it; 0: every
+ * step in order on the context's stream.  Pivots and results are the same
+ * bit for bit either way.  The environment variable HPLX_LOOKAHEAD=0|1 sets
+ * the default of new contexts.  hplx_set_option returns HPLX_ERR_INVALID for
+ * an unknown option or value; hplx_get_option writes the current value. */
+#define HPLX_OPT_LOOKAHEAD 1
+int         hplx_set_option(hplx_ctx* ctx, int option, int value);
+int         hplx_get_option(hplx_ctx* ctx, int option, int* value);
+
 /* ---- plain device memory helpers (for callers without a runtime of their
  * own, e.g. ctypes tests) ------------------------------------------------- */
 int hplx_malloc(hplx_ctx* ctx, size_t bytes, void** dptr);

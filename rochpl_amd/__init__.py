@@ -12,6 +12,7 @@ from __future__ import annotations
 import ctypes
 import os
 import subprocess
+import sys
 
 import numpy as np
 
@@ -33,11 +34,13 @@ HPLX_SWAP00, HPLX_SWAP01, HPLX_SW_MIX, HPLX_NO_SWP = 451, 452, 453, 499
 # mixed precision (no counterpart in the reference)
 HPLX_STORE_FP32, HPLX_STORE_FP16 = 32, 16
 HPLX_UPDATE_FP32, HPLX_UPDATE_BF16, HPLX_UPDATE_FP8 = 0, 1, 2
+HPLX_OPT_LOOKAHEAD = 1
 
 # Every symbol include/hplx.h declares (checked by tests/test_abi.py).
 EXPORTS = [
     "hplx_palg_default", "hplx_palg_check", "hplx_ctx_create", "hplx_ctx_destroy",
     "hplx_last_error", "hplx_set_stream", "hplx_synchronize", "hplx_num_cus",
+    "hplx_set_option", "hplx_get_option",
     "hplx_malloc", "hplx_free", "hplx_memcpy_h2d", "hplx_memcpy_d2h",
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
     "hplx_pdtest_check", "hplx_pdfact", "hplx_pipid", "hplx_dlaswp00N",
@@ -142,6 +145,8 @@ def lib():
         "hplx_set_stream": (i32, [vp, vp]),
         "hplx_synchronize": (i32, [vp]),
         "hplx_num_cus": (i32, [vp]),
+        "hplx_set_option": (i32, [vp, i32, i32]),
+        "hplx_get_option": (i32, [vp, i32, P(i32)]),
         "hplx_malloc": (i32, [vp, sz, P(vp)]),
         "hplx_free": (i32, [vp, vp]),
         "hplx_memcpy_h2d": (i32, [vp, vp, vp, sz]),
@@ -238,10 +243,16 @@ class DeviceArray:
 
     def free(self) -> None:
         if self.ptr:
-            lib().hplx_free(self.ctx.h, ctypes.c_void_p(self.ptr))
+            if self.ctx.h:
+                lib().hplx_free(self.ctx.h, ctypes.c_void_p(self.ptr))
             self.ptr = None
 
     def __del__(self):
+        # No HIP call while the interpreter shuts down: the HIP runtime (and
+        # a profiler attached to it) may already be finalised by then, and
+        # the process is about to release the memory anyway.
+        if sys.is_finalizing():
+            return
         try:
             self.free()
         except Exception:
@@ -284,6 +295,19 @@ class Context:
 
     def set_stream(self, stream_handle: int | None) -> None:
         self._chk(lib().hplx_set_stream(self.h, stream_handle))
+
+    @property
+    def lookahead(self) -> bool:
+        """HPLX_OPT_LOOKAHEAD: panel k+1 factored beside update k."""
+        v = ctypes.c_int()
+        self._chk(lib().hplx_get_option(self.h, HPLX_OPT_LOOKAHEAD,
+                                        ctypes.byref(v)))
+        return bool(v.value)
+
+    @lookahead.setter
+    def lookahead(self, on: bool) -> None:
+        self._chk(lib().hplx_set_option(self.h, HPLX_OPT_LOOKAHEAD,
+                                        1 if on else 0))
 
     def alloc(self, nbytes: int) -> DeviceArray:
         return DeviceArray(self, nbytes)

@@ -111,9 +111,10 @@ class Progress {
     return hipSuccess;
   }
 
-  hipError_t panel_queued(int k) {
+  // panel k's work has been queued; s: the stream its last step is on
+  hipError_t panel_queued(int k, hipStream_t s) {
     if(c_.progress_every <= 0) return hipSuccess;
-    hipError_t r = hipEventRecord(c_.ev_prog[k % (2 * kLag)], c_.stream);
+    hipError_t r = hipEventRecord(c_.ev_prog[k % (2 * kLag)], s);
     if(r != hipSuccess) return r;
     const int done = k - kLag;
     if(done < 0) return hipSuccess;

@@ -34,6 +34,54 @@ using hplx::align_up;
 
 int hpl_ld(int n) { return ((n > 0 ? n : 0) + 95) / 128 * 128 + 32; }
 
+constexpr int kCensusGrid = 16384;  // workgroups of the CU census
+
+// The CU sets the look-ahead hands to a panel factorisation.  A census
+// kernel records the __smid() of the CU each of kCensusGrid short
+// workgroups ran on; vacate set r (r = 1 .. 32) then marks the first r CUs
+// of every XCD, taken round-robin over its shader engines.  If the census
+// did not see the same number of CUs on every XCD (or fewer than r), sets
+// that cannot be formed stay empty and c.cus_per_xcd records how many can.
+int cu_sets(Ctx& c, int* census_dev) {
+  std::vector<int> ids(kCensusGrid);
+  if(hplx::launch_cu_census(c.own_stream, census_dev, kCensusGrid) != hipSuccess ||
+     hipMemcpyAsync(ids.data(), census_dev, sizeof(int) * kCensusGrid,
+                    hipMemcpyDeviceToHost, c.own_stream) != hipSuccess ||
+     hipStreamSynchronize(c.own_stream) != hipSuccess)
+    return HPLX_ERR_HIP;
+  bool seen[512] = {false};
+  for(int v : ids)
+    if(v >= 0 && v < 512) seen[v] = true;
+  // per XCD: CU ids grouped by shader engine, then interleaved over engines
+  std::vector<int> per_xcd[8];
+  int              cnt[8] = {0};
+  for(int x = 0; x < 8; ++x) {
+    std::vector<int> se[4];
+    for(int e = 0; e < 4; ++e)
+      for(int u = 0; u < 16; ++u)
+        if(seen[(x << 6) | (e << 4) | u]) se[e].push_back((x << 6) | (e << 4) | u);
+    for(size_t k = 0; k < 16; ++k)
+      for(int e = 0; e < 4; ++e)
+        if(k < se[e].size()) per_xcd[x].push_back(se[e][k]);
+    cnt[x] = static_cast<int>(per_xcd[x].size());
+  }
+  int r_max = cnt[0];
+  for(int x = 1; x < 8; ++x) r_max = cnt[x] < r_max ? cnt[x] : r_max;
+  if(r_max > hplx::kVacateSets) r_max = hplx::kVacateSets;
+  c.cus_per_xcd = r_max;
+  std::vector<unsigned long long> masks(hplx::kVacateSets * 8, 0ull);
+  for(int r = 1; r <= r_max; ++r)
+    for(int x = 0; x < 8; ++x)
+      for(int k = 0; k < r; ++k) {
+        const int id = per_xcd[x][k];
+        masks[(r - 1) * 8 + (id >> 6)] |= 1ull << (id & 63);
+      }
+  if(hipMemcpy(c.vacate, masks.data(), masks.size() * 8,
+               hipMemcpyHostToDevice) != hipSuccess)
+    return HPLX_ERR_HIP;
+  return HPLX_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -97,29 +145,46 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   const size_t nperm  = 2 * hplx::kMaxNB * sizeof(int);
   const size_t nslots = 2 * static_cast<size_t>(hplx::kMaxGrid + 1) *
                         hplx::kSlotStride * sizeof(double);
-  size_t off_perm  = align_up(nLinv, 256);
-  size_t off_slots = align_up(off_perm + nperm, 256);
-  size_t off_flags = align_up(off_slots + nslots, 256);
-  size_t off_dout  = off_flags + 256;
-  size_t total     = off_dout + 256;
-  if(hipMalloc(&c.small_base, total) != hipSuccess) {
+  size_t off_perm   = align_up(nLinv, 256);
+  size_t off_slots  = align_up(off_perm + nperm, 256);
+  size_t off_flags  = align_up(off_slots + nslots, 256);
+  size_t off_dout   = off_flags + 256;
+  size_t off_linv2  = align_up(off_dout + 256, 256);
+  size_t off_perm2  = align_up(off_linv2 + nLinv, 256);
+  size_t off_claim  = align_up(off_perm2 + nperm, 256);
+  size_t off_stop   = align_up(off_claim + 2 * hplx::kClaimInts * sizeof(int), 256);
+  size_t off_vacate = off_stop + 256;
+  size_t off_census = align_up(off_vacate + hplx::kVacateSets * 8 * 8, 256);
+  size_t total      = off_census + kCensusGrid * sizeof(int);
+  auto undo = [&](int code) {
+    if(c.small_base) (void)hipFree(c.small_base);
+    if(c.ustream) (void)hipStreamDestroy(c.ustream);
+    if(c.pstream) (void)hipStreamDestroy(c.pstream);
     (void)hipStreamDestroy(c.own_stream);
     delete ctx;
-    return HPLX_ERR_ALLOC;
-  }
+    return code;
+  };
+  if(hipStreamCreateWithFlags(&c.ustream, hipStreamNonBlocking) != hipSuccess ||
+     hipStreamCreateWithFlags(&c.pstream, hipStreamNonBlocking) != hipSuccess)
+    return undo(HPLX_ERR_HIP);
+  if(hipMalloc(&c.small_base, total) != hipSuccess) return undo(HPLX_ERR_ALLOC);
   char* b   = static_cast<char*>(c.small_base);
   c.Linv    = reinterpret_cast<double*>(b);
   c.permU   = reinterpret_cast<int*>(b + off_perm);
   c.slots   = reinterpret_cast<double*>(b + off_slots);
   c.flags   = reinterpret_cast<int*>(b + off_flags);
   c.dout    = reinterpret_cast<double*>(b + off_dout);
+  c.Linv2   = reinterpret_cast<double*>(b + off_linv2);
+  c.permU2  = reinterpret_cast<int*>(b + off_perm2);
+  c.claim   = reinterpret_cast<int*>(b + off_claim);
+  c.stop    = reinterpret_cast<int*>(b + off_stop);
+  c.vacate  = reinterpret_cast<unsigned long long*>(b + off_vacate);
+  int* census = reinterpret_cast<int*>(b + off_census);
   if(hipMemset(c.small_base, 0, total) != hipSuccess ||
-     hipDeviceSynchronize() != hipSuccess) {
-    (void)hipFree(c.small_base);
-    (void)hipStreamDestroy(c.own_stream);
-    delete ctx;
-    return HPLX_ERR_HIP;
-  }
+     hipDeviceSynchronize() != hipSuccess)
+    return undo(HPLX_ERR_HIP);
+  if(const char* la = std::getenv("HPLX_LOOKAHEAD")) c.lookahead = std::atoi(la) != 0;
+  if(cu_sets(c, census) != HPLX_SUCCESS) return undo(HPLX_ERR_HIP);
   *out = ctx;
   return HPLX_SUCCESS;
 }
@@ -129,9 +194,14 @@ int hplx_ctx_destroy(hplx_ctx* ctx) {
   Ctx& c = ctx->c;
   (void)hipSetDevice(c.device);
   (void)hipStreamSynchronize(c.stream);
+  if(c.ustream) (void)hipStreamSynchronize(c.ustream);
+  if(c.pstream) (void)hipStreamSynchronize(c.pstream);
   for(hipEvent_t e : c.ev) (void)hipEventDestroy(e);
   for(hipEvent_t e : c.ev_prog) (void)hipEventDestroy(e);
+  for(hipEvent_t e : c.ev_sync) (void)hipEventDestroy(e);
   if(c.small_base) (void)hipFree(c.small_base);
+  if(c.ustream) (void)hipStreamDestroy(c.ustream);
+  if(c.pstream) (void)hipStreamDestroy(c.pstream);
   if(c.own_stream) (void)hipStreamDestroy(c.own_stream);
   delete ctx;
   return HPLX_SUCCESS;
@@ -158,6 +228,24 @@ int hplx_synchronize(hplx_ctx* ctx) {
 }
 
 int hplx_num_cus(hplx_ctx* ctx) { return ctx ? ctx->c.num_cus : 0; }
+
+int hplx_set_option(hplx_ctx* ctx, int option, int value) {
+  int r = use(ctx);
+  if(r) return r;
+  if(option != HPLX_OPT_LOOKAHEAD || (value != 0 && value != 1))
+    return fail(ctx->c, HPLX_ERR_INVALID, "set_option: option %d value %d",
+                option, value);
+  ctx->c.lookahead = value;
+  return HPLX_SUCCESS;
+}
+
+int hplx_get_option(hplx_ctx* ctx, int option, int* value) {
+  if(!ctx || !value) return HPLX_ERR_INVALID;
+  if(option != HPLX_OPT_LOOKAHEAD)
+    return fail(ctx->c, HPLX_ERR_INVALID, "get_option: option %d", option);
+  *value = ctx->c.lookahead;
+  return HPLX_SUCCESS;
+}
 
 int hplx_malloc(hplx_ctx* ctx, size_t bytes, void** dptr) {
   int r = use(ctx);
@@ -265,9 +353,113 @@ int hplx_pdrandmat(hplx_ctx* ctx, int m, int n, int nb, double* A, int lda,
   return HPLX_SUCCESS;
 }
 
+// Steps of the panel loop on the FP64 matrix (shared by the serial and the
+// look-ahead schedules).
+struct Fp64Loop {
+  Ctx&         c;
+  hplx_pmat*   mat;
+  hplx::MatWs* ws;
+  double*      A;
+  int          N, nb, lda, nbmin, nbdiv;
+
+  double* Linv(int k) const { return (k & 1) ? c.Linv2 : c.Linv; }
+  int*    permU(int k) const { return (k & 1) ? c.permU2 : c.permU; }
+  int     width(int k) const { return (N - k * nb) < nb ? (N - k * nb) : nb; }
+
+  // Panel k factored on stream s (its wide products use L11^{-1} buffer k as
+  // scratch), then its composed pivots and L11^{-1} built into buffers k.
+  int panel(int k, hipStream_t s) {
+    const int j = k * nb, jb = width(k);
+    double*   P = A + j + static_cast<size_t>(j) * lda;
+    c.panel_stream_  = s;
+    c.panel_scratch_ = Linv(k);
+    const hipError_t e = hplx::panel_factor(c, nbmin, nbdiv, P, lda, N - j,
+                                            jb, mat->ipiv + j, j);
+    c.panel_stream_  = nullptr;
+    c.panel_scratch_ = nullptr;
+    HIPCHK(c, e);
+    HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, permU(k)));
+    HIPCHK(c, hplx::launch_linv(s, jb, P, lda, Linv(k), nb));
+    return HPLX_SUCCESS;
+  }
+
+  // Columns [c0, c0 + n) right of panel k (c0 relative to the first column
+  // after the panel): row swaps into W, U12 = L11^{-1} W over A12, and (when
+  // the rows below exist and the streamed update takes the shape) U12^T
+  // into W.  Returns in *fast whether the streamed update applies.
+  int prepare(int k, int c0, int n, hipStream_t s, bool* fast) {
+    const int j = k * nb, jb = width(k), m = N - j;
+    double*   P   = A + j + static_cast<size_t>(j) * lda;
+    double*   A12 = A + j + static_cast<size_t>(j + jb + c0) * lda;
+    HIPCHK(c, hplx::launch_swap_gather(s, jb, n, A12, lda, permU(k), ws->W, nb));
+    HIPCHK(c, hplx::launch_dgemm(s, jb, n, jb, 1.0, Linv(k), nb, ws->W, nb,
+                                 0.0, A12, lda, true));
+    *fast = false;
+    if(m - jb > 0) {
+      const int ldu = hplx::ut_ld(n);
+      *fast = hplx::update_fast_ok(m - jb, n, jb, P + jb, lda, ws->W, ldu,
+                                   lda - j - jb, ldu);
+      if(*fast) HIPCHK(c, hplx::launch_transpose(s, jb, n, A12, lda, ws->W, ldu));
+    }
+    return HPLX_SUCCESS;
+  }
+
+  // A22 -= L21 U12 on those columns in one launch, timed by an event pair.
+  int gemm(int k, int c0, int n, hipStream_t s, bool fast) {
+    const int j = k * nb, jb = width(k), m = N - j;
+    if(m - jb <= 0) return HPLX_SUCCESS;
+    double* P   = A + j + static_cast<size_t>(j) * lda;
+    double* A12 = A + j + static_cast<size_t>(j + jb + c0) * lda;
+    hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
+    HIPCHK(c, hipEventRecord(e0, s));
+    if(fast)
+      HIPCHK(c, hplx::launch_update(s, m - jb, n, jb, P + jb, lda, ws->W,
+                                    hplx::ut_ld(n), A12 + jb, lda,
+                                    lda - j - jb, hplx::ut_ld(n)));
+    else
+      HIPCHK(c, hplx::launch_dgemm(s, m - jb, n, jb, -1.0, P + jb, lda, A12,
+                                   lda, 1.0, A12 + jb, lda, false, 1));
+    HIPCHK(c, hipEventRecord(e1, s));
+    c.ev_pairs++;
+    c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(n) * jb;
+    return HPLX_SUCCESS;
+  }
+};
+
+// Event from the context's pool (timing disabled), for cross-stream order.
+static int sync_event(Ctx& c, size_t i, hipEvent_t* e) {
+  while(c.ev_sync.size() <= i) {
+    hipEvent_t x;
+    HIPCHK(c, hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    c.ev_sync.push_back(x);
+  }
+  *e = c.ev_sync[i];
+  return HPLX_SUCCESS;
+}
+
+// Columns a look-ahead step needs right of the next panel before it
+// overlaps the rest of the update with that panel: fewer go serially.
+constexpr int kMinOverlapCols = 256;
+
 // The panel loop over the first `ncols` columns of mat->A (n+1 with b for
 // hplx_pdgesv, n for hplx_pdgetrf).  Returns 0, an error code, or the
 // 1-based column of the first exact zero pivot.
+//
+// Serial schedule (c.lookahead == 0): every step on c.stream in the order
+// panel k, swaps, U12, update of all trailing columns.
+//
+// Look-ahead schedule (depth 1, src/pgesv/HPL_pdgesv.cpp:168-245 with
+// nu0 = JB, src/panel/HPL_pdpanel_init.cpp:176-184): on the update stream U,
+// step k first updates the columns of panel k+1 (swaps, U12, GEMM), then
+// prepares the remaining columns (swaps, U12, U12^T) and queues their update
+// as a persistent launch that leaves the CUs of a vacate set R free; the
+// panel stream P factors panel k+1 on R meanwhile (after the preparation,
+// so that nothing else is queued beside its co-resident leaf grids), sets
+// the launch's stop flag and records an event; U then finishes the update's
+// remaining tiles on every CU.  Each matrix element sees the same
+// operations in the same order as in the serial schedule, so pivots and
+// results are bit-identical.  Steps whose next panel needs more CUs than a
+// vacate set holds, or whose remaining columns are few, run serially.
 static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
                        int ncols) {
   int r;
@@ -279,12 +471,10 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
   if(N > hplx::max_leaf_rows(c))
     return fail(c, HPLX_ERR_UNSUPPORTED, "N=%d exceeds the panel kernel's %d rows",
                 N, hplx::max_leaf_rows(c));
-  hplx::MatWs* ws = static_cast<hplx::MatWs*>(mat->ws);
-  double*      A  = mat->A;
-  hipStream_t  s  = c.stream;
-
+  Fp64Loop L{c, mat, static_cast<hplx::MatWs*>(mat->ws), mat->A, N, nb, lda,
+             palg->nbmin, palg->nbdiv};
   const int npanels = (N + nb - 1) / nb;
-  while(static_cast<int>(c.ev.size()) < 2 * npanels) {
+  while(static_cast<int>(c.ev.size()) < 4 * npanels) {
     hipEvent_t e;
     HIPCHK(c, hipEventCreate(&e));
     c.ev.push_back(e);
@@ -294,51 +484,112 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
   hplx::Progress prog(c, npanels, nb, N);
   HIPCHK(c, prog.init());
 
-  HIPCHK(c, hplx::reset_panel_state(c));
-  for(int j = 0; j < N; j += nb) {
-    const int jb = (N - j) < nb ? (N - j) : nb;
-    const int m  = N - j;
-    double*   P  = A + j + static_cast<size_t>(j) * lda;
-    HIPCHK(c, hplx::panel_factor(c, palg->nbmin, palg->nbdiv, P, lda, m, jb,
-                                 mat->ipiv + j, j));
-    const int nt = ncols - j - jb;
+  const bool  la = c.lookahead && c.cus_per_xcd > 0;
+  hipStream_t U  = la ? c.ustream : c.stream;
+  hipStream_t P  = c.pstream;
+  size_t      nev = 0;
+  hipEvent_t  ev;
+  if(la) {  // U and P start after the caller's stream
+    if((r = sync_event(c, nev++, &ev))) return r;
+    HIPCHK(c, hipEventRecord(ev, c.stream));
+    HIPCHK(c, hipStreamWaitEvent(U, ev, 0));
+    HIPCHK(c, hipStreamWaitEvent(P, ev, 0));
+  }
+  c.panel_stream_ = U;
+  const hipError_t er = hplx::reset_panel_state(c);
+  c.panel_stream_ = nullptr;
+  HIPCHK(c, er);
+  if((r = L.panel(0, U))) return r;
+  for(int k = 0; k < npanels; ++k) {
+    const int j = k * nb, jb = L.width(k);
+    const int nt = ncols - j - jb;  // trailing columns (with b)
+    const int jb1 = k + 1 < npanels ? L.width(k + 1) : 0;
+    bool      fast;
     if(nt > 0) {
-      double* A12 = A + j + static_cast<size_t>(j + jb) * lda;
-      HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
-      HIPCHK(c, hplx::launch_linv(s, jb, P, lda, c.Linv, nb));
-      HIPCHK(c, hplx::launch_swap_gather(s, jb, nt, A12, lda, c.permU, ws->W, nb));
-      HIPCHK(c, hplx::launch_dgemm(s, jb, nt, jb, 1.0, c.Linv, nb, ws->W, nb,
-                                   0.0, A12, lda, true));
-      if(m - jb > 0) {
-        hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
-        // U12^T into W (free again), then the streamed update; shapes the
-        // update kernel does not take (K not a multiple of 8) use the
-        // general GEMM
-        const int ldu    = hplx::ut_ld(nt);
-        const int a_rows = lda - j - jb;  // rows readable below P + jb
-        if(hplx::update_fast_ok(m - jb, nt, jb, P + jb, lda, ws->W, ldu,
-                                a_rows, ldu)) {
-          HIPCHK(c, hplx::launch_transpose(s, jb, nt, A12, lda, ws->W, ldu));
-          HIPCHK(c, hipEventRecord(e0, s));
-          HIPCHK(c, hplx::launch_update(s, m - jb, nt, jb, P + jb, lda, ws->W,
-                                        ldu, A12 + jb, lda, a_rows, ldu));
+      // columns of panel k+1 first (all of them if there is no next panel)
+      const int la_cols = jb1 > 0 ? jb1 : nt;
+      const int rest    = nt - la_cols;
+      if((r = L.prepare(k, 0, la_cols, U, &fast)) ||
+         (r = L.gemm(k, 0, la_cols, U, fast)))
+        return r;
+      // the CUs the next panel's leaves need: one per 512 * RPT rows, taken
+      // evenly from the 8 XCDs and, within each, from all 4 shader engines
+      // (a workgroup waits for a CU of the engine it was handed to, so an
+      // engine without a free CU would hold up the panel's kernels)
+      const int rows1 = N - j - jb;
+      const int G1    = (rows1 + hplx::kLeafThreads * hplx::kLeafRPT - 1) /
+                     (hplx::kLeafThreads * hplx::kLeafRPT);
+      const int rR    = (G1 + 31) / 32 * 4;
+      const bool overlap = la && jb1 > 0 && rest >= kMinOverlapCols &&
+                           rR <= c.cus_per_xcd && rR < 28;
+      if(overlap) {
+        if((r = L.prepare(k, la_cols, rest, U, &fast))) return r;
+        int* claim = c.claim + (k & 1) * hplx::kClaimInts;
+        int* stop  = c.stop + (k & 1) * 32;
+        HIPCHK(c, hplx::launch_zero_ints(U, claim, hplx::kClaimInts));
+        HIPCHK(c, hplx::launch_zero_ints(U, stop, 1));
+        hipEvent_t ev_prep, ev_pan;
+        if((r = sync_event(c, nev++, &ev_prep)) ||
+           (r = sync_event(c, nev++, &ev_pan)))
+          return r;
+        HIPCHK(c, hipEventRecord(ev_prep, U));
+        // panel k+1 on P, on the CUs the update leaves free
+        HIPCHK(c, hipStreamWaitEvent(P, ev_prep, 0));
+        if((r = L.panel(k + 1, P))) return r;
+        HIPCHK(c, hplx::launch_set_flag(P, stop));
+        HIPCHK(c, hipEventRecord(ev_pan, P));
+        const int m = N - j;
+        if(m - jb > 0) {
+          double* Pk  = L.A + j + static_cast<size_t>(j) * lda;
+          double* A12 = L.A + j + static_cast<size_t>(j + jb + la_cols) * lda;
+          const int ldu = hplx::ut_ld(rest);
+          hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
+          HIPCHK(c, hipEventRecord(e0, U));
+          if(fast) {
+            HIPCHK(c, hplx::launch_update_claimed(
+                          U, m - jb, rest, jb, Pk + jb, lda, L.ws->W, ldu,
+                          A12 + jb, lda, lda - j - jb, ldu, claim,
+                          c.vacate + 8 * (rR - 1), stop, 2 * c.num_cus));
+            HIPCHK(c, hipStreamWaitEvent(U, ev_pan, 0));
+            HIPCHK(c, hplx::launch_update_resume(
+                          U, m - jb, rest, jb, Pk + jb, lda, L.ws->W, ldu,
+                          A12 + jb, lda, lda - j - jb, ldu, claim));
+          } else {
+            HIPCHK(c, hipStreamWaitEvent(U, ev_pan, 0));
+            HIPCHK(c, hplx::launch_dgemm(U, m - jb, rest, jb, -1.0, Pk + jb,
+                                         lda, A12, lda, 1.0, A12 + jb, lda,
+                                         false, 1));
+          }
+          HIPCHK(c, hipEventRecord(e1, U));
+          c.ev_pairs++;
+          c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(rest) * jb;
         } else {
-          HIPCHK(c, hipEventRecord(e0, s));
-          HIPCHK(c, hplx::launch_dgemm(s, m - jb, nt, jb, -1.0, P + jb, lda,
-                                       A12, lda, 1.0, A12 + jb, lda, false, 1));
+          HIPCHK(c, hipStreamWaitEvent(U, ev_pan, 0));
         }
-        HIPCHK(c, hipEventRecord(e1, s));
-        c.ev_pairs++;
-        c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
+      } else {
+        if(jb1 > 0 && (r = L.panel(k + 1, U))) return r;
+        if(rest > 0 && ((r = L.prepare(k, la_cols, rest, U, &fast)) ||
+                        (r = L.gemm(k, la_cols, rest, U, fast))))
+          return r;
       }
+    } else if(jb1 > 0) {
+      if((r = L.panel(k + 1, U))) return r;
     }
-    HIPCHK(c, prog.panel_queued(j / nb));
+    HIPCHK(c, prog.panel_queued(k, U));
+  }
+  if(la) {  // the caller's stream continues after U and P
+    hipEvent_t eu, ep;
+    if((r = sync_event(c, nev++, &eu)) || (r = sync_event(c, nev++, &ep))) return r;
+    HIPCHK(c, hipEventRecord(eu, U));
+    HIPCHK(c, hipEventRecord(ep, P));
+    HIPCHK(c, hipStreamWaitEvent(c.stream, eu, 0));
+    HIPCHK(c, hipStreamWaitEvent(c.stream, ep, 0));
   }
   // zero pivot or barrier failure?  (one synchronisation)
   int flags[2] = {0, 0};
   HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
-                           s));
-  HIPCHK(c, hipStreamSynchronize(s));
+                           c.stream));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
   if(flags[1])
     return fail(c, HPLX_ERR_HIP,
                 "panel kernel wait timed out (workgroups not co-resident)");

@@ -42,6 +42,29 @@ struct Ctx {
   double*   slots      = nullptr;    // 2 x (kMaxGrid+1) x kSlotStride
   int*      flags      = nullptr;    // [0] first zero pivot (1-based), [1] error
   double*   dout       = nullptr;    // 16 doubles of small results
+  // second L11^{-1} / composed-pivot buffers: with the look-ahead, panel k+1
+  // is factored (and its L11^{-1} / permU built) while update k still reads
+  // those of panel k
+  double*   Linv2      = nullptr;
+  int*      permU2     = nullptr;
+  int*      claim      = nullptr;    // 2 sets of kClaimInts update-claim counters
+  int*      stop       = nullptr;    // 2 flags: claims of update k stop
+  unsigned long long* vacate = nullptr;  // kVacateSets x 8 words (see cu_sets)
+
+  // Where panel_factor queues its work and which kMaxNB^2 buffer its wide
+  // products use as scratch (null: stream / Linv).
+  hipStream_t panel_stream_ = nullptr;
+  double*     panel_scratch_ = nullptr;
+
+  // Look-ahead (hplx_set_option HPLX_OPT_LOOKAHEAD): internal streams of the
+  // update and the panel, and the CU sets a panel may be given: vacate set r
+  // holds the first r CUs of every XCD (CU ids from a census at context
+  // creation; r = 1 .. cus_per_xcd).
+  int                     lookahead   = 1;
+  hipStream_t             ustream     = nullptr;
+  hipStream_t             pstream     = nullptr;
+  int                     cus_per_xcd = 0;  // 0: census incomplete, no look-ahead
+  std::vector<hipEvent_t> ev_sync;         // timing-disabled, cross-stream order
 
   // progress lines from the panel loop (HPLX_PROGRESS=<panels>, the
   // counterpart of the reference's HPL_PROGRESS_REPORT build option)
@@ -52,6 +75,14 @@ struct Ctx {
   int                     ev_pairs = 0;
   double                  gemm_flops = 0.0;
 };
+
+inline hipStream_t panel_stream(const Ctx& c) {
+  return c.panel_stream_ ? c.panel_stream_ : c.stream;
+}
+inline double* panel_scratch(const Ctx& c) {
+  return c.panel_scratch_ ? c.panel_scratch_ : c.Linv;
+}
+constexpr int kVacateSets = 32;  // CUs per XCD on MI355X
 
 struct MatWs {
   double* W     = nullptr;  // max(nb*(n+1), check scratch) doubles
@@ -139,6 +170,31 @@ bool       update_fast_ok(int m, int n, int k, const double* A, int lda,
 hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,
                          int lda, const double* Bt, int ldb, double* C,
                          int ldc, int a_rows, int b_cols, bool panel = false);
+// The same update split between two launches for the look-ahead:
+// launch_update_claimed runs `grid` persistent workgroups (one per CU) that
+// take tiles through the per-XCD counters claim[0], claim[32], ...,
+// claim[224] (zeroed before the launch) until none are left or *stop is
+// non-zero; a workgroup on a CU whose __smid() bit is set in vacate (8 words
+// in device memory; null: none) exits at once, leaving that CU to the panel
+// factorisation.  launch_update_resume, queued after the claimed launch has
+// finished, updates every tile the claims did not reach.
+hipError_t launch_update_claimed(hipStream_t s, int m, int n, int k,
+                                 const double* A, int lda, const double* Bt,
+                                 int ldb, double* C, int ldc, int a_rows,
+                                 int b_cols, int* claim,
+                                 const unsigned long long* vacate,
+                                 const int* stop, int grid);
+hipError_t launch_update_resume(hipStream_t s, int m, int n, int k,
+                                const double* A, int lda, const double* Bt,
+                                int ldb, double* C, int ldc, int a_rows,
+                                int b_cols, const int* claimed);
+constexpr int kClaimInts = 8 * 32;  // ints of one set of claim counters
+hipError_t launch_zero_ints(hipStream_t s, int* p, int n);
+// *p = 1 with an agent-scope store (read by running kernels on any XCD).
+hipError_t launch_set_flag(hipStream_t s, int* p);
+// out[b] = __smid() of the CU that ran workgroup b (grid workgroups that
+// each stay ~10 us, so that they spread over the device).
+hipError_t launch_cu_census(hipStream_t s, int* out, int grid);
 // B (cols x rows, ldb) = A^T for A (rows x cols, lda), column-major both.
 hipError_t launch_transpose(hipStream_t s, int rows, int cols, const double* A,
                             int lda, double* B, int ldb);

@@ -234,7 +234,7 @@ hipError_t launch_leaf(Ctx& c, V* P, int lda, int m, int JB, int c0, int w,
   if(c.leaf_launches >= (1u << 26) - 1) {
     const int n = 2 * (kMaxGrid + 1) * kSlotStride;
     hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0,
-                       c.stream, c.slots, n);
+                       panel_stream(c), c.slots, n);
     hipError_t e = hipGetLastError();
     if(e != hipSuccess) return e;
     c.leaf_launches = 0;
@@ -263,7 +263,7 @@ hipError_t launch_leaf(Ctx& c, V* P, int lda, int m, int JB, int c0, int w,
   // the device.)
   void* args[] = {&la};
   return hipLaunchKernel(leaf_entry<V>(), dim3(G), dim3(kLeafThreads), args,
-                         0, c.stream);
+                         0, panel_stream(c));
 }
 
 // C (m x n) -= A (m x k) B (k x n) inside the panel recursion
@@ -274,22 +274,28 @@ hipError_t launch_leaf(Ctx& c, V* P, int lda, int m, int JB, int c0, int w,
 inline hipError_t panel_gemm(Ctx& c, int m, int n, int k, const double* A,
                              int lda, const double* B, int ldb, double* C,
                              int ldc) {
-  const int ldt = ut_ld(n);
+  const int   ldt = ut_ld(n);
+  double*     Bt  = panel_scratch(c);
+  hipStream_t s   = panel_stream(c);
+  // rows the update may read below A: m rounded up to even (its row pairs
+  // must keep their alignment).  Row m of A's k columns is readable: each
+  // of them is followed by a further column of the panel (B's first
+  // column), so the extra element lies inside the panel's storage.
+  const int a_rows = (m + 1) & ~1;
   if(n >= 128 && static_cast<size_t>(k) * ldt <= static_cast<size_t>(kMaxNB) * kMaxNB &&
-     update_fast_ok(m, n, k, A, lda, c.Linv, ldt, m, n)) {
-    hipError_t e = launch_transpose(c.stream, k, n, B, ldb, c.Linv, ldt);
+     update_fast_ok(m, n, k, A, lda, Bt, ldt, a_rows, ldt)) {
+    hipError_t e = launch_transpose(s, k, n, B, ldb, Bt, ldt);
     if(e != hipSuccess) return e;
-    return launch_update(c.stream, m, n, k, A, lda, c.Linv, ldt, C, ldc, m, n,
+    return launch_update(s, m, n, k, A, lda, Bt, ldt, C, ldc, a_rows, ldt,
                          true);
   }
-  return launch_dgemm(c.stream, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc,
-                      false);
+  return launch_dgemm(s, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc, false);
 }
 inline hipError_t panel_gemm(Ctx& c, int m, int n, int k, const float* A,
                              int lda, const float* B, int ldb, float* C,
                              int ldc) {
-  return launch_sgemm(c.stream, m, n, k, -1.0, A, lda, B, ldb, 1.0, C, ldc,
-                      false);
+  return launch_sgemm(panel_stream(c), m, n, k, -1.0, A, lda, B, ldb, 1.0, C,
+                      ldc, false);
 }
 
 // Columns [c0, c0+n) of the panel, rows [c0, m).
@@ -310,7 +316,7 @@ hipError_t leaf_or_chunks(Ctx& c, V* P, int lda, int m, int JB, int c0, int n,
     if(rest > 0) {
       V* L11 = P + top + static_cast<size_t>(top) * lda;
       V* A12 = P + top + static_cast<size_t>(top + w) * lda;
-      e = launch_trsm_colsweep(c.stream, w, rest, L11, lda, A12, lda);
+      e = launch_trsm_colsweep(panel_stream(c), w, rest, L11, lda, A12, lda);
       if(e != hipSuccess) return e;
       e = panel_gemm(c, m - top - w, rest, w, L11 + w, lda, A12, lda,
                      A12 + w, lda);
@@ -337,7 +343,7 @@ hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, V* P, int lda, int m, int JB,
     if(nrem > 0) {
       V* L11 = P + top + static_cast<size_t>(top) * lda;
       V* A12 = P + top + static_cast<size_t>(top + jb) * lda;
-      e = launch_trsm_colsweep(c.stream, jb, nrem, L11, lda, A12, lda);
+      e = launch_trsm_colsweep(panel_stream(c), jb, nrem, L11, lda, A12, lda);
       if(e != hipSuccess) return e;
       e = panel_gemm(c, m - top - jb, nrem, jb, L11 + jb, lda, A12, lda,
                      A12 + jb, lda);
@@ -376,7 +382,7 @@ hipError_t leaf_occupancy(int* blocks_per_cu) {
 }
 
 hipError_t reset_panel_state(Ctx& c) {
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(1), 0, c.stream, c.flags,
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(1), 0, panel_stream(c), c.flags,
                      c.flags + 1);
   return hipGetLastError();
 }

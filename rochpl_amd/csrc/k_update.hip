@@ -27,6 +27,15 @@
 //   - workgroup -> tile mapping XCD-aware (8 XCDs, groups of 8 tile rows);
 //   - interior tiles update C with up to 16 loads per lane in flight.
 //
+// Claimed mode (UpdSched.claim set; the look-ahead's split update): a fixed
+// grid of workgroups loops over tiles, taking them from one counter per XCD
+// (its share of the tile order above; an XCD whose share is exhausted takes
+// tiles of the others), so two launches can drain one update together.  The
+// next claim is issued before a tile's K loop and read after it.  A
+// workgroup that lands on a CU of UpdSched.vacate (a __smid() bitmap) leaves
+// at once: while such a launch holds the other CUs, those CUs stay free for
+// the panel factorisation queued beside it.
+//
 // The stage loads come from lds_stream.hpp; the kernel waits explicitly, no
 // compiler-issued vector memory access is in flight inside the K loop, and
 // all stage loads are drained before the epilogue.
@@ -43,30 +52,72 @@ constexpr int kUXcds = 8, kUGroupM = 8;
 
 typedef double ud4 __attribute__((ext_vector_type(4)));
 
+// Tiles of the claimed mode: counters claim[32 x] for XCD x, tile ranges
+// [x per, (x + 1) per) of the order below, handed out kClaimChunk at a time.
+constexpr int kClaimChunk = 1;
+struct UpdSched {
+  int*                      claim;   // per-XCD counters
+  int                       per;
+  const unsigned long long* vacate;  // 8 words: __smid() values whose CUs
+                                     // this launch leaves (null: none)
+  const int*                stop;    // claims stop once *stop != 0
+};
+
+__device__ inline bool on_vacated_cu(const UpdSched& s) {
+  if(!s.vacate) return false;
+  const unsigned id = __smid() & 511u;
+  return (s.vacate[id >> 6] >> (id & 63)) & 1ull;
+}
+
+// First claim on the first XCD share not yet found exhausted, starting with
+// this workgroup's XCD: returns (XCD << 24 | counter value), or -1.
+__device__ inline int claim_issue(int* claim, int x, unsigned dead) {
+  for(int d = 0; d < kUXcds; ++d) {
+    const int y = (x + d) & (kUXcds - 1);
+    if(!((dead >> y) & 1u))
+      return (y << 24) | atomicAdd(&claim[32 * y], kClaimChunk);
+  }
+  return -1;
+}
+
+// The tile a claim stands for, or further claims on the other shares until
+// one succeeds; -1 once every share is exhausted.
+__device__ inline int claim_resolve(int* claim, int x, int v, int per, int T,
+                                    unsigned& dead) {
+  while(v >= 0) {
+    const int y = v >> 24, k = v & 0xFFFFFF;
+    if(k < per && y * per + k < T) return y * per + k;
+    dead |= 1u << y;
+    v = claim_issue(claim, x, dead);
+  }
+  return -1;
+}
+
 #define HPLX_USLOT(n)                                             \
   __shared__ __attribute__((aligned(16))) double sA##n[UBK][UROW]; \
   __shared__ __attribute__((aligned(16))) double sB##n[UBK][UROW];
 
-// C (M x N) -= A (M x K, column-major) * Bt^T (Bt: K rows of N, row-major,
-// i.e. column-major N x K).  a_rows / b_cols: readable extent of A's columns
-// / Bt's rows from the given base (clamp bound for edge tiles).
-// kPanel marks the panel products (HPL_pdrpanrlT's GEMMs) so that kernel
-// traces tell them from the trailing update; the code is the same.
-template <bool kPanel>
-__global__ __launch_bounds__(512, 2) void update_kernel(
-    int M, int N, int K, const double* __restrict__ A, int lda,
+// One 128 x 128 tile of C (M x N) -= A (M x K, column-major) * Bt^T (Bt: K
+// rows of N, row-major, i.e. column-major N x K).  a_rows / b_cols: readable
+// extent of A's columns / Bt's rows from the given base (clamp bound for
+// edge tiles).
+__device__ __forceinline__ void update_tile(
+    int tile, int M, int N, int K, const double* __restrict__ A, int lda,
     const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
-    int tiles_m, int tiles_n, int a_rows, int b_cols) {
-  HPLX_USLOT(0) HPLX_USLOT(1) HPLX_USLOT(2) HPLX_USLOT(3)
-
-  const int T = tiles_m * tiles_n, per = gridDim.x / kUXcds, bid = blockIdx.x;
-  const int tile = (bid % kUXcds) * per + bid / kUXcds;
-  if(tile >= T) return;
+    int tiles_n, int tiles_m, int a_rows, int b_cols,
+    double (*sA0)[UROW], double (*sB0)[UROW], double (*sA1)[UROW],
+    double (*sB1)[UROW], double (*sA2)[UROW], double (*sB2)[UROW],
+    double (*sA3)[UROW], double (*sB3)[UROW]) {
   const int width = kUGroupM * tiles_n, group = tile / width;
   const int firstm = group * kUGroupM, gsz = min(tiles_m - firstm, kUGroupM);
   const int within = tile - group * width;
   const int m0 = (firstm + within % gsz) * 128, n0 = (within / gsz) * 128;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // the thread index passes through an empty volatile asm so that a caller's
+  // tile loop cannot hoist the per-lane addressing out of the loop and hold
+  // it in registers across the K loop
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  const int lane = t & 63, wave = t >> 6;
   const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 32;
   const int l16 = lane & 15, lk = lane >> 4;
 
@@ -179,6 +230,92 @@ __global__ __launch_bounds__(512, 2) void update_kernel(
       }
   }
 }
+
+// kPanel marks the panel products (HPL_pdrpanrlT's GEMMs) so that kernel
+// traces tell them from the trailing update; the code is the same.
+template <bool kPanel>
+__global__ __launch_bounds__(512, 2) void update_kernel(
+    int M, int N, int K, const double* __restrict__ A, int lda,
+    const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
+    int tiles_m, int tiles_n, int a_rows, int b_cols) {
+  HPLX_USLOT(0) HPLX_USLOT(1) HPLX_USLOT(2) HPLX_USLOT(3)
+  const int T = tiles_m * tiles_n, per = gridDim.x / kUXcds, bid = blockIdx.x;
+  const int tile = (bid % kUXcds) * per + bid / kUXcds;
+  if(tile >= T) return;
+  update_tile(tile, M, N, K, A, lda, Bt, ldb, C, ldc, tiles_n, tiles_m, a_rows,
+              b_cols, sA0, sB0, sA1, sB1, sA2, sB2, sA3, sB3);
+}
+
+// The rest of an update after a claimed launch has stopped: the same
+// one-tile-per-workgroup kernel, starting each XCD share where the claims
+// ended (claimed[32 x], read after the claimed launch has finished; grid
+// 8 x per, workgroups past their share's end exit).
+__global__ __launch_bounds__(512, 2) void update_resume_kernel(
+    int M, int N, int K, const double* __restrict__ A, int lda,
+    const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
+    int tiles_m, int tiles_n, int a_rows, int b_cols,
+    const int* __restrict__ claimed, int per) {
+  HPLX_USLOT(0) HPLX_USLOT(1) HPLX_USLOT(2) HPLX_USLOT(3)
+  const int x = blockIdx.x % kUXcds, i = blockIdx.x / kUXcds;
+  const int done = min(claimed[32 * x], per);
+  const int tile = x * per + done + i;
+  if(done + i >= per || tile >= tiles_m * tiles_n) return;
+  update_tile(tile, M, N, K, A, lda, Bt, ldb, C, ldc, tiles_n, tiles_m, a_rows,
+              b_cols, sA0, sB0, sA1, sB1, sA2, sB2, sA3, sB3);
+}
+
+// The claimed mode: a persistent grid whose workgroups take kClaimChunk
+// consecutive tiles per counter update (thread 0, after the epilogue of the
+// chunk's last tile) until the tiles run out or *stop becomes non-zero.  The
+// claim state lives in LDS.  The tile loop needs more than the 128 VGPRs of
+// update_kernel, so this kernel runs one workgroup per CU; it only covers
+// the part of an update that overlaps the panel factorisation, after which
+// update_resume_kernel finishes the update at full occupancy.
+struct UpdClaim {
+  int      tile, end;  // current chunk [tile, end), tile < 0: none left
+  unsigned dead;
+};
+
+__global__ __launch_bounds__(512, 2) void update_claimed_kernel(
+    int M, int N, int K, const double* __restrict__ A, int lda,
+    const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
+    int tiles_m, int tiles_n, int a_rows, int b_cols, UpdSched sch) {
+  HPLX_USLOT(0) HPLX_USLOT(1) HPLX_USLOT(2) HPLX_USLOT(3)
+  __shared__ UpdClaim sc;
+  // all waves of a workgroup share one CU, so this exit is uniform
+  if(on_vacated_cu(sch)) return;
+  const int T = tiles_m * tiles_n;
+  // thread 0: claim the next chunk into sc, or none once *stop is set
+  auto claim_next = [&](unsigned dead) {
+    int t0 = -1;
+    if(!__hip_atomic_load(sch.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      const int x = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) &
+                    (kUXcds - 1);  // XCC_ID[3:0]
+      t0 = claim_resolve(sch.claim, x, claim_issue(sch.claim, x, dead),
+                         sch.per, T, dead);
+    }
+    sc.tile = t0;
+    // a chunk ends at its XCD share's end and at the last tile
+    sc.end  = t0 < 0 ? t0 : min(min(t0 + kClaimChunk, T),
+                                (t0 / sch.per + 1) * sch.per);
+    sc.dead = dead;
+  };
+  if(threadIdx.x == 0) claim_next(0u);
+  __syncthreads();
+  for(;;) {
+    const int t0 = __builtin_amdgcn_readfirstlane(sc.tile);
+    if(t0 < 0) break;
+    const int end = __builtin_amdgcn_readfirstlane(sc.end);
+    for(int tile = t0; tile < end; ++tile) {
+      if(tile > t0) __syncthreads();  // the LDS ring is free again
+      update_tile(tile, M, N, K, A, lda, Bt, ldb, C, ldc, tiles_n, tiles_m,
+                  a_rows, b_cols, sA0, sB0, sA1, sB1, sA2, sB2, sA3, sB3);
+    }
+    __syncthreads();  // every wave is done with sc and the LDS ring
+    if(threadIdx.x == 0) claim_next(sc.dead);
+    __syncthreads();
+  }
+}
 #undef HPLX_USLOT
 
 // B (cols x rows, row-major with ldb) = A^T for A (rows x cols, column-major
@@ -210,28 +347,128 @@ bool update_fast_ok(int m, int n, int k, const double* A, int lda,
   const auto al16 = [](const void* p) {
     return reinterpret_cast<uintptr_t>(p) % 16 == 0;
   };
+  // the stage loads move row (column) pairs: an edge pair is clamped to
+  // start at a_rows - 2 (b_cols - 2), which keeps its place in the pair grid
+  // only when that extent is even
   return m > 0 && n > 0 && k > 0 && k % UBK == 0 && al16(A) && al16(Bt) &&
          lda % 2 == 0 && ldb % 2 == 0 && a_rows >= 2 && b_cols >= 2 &&
-         a_rows >= m && b_cols >= n;
+         a_rows % 2 == 0 && b_cols % 2 == 0 && a_rows >= m && b_cols >= n;
 }
+
+namespace {
+
+struct UpdGrid {
+  int  tiles_m, tiles_n;
+  long T;
+};
+
+hipError_t update_grid(int m, int n, int k, const double* A, int lda,
+                       const double* Bt, int ldb, int a_rows, int b_cols,
+                       UpdGrid* g) {
+  if(!update_fast_ok(m, n, k, A, lda, Bt, ldb, a_rows, b_cols))
+    return hipErrorInvalidValue;
+  g->tiles_m = (m + 127) / 128;
+  g->tiles_n = (n + 127) / 128;
+  g->T       = static_cast<long>(g->tiles_m) * g->tiles_n;
+  // the claimed mode's counters hold a share index below 2^24
+  if(g->T > 0x7fffffffL || (g->T + kUXcds - 1) / kUXcds >= (1L << 24))
+    return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
+// *p = 1, visible to workgroups on every XCD while they run.
+__global__ void set_flag_kernel(int* p) {
+  __hip_atomic_store(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = 0;
+}
+
+// Each workgroup records the __smid() of the CU it ran on, then waits about
+// `spin` clock ticks so that the grid spreads over every CU.
+__global__ void cu_census_kernel(int* out, long long spin) {
+  if(threadIdx.x == 0) out[blockIdx.x] = static_cast<int>(__smid() & 511u);
+  const long long t0 = clock64();
+  while(clock64() - t0 < spin) {
+  }
+}
+
+}  // namespace
 
 hipError_t launch_update(hipStream_t s, int m, int n, int k, const double* A,
                          int lda, const double* Bt, int ldb, double* C, int ldc,
                          int a_rows, int b_cols, bool panel) {
-  if(!update_fast_ok(m, n, k, A, lda, Bt, ldb, a_rows, b_cols))
-    return hipErrorInvalidValue;
-  const int  tiles_m = (m + 127) / 128, tiles_n = (n + 127) / 128;
-  const long T       = static_cast<long>(tiles_m) * tiles_n;
-  const long grid    = (T + kUXcds - 1) / kUXcds * kUXcds;
-  if(grid > 0x7fffffffL) return hipErrorInvalidValue;
+  UpdGrid    g;
+  hipError_t e = update_grid(m, n, k, A, lda, Bt, ldb, a_rows, b_cols, &g);
+  if(e != hipSuccess) return e;
+  const long grid = (g.T + kUXcds - 1) / kUXcds * kUXcds;
   if(panel)
     hipLaunchKernelGGL(update_kernel<true>, dim3(static_cast<unsigned>(grid)),
                        dim3(512), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
-                       tiles_m, tiles_n, a_rows, b_cols);
+                       g.tiles_m, g.tiles_n, a_rows, b_cols);
   else
     hipLaunchKernelGGL(update_kernel<false>, dim3(static_cast<unsigned>(grid)),
                        dim3(512), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
-                       tiles_m, tiles_n, a_rows, b_cols);
+                       g.tiles_m, g.tiles_n, a_rows, b_cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_claimed(hipStream_t s, int m, int n, int k,
+                                 const double* A, int lda, const double* Bt,
+                                 int ldb, double* C, int ldc, int a_rows,
+                                 int b_cols, int* claim,
+                                 const unsigned long long* vacate,
+                                 const int* stop, int grid) {
+  if(!claim || !stop || grid < 1) return hipErrorInvalidValue;
+  UpdGrid    g;
+  hipError_t e = update_grid(m, n, k, A, lda, Bt, ldb, a_rows, b_cols, &g);
+  if(e != hipSuccess) return e;
+  UpdSched sch{};
+  sch.claim  = claim;
+  sch.per    = static_cast<int>((g.T + kUXcds - 1) / kUXcds);
+  sch.vacate = vacate;
+  sch.stop   = stop;
+  hipLaunchKernelGGL(update_claimed_kernel, dim3(grid), dim3(512), 0, s, m, n,
+                     k, A, lda, Bt, ldb, C, ldc, g.tiles_m, g.tiles_n, a_rows,
+                     b_cols, sch);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_resume(hipStream_t s, int m, int n, int k,
+                                const double* A, int lda, const double* Bt,
+                                int ldb, double* C, int ldc, int a_rows,
+                                int b_cols, const int* claimed) {
+  if(!claimed) return hipErrorInvalidValue;
+  UpdGrid    g;
+  hipError_t e = update_grid(m, n, k, A, lda, Bt, ldb, a_rows, b_cols, &g);
+  if(e != hipSuccess) return e;
+  const long per  = (g.T + kUXcds - 1) / kUXcds;
+  const long grid = per * kUXcds;
+  hipLaunchKernelGGL(update_resume_kernel, dim3(static_cast<unsigned>(grid)),
+                     dim3(512), 0, s, m, n, k, A, lda, Bt, ldb, C, ldc,
+                     g.tiles_m, g.tiles_n, a_rows, b_cols, claimed,
+                     static_cast<int>(per));
+  return hipGetLastError();
+}
+
+hipError_t launch_set_flag(hipStream_t s, int* p) {
+  hipLaunchKernelGGL(set_flag_kernel, dim3(1), dim3(1), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(hipStream_t s, int* p, int n) {
+  if(n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                     p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_cu_census(hipStream_t s, int* out, int grid) {
+  if(grid < 1) return hipSuccess;
+  hipLaunchKernelGGL(cu_census_kernel, dim3(grid), dim3(64), 0, s, out,
+                     20000LL);
   return hipGetLastError();
 }
 

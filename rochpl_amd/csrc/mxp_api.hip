@@ -196,7 +196,7 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
         c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
       }
     }
-    HIPCHK(c, prog.panel_queued(j / nb));
+    HIPCHK(c, prog.panel_queued(j / nb, s));
   }
   int flags[2] = {0, 0};
   HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,

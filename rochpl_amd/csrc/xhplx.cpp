@@ -8,8 +8,11 @@
 // implement, are counted as skipped with a message, as the reference counts
 // tests it cannot run.
 //
-// Extra flag (not in the reference): --parse-only echoes the parameters and
-// exits without touching the GPU.
+// Extra flags (not in the reference): --parse-only echoes the parameters and
+// exits without touching the GPU; --dump DIR writes each solved test's x
+// (N doubles) and pivot vector (N int32, 0-based global rows) to
+// DIR/xhplx_N<N>_NB<NB>_NBMIN<nbmin>_NDIV<ndiv>_it<it>.{x,ipiv}, so that a
+// checker can recompute the residual and compare the pivots.
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -42,6 +45,7 @@ struct Params {
   int              align    = 8;
   double           frac     = 0.3;
   int              its      = 1;
+  std::string      dump_dir;  // --dump: where x and the pivots go
 };
 
 struct Counts {
@@ -276,6 +280,30 @@ double wall() {
   return tv.tv_sec + 1e-6 * tv.tv_usec;
 }
 
+// --dump: x and the pivot vector of one solve, raw little-endian arrays
+bool dump_solution(hplx_ctx* ctx, const Params& p, const hplx_pmat& mat, int N,
+                   int NB, int nbm, int ndv, int it) {
+  std::vector<double> x(N);
+  std::vector<int>    piv(N);
+  if(hplx_memcpy_d2h(ctx, x.data(), mat.X, sizeof(double) * N) != HPLX_SUCCESS ||
+     hplx_memcpy_d2h(ctx, piv.data(), mat.ipiv, sizeof(int) * N) != HPLX_SUCCESS)
+    return false;
+  char base[512];
+  std::snprintf(base, sizeof(base), "%s/xhplx_N%d_NB%d_NBMIN%d_NDIV%d_it%d",
+                p.dump_dir.c_str(), N, NB, nbm, ndv, it);
+  const std::string names[2] = {std::string(base) + ".x",
+                                std::string(base) + ".ipiv"};
+  const void*  data[2]  = {x.data(), piv.data()};
+  const size_t bytes[2] = {sizeof(double) * N, sizeof(int) * N};
+  for(int f = 0; f < 2; ++f) {
+    FILE* fp = std::fopen(names[f].c_str(), "wb");
+    if(!fp) return false;
+    const bool ok = std::fwrite(data[f], 1, bytes[f], fp) == bytes[f];
+    if(std::fclose(fp) != 0 || !ok) return false;
+  }
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // one test (src/HPL_pdtest.cpp:21-495)
 // ---------------------------------------------------------------------------
@@ -338,6 +366,8 @@ void run_test(hplx_ctx* ctx, const Params& p, int N, int NB, int pf, int nbm,
                  1, 1, t, gflops);
     std::fprintf(g_out, "HPL_pdgesv() start time %s\n", std::ctime(&ts));
     std::fprintf(g_out, "HPL_pdgesv() end time   %s\n", std::ctime(&te));
+    if(!p.dump_dir.empty() && !dump_solution(ctx, p, mat, N, NB, nbm, ndv, it))
+      warn("could not write the --dump files of N=%d NB=%d", N, NB);
     if(rc > 0) {
       warn("exact zero pivot at column %d", rc);
       cnt.kfail++;
@@ -387,7 +417,8 @@ void usage() {
       "-i  [ --input ]  arg (=HPL.dat)    input file; other options are then ignored\n"
       "-h  [ --help ]                     this message\n"
       "--version                          version\n"
-      "--parse-only                       echo the parameters and exit\n");
+      "--parse-only                       echo the parameters and exit\n"
+      "--dump DIR                         write each test's x and pivots to DIR\n");
 }
 
 }  // namespace
@@ -397,7 +428,7 @@ int main(int argc, char** argv) {
   int         P = 1, Q = 1, n = 45312, nb = 384, it = 1;
   double      frac = 0.3;
   bool        cmdline = false, inputfile = false, parse_only = false;
-  std::string infile = "HPL.dat";
+  std::string infile = "HPL.dat", dump_dir;
   for(int i = 1; i < argc; ++i) {
     const char* a   = argv[i];
     auto        val = [&](const char* flag) -> const char* {
@@ -415,6 +446,8 @@ int main(int argc, char** argv) {
       return 0;
     } else if(!std::strcmp(a, "--parse-only")) {
       parse_only = true;
+    } else if(!std::strcmp(a, "--dump")) {
+      dump_dir = val(a);
     } else if(!std::strcmp(a, "-P") || !std::strcmp(a, "--ranksP")) {
       P = std::atoi(val(a));
       cmdline = true;
@@ -464,8 +497,9 @@ int main(int argc, char** argv) {
   } else if(parse_hpldat(infile.c_str(), p)) {
     return 1;
   }
-  p.frac = frac;
-  p.its  = it > 0 ? it : 1;
+  p.frac     = frac;
+  p.its      = it > 0 ? it : 1;
+  p.dump_dir = dump_dir;
   if(p.devout == 6)
     g_out = stdout;
   else if(p.devout == 7)

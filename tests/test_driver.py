@@ -96,3 +96,20 @@ def test_help_and_version(tmp_path):
     assert h.returncode == 0 and "-NB" in h.stdout
     v = subprocess.run([XHPLX, "--version"], capture_output=True, text=True)
     assert v.returncode == 0 and "version" in v.stdout
+
+
+def test_echo_wraps_long_lists_like_the_reference(tmp_path):
+    """HPL_pdinfo's echo (src/HPL_pdinfo.cpp:1288-1310): "%8d " per value,
+    eight to a line, continuation lines indented by eight blanks."""
+    p = tmp_path / "HPL.dat"
+    lines = open(CONFIG1).read().splitlines()
+    ns = list(range(1000, 11000, 1000))        # 10 values
+    lines[4] = "10           # of problems sizes (N)"
+    lines[5] = " ".join(str(v) for v in ns) + "  Ns"
+    p.write_text("\n".join(lines) + "\n")
+    r = run(["-i", str(p)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout.splitlines()
+    i = out.index("N      :" + "".join("%8d " % v for v in ns[:8]))
+    assert out[i + 1] == "        " + "".join("%8d " % v for v in ns[8:])
+    assert "NB     :" + "%8d " % 256 in out

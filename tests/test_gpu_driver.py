@@ -1,16 +1,40 @@
-"""End-to-end run of the xhplx driver on the config-1 HPL.dat (N=4096,
-NB=256, P=Q=1): the reference's own acceptance check is the PASSED line."""
+"""End-to-end runs of the xhplx driver.
+
+config 1 (N=4096, NB=256, P=Q=1): the reference's own acceptance check, the
+PASSED line.  A multi-value HPL.dat (src/HPL_pddriver.cpp:141-220 loops over
+N x NB x PFACT x NBMIN and the grids): every test it runs is checked against
+the oracle (pivots equal to LAPACK dgetrf's, x equal to LAPACK's solution to
+the accuracy the residual implies, the printed residuals equal to the
+library's residual check of the dumped x and within rounding of the
+oracle's), every line it prints is re-formatted with
+the reference's printf formats and must come out byte for byte the same, and
+the summary counts the tests it cannot run (Crout panels, a 1x2 grid) as
+skipped."""
 import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
+import scipy.linalg as sl
+
+import rochpl_amd as rp
+from oracle import oracle as o
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 XHPLX = os.path.join(ROOT, "rochpl_amd", "xhplx")
 CONFIG1 = os.path.join(ROOT, "tests", "golden", "HPL_config1.dat")
+MULTI = os.path.join(ROOT, "tests", "golden", "HPL_multi.dat")
 
 pytestmark = pytest.mark.gpu
+
+# printf formats of the reference's result lines
+W_FMT = "W%c%1d%c%c%1d%c%1d%12d %5d %5d %5d %18.2f     %18.3e"  # HPL_pdtest.cpp:191-206
+RESID_FMT = "%s%16.7f%s%s"                                       # HPL_pdtest.cpp:445-450
+RESID_LABEL = "||Ax-b||_oo/(eps*(||A||_oo*||x||_oo+||b||_oo)*N)= "
+SUMMARY_FMT = "         %6d %s"                                   # HPL_pddriver.cpp:250-270
+W_RE = re.compile(r"^W([RC])(\d)([0-5])([LCR])(\d)([LCR])(\d+)\s+(\d+)\s+(\d+)"
+                  r"\s+(\d+)\s+(\d+)\s+(\S+)\s+(\S+)$")
 
 
 def test_xhplx_config1_passes(tmp_path):
@@ -26,3 +50,83 @@ def test_xhplx_config1_passes(tmp_path):
     m = re.search(r"\)\*N\)=\s+(\S+) \.\.\.\.\.\. PASSED", out)
     assert m and float(m.group(1)) < 16.0, out
     assert "1 tests completed and passed residual checks" in out
+
+
+def test_xhplx_multi_value_loops_match_the_oracle(ctx, tmp_path):
+    print("[gpu-case] xhplx -i HPL_multi.dat --dump", flush=True)
+    dump = tmp_path / "dump"
+    dump.mkdir()
+    r = subprocess.run([XHPLX, "-i", MULTI, "--dump", str(dump)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    # the tests xhplx runs, in the reference's loop order (grid, N, NB,
+    # PFACT, NBMIN, NDIV, RFACT, BCAST, DEPTH); Crout panels are skipped
+    expect = [(n, nb, nbmin) for n in (1000, 4096) for nb in (64, 256)
+              for nbmin in (32, 16)]
+    wl = [i for i, ln in enumerate(lines) if ln.startswith("W")]
+    assert len(wl) == len(expect), r.stdout
+    lu_cache = {}
+    for (n, nb, nbmin), i in zip(expect, wl):
+        line = lines[i]
+        m = W_RE.match(line)
+        assert m, line
+        g = m.groups()
+        vals = (g[0], int(g[1]), g[2], g[3], int(g[4]), g[5], int(g[6]),
+                int(g[7]), int(g[8]), int(g[9]), int(g[10]), float(g[11]),
+                float(g[12]))
+        assert W_FMT % vals == line, (W_FMT % vals, line)
+        assert vals[:11] == ("C", 1, "0", "R", 2, "R", nbmin, n, nb, 1, 1)
+        assert vals[11] > 0 and vals[12] > 0
+        # the residual line that follows the W-line and the time stamps
+        k = next(j for j in range(i, len(lines))
+                 if lines[j].startswith(RESID_LABEL))
+        assert lines[k - 1] == "-" * 80
+        resid = float(lines[k][len(RESID_LABEL):].split()[0])
+        assert RESID_FMT % (RESID_LABEL, resid, " ...... ", "PASSED") == lines[k]
+        r0 = re.match(r"^\(reference \|r_0\| form of the residual:\s+(\S+)\)$",
+                      lines[k + 1])
+        assert r0, lines[k + 1]
+        # against the oracle: the dumped x and pivots of this test
+        base = dump / ("xhplx_N%d_NB%d_NBMIN%d_NDIV2_it0" % (n, nb, nbmin))
+        x = np.fromfile(str(base) + ".x", dtype=np.float64)
+        ipiv = np.fromfile(str(base) + ".ipiv", dtype=np.int32)
+        assert x.shape == (n,) and ipiv.shape == (n,)
+        # the printed values are the library's check of this x ...
+        mat = ctx.pmat_alloc(n, nb)
+        try:
+            rp.lib().hplx_memcpy_h2d(ctx.h, rp.ctypes.c_void_p(mat.X),
+                                     x.ctypes.data_as(rp.ctypes.c_void_p),
+                                     x.nbytes)
+            dev = ctx.pdtest_check(mat)
+        finally:
+            ctx.pmat_free(mat)
+        assert "%16.7f" % dev["resid"] == "%16.7f" % resid
+        assert "%16.7f" % dev["resid_r0_form"] == "%16.7f" % \
+            float(r0.group(1))
+        # ... and the oracle's, up to the rounding of r = Ax - b itself: both
+        # sum N products per row in different orders, and the scaled
+        # residual is measured in units of that rounding (its evaluation
+        # error is of order 1/N; 2/sqrt(N) leaves a wide margin)
+        chk = o.residual(n, x)
+        assert chk["resid"] < 16.0
+        assert abs(resid - chk["resid"]) <= 2.0 / np.sqrt(n), \
+            (resid, chk["resid"])
+        if n not in lu_cache:
+            A = o.matgen(n, n)
+            lu_cache[n] = sl.lu_factor(A)
+        assert np.array_equal(ipiv, lu_cache[n][1]), (n, nb, nbmin)
+        # x solves the system to the accuracy the residual states
+        xr = sl.lu_solve(lu_cache[n], o.matgen(n, n + 1)[:, n])
+        assert np.max(np.abs(x - xr)) <= 1e-8 * np.max(np.abs(xr))
+    # summary: 2 grids x 2 N x 2 NB x 2 PFACT x 2 NBMIN = 32 tests; the 8
+    # P=Q=1 Right-looking ones ran, the Crout ones and the 1x2 grid skipped
+    text = r.stdout
+    assert "Finished %6d tests with the following results:" % 32 in text
+    assert SUMMARY_FMT % (8, "tests completed and passed residual checks,") \
+        in lines
+    assert SUMMARY_FMT % (0, "tests completed and failed residual checks,") \
+        in lines
+    assert SUMMARY_FMT % (24, "tests skipped because of illegal input "
+                              "values.") in lines
+    assert "End of Tests." in lines

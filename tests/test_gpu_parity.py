@@ -97,8 +97,11 @@ def test_dgemm_lower_product_via_pdupdate_shape(ctx):
 # ---------------------------------------------------------------------------
 # Panel factorisation: HPL_pdfact / HPL_pdrpanrlT / HPL_pdpanrlT
 # ---------------------------------------------------------------------------
+# (3001, 256): an odd panel height under a 128-wide panel product (the
+# streamed update reads row pairs; the last row of an odd count once came
+# from the pair before it)
 @pytest.mark.parametrize("m,jb,nbmin,ndiv", [
-    (3000, 256, 32, 2), (700, 96, 32, 2), (20000, 64, 32, 2),
+    (3000, 256, 32, 2), (3001, 256, 32, 2), (700, 96, 32, 2), (20000, 64, 32, 2),
     (1500, 100, 16, 3), (900, 128, 64, 2), (64, 64, 32, 2), (5, 3, 32, 2),
     (70000, 32, 32, 2)])
 def test_pdfact_matches_oracle(ctx, m, jb, nbmin, ndiv):
@@ -257,8 +260,8 @@ def test_pdtrsv_matches_oracle(ctx, n, nb):
 # ---------------------------------------------------------------------------
 # Whole solve: HPL_pdgesv + HPL_pdtest check
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("n,nb", [(4096, 256), (1000, 64), (257, 32),
-                                  (130, 7), (8, 4), (2000, 896)])
+@pytest.mark.parametrize("n,nb", [(4096, 256), (4097, 256), (1000, 64),
+                                  (257, 32), (130, 7), (8, 4), (2000, 896)])
 def test_pdgesv_matches_oracle(ctx, n, nb):
     say("pdgesv n=%d nb=%d" % (n, nb))
     mat = ctx.pmat_alloc(n, nb)
@@ -360,3 +363,35 @@ def test_pdgesv_repeat_is_deterministic(ctx):
     finally:
         ctx.pmat_free(mat)
     assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
+
+
+@pytest.mark.parametrize("n,nb", [(16384, 512), (12000, 384), (9001, 256)])
+def test_lookahead_is_bit_identical_to_serial(ctx, n, nb):
+    """Depth-1 look-ahead (panel k+1 factored on a second stream beside the
+    update of step k) against every step in order on one stream: the same
+    pivots, the same factored [A|b] and the same x, bit for bit; and both
+    pass the HPL residual check.  9001 = 35 x 256 + 41: a ragged last panel
+    whose K (41) the streamed update does not take."""
+    say("look-ahead vs serial n=%d nb=%d" % (n, nb))
+    res = {}
+    mat = ctx.pmat_alloc(n, nb)
+    try:
+        for la in (False, True):
+            ctx.lookahead = la
+            ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+            assert ctx.pdgesv(rp.palg_default(), mat) == 0
+            res[la] = (ctx.download_vec(mat.ipiv, n, np.int32),
+                       ctx.download_vec(mat.X, n),
+                       ctx.download_matrix(mat, rows=n, cols=n + 1),
+                       ctx.pdtest_check(mat)["resid"])
+    finally:
+        ctx.lookahead = True
+        ctx.pmat_free(mat)
+    (p0, x0, f0, r0), (p1, x1, f1, r1) = res[False], res[True]
+    assert np.array_equal(p0, p1)
+    assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64))
+    assert np.array_equal(np.asarray(f0).view(np.uint64),
+                          np.asarray(f1).view(np.uint64))
+    assert r0 < 16.0 and r1 < 16.0
+    print("[lookahead] n=%d nb=%d: identical; resid %.4f" % (n, nb, r1),
+          flush=True)
