@@ -276,6 +276,10 @@ struct UpdClaim {
   unsigned dead;
 };
 
+// The launch also reserves the rest of the CU's LDS (dynamic shared memory it
+// does not use), so that no other kernel's workgroup that uses LDS (every
+// kernel of the panel factorisation does) is placed beside it: those run on
+// the vacated CUs only, instead of competing with the update for a SIMD.
 __global__ __launch_bounds__(512, 2) void update_claimed_kernel(
     int M, int N, int K, const double* __restrict__ A, int lda,
     const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
@@ -430,9 +434,29 @@ hipError_t launch_update_claimed(hipStream_t s, int m, int n, int k,
   sch.per    = static_cast<int>((g.T + kUXcds - 1) / kUXcds);
   sch.vacate = vacate;
   sch.stop   = stop;
-  hipLaunchKernelGGL(update_claimed_kernel, dim3(grid), dim3(512), 0, s, m, n,
-                     k, A, lda, Bt, ldb, C, ldc, g.tiles_m, g.tiles_n, a_rows,
-                     b_cols, sch);
+  // LDS left on a CU beside one workgroup of this kernel, reserved by it
+  static int pad = -1;
+  if(pad < 0) {
+    int dev = 0, per_cu = 0;
+    hipFuncAttributes fa;
+    e = hipGetDevice(&dev);
+    if(e == hipSuccess)
+      e = hipDeviceGetAttribute(
+          &per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+    if(e == hipSuccess)
+      e = hipFuncGetAttributes(&fa,
+                               reinterpret_cast<const void*>(update_claimed_kernel));
+    if(e != hipSuccess) return e;
+    const int rest = per_cu - static_cast<int>(fa.sharedSizeBytes);
+    if(rest > 0)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(update_claimed_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, rest);
+    if(e != hipSuccess) return e;
+    pad = rest > 0 ? rest : 0;
+  }
+  hipLaunchKernelGGL(update_claimed_kernel, dim3(grid), dim3(512), pad, s, m,
+                     n, k, A, lda, Bt, ldb, C, ldc, g.tiles_m, g.tiles_n,
+                     a_rows, b_cols, sch);
   return hipGetLastError();
 }
 
