@@ -436,8 +436,9 @@ def _pmc_per_dispatch(path, counter, match):
 
 def update_probe(m, n, jb):
     """The FP64 trailing update (HPL_pdupdateTT: swaps, U12 = L11^-1 A12,
-    A22 -= L21 U12) alone on an m x (jb + n) block, three times: the shape
-    the HBM counters are read on (under rocprofv3 --pmc the whole solve ends
+    A22 -= L21 U12) alone on an m x (jb + n) block, three times, with the
+    row interchanges of a random panel: the shape the HBM counters are read
+    on (under rocprofv3 --pmc the whole solve ends
     with a host-side crash in the profiler right after the matrix
     allocation, at N = 193280 and at N = 163840)."""
     import numpy as np
@@ -446,9 +447,14 @@ def update_probe(m, n, jb):
     lda = m + 32
     dA = ctx.alloc(8 * lda * (jb + n))
     ctx.pdrandmat(m, jb + n, jb, dA.ptr, lda, rp.HPL_ISEED, 0.0)
-    perm = np.concatenate([np.arange(jb), -np.ones(jb)]).astype(np.int32)
+    # the interchanges of a panel of a random matrix: pivot i drawn from the
+    # rows at or below i (composed on the device, as in the solve)
+    rng = np.random.default_rng(7)
+    ipiv = np.array([rng.integers(i, m) for i in range(jb)], dtype=np.int32)
+    dpiv = ctx.alloc(4 * jb)
+    dpiv.upload(ipiv)
     dperm = ctx.alloc(8 * jb)
-    dperm.upload(perm)
+    ctx.pipid(jb, dpiv.ptr, dperm.ptr)
     dW = ctx.alloc(8 * jb * (n + 4))
     for _ in range(3):
         ctx.pdupdate(dA.ptr, lda, m, jb, n, dperm.ptr, dW.ptr)
