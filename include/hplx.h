@@ -174,6 +174,22 @@ int hplx_pdgetrf(hplx_ctx* ctx, const hplx_palg* palg, hplx_pmat* mat);
 int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
                  double* B, int ldb);
 
+/* ---- hplx_pdgetrs for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Needs no context: the same kernels as
+ * hplx_pdgetrs, queued on `stream` (a hipStream_t of the current device;
+ * NULL = its null stream) in the same order, so the result is bit-identical;
+ * nothing is allocated and nothing is synchronised (the caller waits on the
+ * stream).  Scratch: one file-scope device array of 8 MiB + 8 KiB per
+ * device, reserved when the library's code object loads and held until the
+ * process ends; every call on a device shares it, so calls that overlap on
+ * one device (on different streams, or from several threads) race.
+ * mat->nb above 1024 (the scratch's pivot slots) returns
+ * HPLX_ERR_UNSUPPORTED; bad arguments HPLX_ERR_INVALID; a launch failure
+ * HPLX_ERR_HIP (no error string: there is no context to hold one). */
+int hplx_pdgetrs_noalloc(const hplx_pmat* mat, int trans, int nrhs, double* B,
+                         int ldb, void* stream);
+
 /* ---- adjoint of X = A^{-1} B (new): with the factors of A in mat, the
  * forward solution X (n x nrhs, ldx) and the upstream gradient G (ldg):
  *   gradB = A^{-T} G          (n x nrhs, ldgb)

@@ -45,7 +45,7 @@ EXPORTS = [
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
     "hplx_pdtest_check", "hplx_pdfact", "hplx_pipid", "hplx_dlaswp00N",
     "hplx_pdupdate", "hplx_dgemm", "hplx_pdtrsv", "hplx_update_stats",
-    "hplx_pdgetrf", "hplx_pdgetrs", "hplx_pdgesv_adjoint",
+    "hplx_pdgetrf", "hplx_pdgetrs", "hplx_pdgetrs_noalloc", "hplx_pdgesv_adjoint",
     "hplx_mxp_opts_default", "hplx_mxp_alloc", "hplx_mxp_free",
     "hplx_mxp_matgen", "hplx_mxp_pdgetrf", "hplx_mxp_pdgetrs",
     "hplx_mxp_pdgesv", "hplx_mxp_residual", "hplx_mxp_update", "hplx_spdfact",
@@ -166,6 +166,7 @@ def lib():
         "hplx_update_stats": (i32, [vp, P(f64), P(i32), P(f64)]),
         "hplx_pdgetrf": (i32, [vp, P(PAlg), P(PMat)]),
         "hplx_pdgetrs": (i32, [vp, P(PMat), i32, i32, vp, i32]),
+        "hplx_pdgetrs_noalloc": (i32, [P(PMat), i32, i32, vp, i32, vp]),
         "hplx_pdgesv_adjoint": (i32, [vp, P(PMat), i32, vp, i32, vp, i32, vp,
                                       i32, vp, i32]),
         "hplx_mxp_opts_default": (None, [P(MxpOpts)]),

@@ -84,6 +84,91 @@ int cu_sets(Ctx& c, int* census_dev) {
 
 }  // namespace
 
+namespace hplx {
+
+// The solve with the factors of a pmat (hplx_pdgetrs), queued on s with the
+// given scratch: permU (2 kMaxNB ints) and the split-K partial products of
+// the transposed solve (work_doubles doubles; fewer only means fewer
+// splits).  No synchronisation, no allocation.
+hipError_t getrs_enqueue(hipStream_t s, const double* A, int N, int nb,
+                         int lda, const int* ipiv, int trans, int nrhs,
+                         double* B, int ldb, int* permU, double* work,
+                         size_t work_doubles) {
+#define HCHK(expr)                    \
+  do {                                \
+    const hipError_t e_ = (expr);     \
+    if(e_ != hipSuccess) return e_;   \
+  } while(0)
+  const int    np = (N + nb - 1) / nb;
+  // C (m x nrhs, ldb) -= Ablk (m x jb, lda) * Y (jb x nrhs, ldb) on FP64 MFMA
+  auto sub_product = [&](int m, int jb, const double* Ablk, const double* Y,
+                         double* Cblk) -> hipError_t {
+    return hplx::launch_nn_sub(s, jb, m, nrhs, Ablk, lda, Y, ldb, Cblk, ldb);
+  };
+  if(!trans) {
+    // L solve with the panel-by-panel interchanges, as b is carried through
+    // the factorisation (HPL_pdgesv), then U x = y (HPL_pdtrsv)
+    for(int p = 0; p < np; ++p) {
+      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb;
+      HCHK(hplx::launch_pipid(s, jb, ipiv + j, j, permU));
+      HCHK(hplx::launch_swap_inplace(s, jb, nrhs, B + j, ldb, permU));
+      HCHK(hplx::launch_tri_solve(s, true, false, true, jb, nrhs,
+                                  A + j + static_cast<size_t>(j) * lda,
+                                  lda, B + j, ldb));
+      if(N - j - jb > 0)
+        HCHK(sub_product(N - j - jb, jb,
+                              A + j + jb + static_cast<size_t>(j) * lda, B + j,
+                              B + j + jb));
+    }
+    for(int p = np - 1; p >= 0; --p) {
+      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb;
+      HCHK(hplx::launch_tri_solve(s, false, false, false, jb, nrhs,
+                                  A + j + static_cast<size_t>(j) * lda,
+                                  lda, B + j, ldb));
+      if(j > 0)
+        HCHK(sub_product(j, jb, A + static_cast<size_t>(j) * lda, B + j,
+                              B));
+    }
+  } else {
+    // A = P0^T L0 P1^T L1 ... U  =>  A^T z = g  is solved as
+    // w = U^{-T} g, then for panels from the last: z_p -= L21^T z_below,
+    // z_p = L11^{-T} z_p, z = P_p^T z.
+    for(int p = 0; p < np; ++p) {
+      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb, e = j + jb;
+      HCHK(hplx::launch_tri_solve(s, true, true, false, jb, nrhs,
+                                  A + j + static_cast<size_t>(j) * lda,
+                                  lda, B + j, ldb));
+      // z_below -= U12^T z_p (U12 = rows j..e of the columns right of the
+      // panel)
+      if(e < N)
+        HCHK(hplx::launch_tn_sub(s, jb, N - e, nrhs,
+                                 A + j + static_cast<size_t>(e) * lda,
+                                 lda, B + j, ldb, B + e, ldb, work,
+                                 work_doubles));
+    }
+    // z_p -= L21^T z_below reduces over all N - e rows below the panel for
+    // a jb x nrhs result: split over K, the partials in the work buffer,
+    // while they fit
+    for(int p = np - 1; p >= 0; --p) {
+      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb, e = j + jb;
+      if(e < N)
+        HCHK(hplx::launch_tn_sub(s, N - e, jb, nrhs,
+                                 A + e + static_cast<size_t>(j) * lda,
+                                 lda, B + e, ldb, B + j, ldb, work,
+                                 work_doubles));
+      HCHK(hplx::launch_tri_solve(s, false, true, true, jb, nrhs,
+                                  A + j + static_cast<size_t>(j) * lda,
+                                  lda, B + j, ldb));
+      HCHK(hplx::launch_pipid(s, jb, ipiv + j, j, permU));
+      HCHK(hplx::launch_swap_inverse(s, jb, nrhs, B + j, ldb, permU));
+    }
+  }
+#undef HCHK
+  return hipSuccess;
+}
+
+}  // namespace hplx
+
 extern "C" {
 
 void hplx_palg_default(hplx_palg* p) {
@@ -628,75 +713,10 @@ int hplx_pdgetrs(hplx_ctx* ctx, const hplx_pmat* mat, int trans, int nrhs,
      (trans != 0 && trans != 1))
     return fail(c, HPLX_ERR_INVALID, "pdgetrs: bad arguments");
   if(nrhs == 0) return HPLX_SUCCESS;
-  const int    N = mat->n, nb = mat->nb, lda = mat->ld;
-  const double* A = mat->A;
-  hipStream_t  s = c.stream;
-  const int    np = (N + nb - 1) / nb;
-  const size_t work = static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB;
-  // C (m x nrhs, ldb) -= Ablk (m x jb, lda) * Y (jb x nrhs, ldb) on FP64 MFMA
-  auto sub_product = [&](int m, int jb, const double* Ablk, const double* Y,
-                         double* Cblk) -> hipError_t {
-    return hplx::launch_nn_sub(s, jb, m, nrhs, Ablk, lda, Y, ldb, Cblk, ldb);
-  };
-  if(!trans) {
-    // L solve with the panel-by-panel interchanges, as b is carried through
-    // the factorisation (HPL_pdgesv), then U x = y (HPL_pdtrsv)
-    for(int p = 0; p < np; ++p) {
-      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb;
-      HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
-      HIPCHK(c, hplx::launch_swap_inplace(s, jb, nrhs, B + j, ldb, c.permU));
-      HIPCHK(c, hplx::launch_tri_solve(s, true, false, true, jb, nrhs,
-                                       A + j + static_cast<size_t>(j) * lda,
-                                       lda, B + j, ldb));
-      if(N - j - jb > 0)
-        HIPCHK(c, sub_product(N - j - jb, jb,
-                              A + j + jb + static_cast<size_t>(j) * lda, B + j,
-                              B + j + jb));
-    }
-    for(int p = np - 1; p >= 0; --p) {
-      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb;
-      HIPCHK(c, hplx::launch_tri_solve(s, false, false, false, jb, nrhs,
-                                       A + j + static_cast<size_t>(j) * lda,
-                                       lda, B + j, ldb));
-      if(j > 0)
-        HIPCHK(c, sub_product(j, jb, A + static_cast<size_t>(j) * lda, B + j,
-                              B));
-    }
-  } else {
-    // A = P0^T L0 P1^T L1 ... U  =>  A^T z = g  is solved as
-    // w = U^{-T} g, then for panels from the last: z_p -= L21^T z_below,
-    // z_p = L11^{-T} z_p, z = P_p^T z.
-    for(int p = 0; p < np; ++p) {
-      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb, e = j + jb;
-      HIPCHK(c, hplx::launch_tri_solve(s, true, true, false, jb, nrhs,
-                                       A + j + static_cast<size_t>(j) * lda,
-                                       lda, B + j, ldb));
-      // z_below -= U12^T z_p (U12 = rows j..e of the columns right of the
-      // panel)
-      if(e < N)
-        HIPCHK(c, hplx::launch_tn_sub(s, jb, N - e, nrhs,
-                                      A + j + static_cast<size_t>(e) * lda,
-                                      lda, B + j, ldb, B + e, ldb, c.Linv,
-                                      work));
-    }
-    // z_p -= L21^T z_below reduces over all N - e rows below the panel for
-    // a jb x nrhs result: split over K, the partials in the idle L11^{-1}
-    // buffer, while they fit
-    for(int p = np - 1; p >= 0; --p) {
-      const int j = p * nb, jb = (N - j) < nb ? (N - j) : nb, e = j + jb;
-      if(e < N)
-        HIPCHK(c, hplx::launch_tn_sub(s, N - e, jb, nrhs,
-                                      A + e + static_cast<size_t>(j) * lda,
-                                      lda, B + e, ldb, B + j, ldb, c.Linv,
-                                      work));
-      HIPCHK(c, hplx::launch_tri_solve(s, false, true, true, jb, nrhs,
-                                       A + j + static_cast<size_t>(j) * lda,
-                                       lda, B + j, ldb));
-      HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
-      HIPCHK(c, hplx::launch_swap_inverse(s, jb, nrhs, B + j, ldb, c.permU));
-    }
-  }
-  HIPCHK(c, hipStreamSynchronize(s));
+  HIPCHK(c, hplx::getrs_enqueue(c.stream, mat->A, mat->n, mat->nb, mat->ld,
+                                mat->ipiv, trans, nrhs, B, ldb, c.permU, c.Linv,
+                                static_cast<size_t>(hplx::kMaxNB) * hplx::kMaxNB));
+  HIPCHK(c, hipStreamSynchronize(c.stream));
   return HPLX_SUCCESS;
 }
 

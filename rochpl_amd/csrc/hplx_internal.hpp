@@ -224,6 +224,16 @@ hipError_t launch_outer_neg(hipStream_t s, int K, int m, int n,
 hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                        double* x);
 
+// The solve with the factors of hplx_pdgetrf / hplx_pdgesv (hplx_pdgetrs)
+// queued on s: permU is 2 kMaxNB ints of scratch, work holds the split-K
+// partial products of the transposed solve (work_doubles doubles; a smaller
+// buffer only means fewer splits).  Neither needs to start zeroed.  No
+// allocation and no synchronisation.
+hipError_t getrs_enqueue(hipStream_t s, const double* A, int N, int nb,
+                         int lda, const int* ipiv, int trans, int nrhs,
+                         double* B, int ldb, int* permU, double* work,
+                         size_t work_doubles);
+
 size_t     check_part_len(int n);  // doubles of scratch launch_check needs
 hipError_t launch_check(hipStream_t s, int n, int seed, double diag_shift,
                         const double* X, double* part, size_t part_len,
