@@ -209,7 +209,7 @@ def run_fp64(ctx, rp, args):
     log("fp64: allocating N=%d NB=%d (%.1f GiB)" % (
         n, nb, rp.hpl_ld(n) * (n + 1) * 8 / 2 ** 30))
     mat = ctx.pmat_alloc(n, nb)
-    times, checks, stats = [], [], []
+    times, checks, stats, kstats = [], [], [], []
     # Warm-up solves only where they are short: above N = 32768 one solve
     # takes tens of seconds and the code objects are loaded by the first
     # panel, so the requested warm-up steps are not run there.
@@ -235,6 +235,7 @@ def run_fp64(ctx, rp, args):
         if info != 0:
             raise SystemExit("pdgesv: exact zero pivot at column %d" % info)
         st = ctx.update_stats()
+        kinds = {k: ctx.update_timing(k) for k in (1, 2, 3)}
         chk = ctx.pdtest_check(mat)
         last = time.monotonic() - s0
         log("fp64 %s step %d: %.3f s  %.2f TFLOP/s  resid %.4f  trailing GEMM "
@@ -249,6 +250,7 @@ def run_fp64(ctx, rp, args):
             times.append(t)
             checks.append(chk)
             stats.append(st)
+            kstats.append(kinds)
         else:
             warm_done += 1
 
@@ -279,7 +281,26 @@ def run_fp64(ctx, rp, args):
     gemm_ms = sum(s[0] for s in stats)
     gemm_launch = sum(s[1] for s in stats)
     gemm_flops = sum(s[2] for s in stats)
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    # per kind of update launch (hplx_update_timing): 1 one launch per
+    # update (the look-ahead's next-panel columns; every step when serial),
+    # 2 the claimed part beside the panel, 3 the resume part on every CU.
+    # The roofline is that of the kernel with the most time.
+    names = {1: "update_kernel<false> (one launch per update)",
+             2: "update_claimed_kernel (beside the panel)",
+             3: "update_resume_kernel (rest of the update, every CU)"}
+    per_kind = {}
+    for k in (1, 2, 3):
+        ms = sum(ks[k][0] for ks in kstats)
+        cnt = sum(ks[k][1] for ks in kstats)
+        fl = sum(ks[k][2] for ks in kstats)
+        per_kind[k] = {"kernel": names[k], "launches": cnt,
+                       "ms": round(ms, 2),
+                       "avg_launch_ms": round(ms / cnt, 4) if cnt else None,
+                       "tflops": round(fl / (ms * 1e-3) / 1e12, 3)
+                       if ms > 0 else None, "flops": fl}
+    dom = max(per_kind, key=lambda k: per_kind[k]["ms"])
+    d = per_kind[dom]
+    achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
     return {
         "value": hpl_flops(n) / t_mean / 1e12, "t_mean": t_mean,
         "config": {"workload": "hpl_pdgesv_fp64_p1q1", "N": n, "NB": nb,
@@ -292,10 +313,17 @@ def run_fp64(ctx, rp, args):
                      "traffic": traffic,
                      "traffic_measured": traffic_probe,
                      "kernel": "trailing-update FP64 MFMA GEMM "
-                               "(A22 -= L21*U12), %d launches, %.1f ms"
-                               % (gemm_launch, gemm_ms),
-                     "avg_launch_ms": round(gemm_ms / max(gemm_launch, 1), 4),
-                     "flops_per_launch": gemm_flops / max(gemm_launch, 1)},
+                               "(A22 -= L21*U12): %s, %d launches, %.1f ms"
+                               % (d["kernel"], d["launches"], d["ms"]),
+                     "avg_launch_ms": d["avg_launch_ms"],
+                     "flops_per_launch": d["flops"] / max(d["launches"], 1),
+                     "all_update_launches": {
+                         "launches": gemm_launch, "ms": round(gemm_ms, 2),
+                         "tflops": round(gemm_flops / (gemm_ms * 1e-3) / 1e12,
+                                         3) if gemm_ms > 0 else None},
+                     "by_kernel": {str(k): {kk: vv for kk, vv in v.items()
+                                            if kk != "flops"}
+                                   for k, v in per_kind.items()}},
         "residual": round(max(c["resid"] for c in checks), 6),
         "residual_r0_form": round(max(c["resid_r0_form"] for c in checks), 6),
         "tflops_ref_formula": round(ref_flops(n) / t_mean / 1e12, 3),
@@ -461,6 +489,45 @@ def update_probe(m, n, jb):
     ctx.synchronize()
     ctx.close()
     print(json.dumps({"update_probe": {"m": m - jb, "n": n, "k": jb},
+                      "launches": 3}), flush=True)
+
+
+def mxp_probe(form, m, n, k):
+    """The low-precision trailing update alone (packing + GEMM, through
+    hplx_mxp_update), three times on an m x n x k shape: FP8 with FP16 C (the
+    FP8 form) or 3-term BF16 with FP32 C (the BF16 form); the shape the
+    kernels' counters are read on."""
+    import numpy as np
+    import rochpl_amd as rp
+    ctx = rp.Context(0)
+    fp8 = form == "fp8"
+    storage = rp.HPLX_STORE_FP16 if fp8 else rp.HPLX_STORE_FP32
+    csize = 2 if fp8 else 4
+    dL = ctx.alloc(4 * m * k)
+    dU = ctx.alloc(4 * k * n)
+    dC = ctx.alloc(csize * m * n)
+    rng = np.random.default_rng(3)
+    # operands of the size the factorisation produces (|L| <= 1, U of the
+    # scale of A's entries), generated block by block on the host
+    cdt = np.float16 if fp8 else np.float32
+    for dst, rows, cols, scale, dt in ((dL, m, k, 1.0, np.float32),
+                                       (dU, k, n, 0.5, np.float32),
+                                       (dC, m, n, 0.5, cdt)):
+        size = np.dtype(dt).itemsize
+        step = max(1, (64 << 20) // (size * rows))
+        for c0 in range(0, cols, step):
+            blk = rng.uniform(-scale, scale, (rows, min(step, cols - c0)))
+            dst.upload(np.asfortranarray(blk, dtype=dt),
+                       byte_offset=size * rows * c0)
+    for _ in range(3):
+        ctx.mxp_update(rp.HPLX_UPDATE_FP8 if fp8 else rp.HPLX_UPDATE_BF16,
+                       1 if fp8 else 3, storage, m, n, k, dL.ptr, m, dU.ptr, k,
+                       dC.ptr, m)
+    ctx.synchronize()
+    for d in (dL, dU, dC):
+        d.free()
+    ctx.close()
+    print(json.dumps({"mxp_probe": {"form": form, "m": m, "n": n, "k": k},
                       "launches": 3}), flush=True)
 
 
@@ -669,6 +736,10 @@ def main():
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only run a device copy of this size (counter "
                          "calibration) and exit")
+    ap.add_argument("--mxp-probe", nargs=4, default=None,
+                    metavar=("FORM", "M", "N", "K"),
+                    help="run the BF16 or FP8 trailing update alone (fp8|bf16 "
+                         "m n k), for counter collection")
     ap.add_argument("--update-probe", nargs=2, type=int, default=None,
                     metavar=("M", "N"),
                     help="only run the FP64 trailing update on an M x (NB+N) "
@@ -691,6 +762,10 @@ def main():
         return
     if args.update_probe:
         update_probe(args.update_probe[0], args.update_probe[1], args.nb)
+        return
+    if args.mxp_probe:
+        f, m, n, k = args.mxp_probe
+        mxp_probe(f, int(m), int(n), int(k))
         return
     if args.copy_probe_gib > 0:
         copy_probe(args.copy_probe_gib)

@@ -261,6 +261,15 @@ int hplx_pdtrsv(hplx_ctx* ctx, int n, int nb, const double* A, int lda,
 int hplx_update_stats(hplx_ctx* ctx, double* gemm_ms, int* launches,
                       double* gemm_flops);
 
+/* The same, restricted to one kind of launch: 1 a whole update in one launch
+ * (the look-ahead's columns of the next panel, and every update of the
+ * serial schedule), 2 the claimed part of a split update (beside the next
+ * panel's factorisation, on the CUs it leaves), 3 the resume part of a split
+ * update (on every CU); 0 all.  Flops of kinds 2 and 3 are those of the
+ * tiles each part took (from the per-XCD claim counts the solve logs). */
+int hplx_update_timing(hplx_ctx* ctx, int kind, double* ms, int* launches,
+                       double* flops);
+
 /* ======================================================================
  * Mixed precision (no counterpart in the reference, which is FP64 only).
  *

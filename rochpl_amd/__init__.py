@@ -45,6 +45,7 @@ EXPORTS = [
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
     "hplx_pdtest_check", "hplx_pdfact", "hplx_pipid", "hplx_dlaswp00N",
     "hplx_pdupdate", "hplx_dgemm", "hplx_pdtrsv", "hplx_update_stats",
+    "hplx_update_timing",
     "hplx_pdgetrf", "hplx_pdgetrs", "hplx_pdgetrs_noalloc", "hplx_pdgesv_adjoint",
     "hplx_mxp_opts_default", "hplx_mxp_alloc", "hplx_mxp_free",
     "hplx_mxp_matgen", "hplx_mxp_pdgetrf", "hplx_mxp_pdgetrs",
@@ -164,6 +165,7 @@ def lib():
                              vp, i32]),
         "hplx_pdtrsv": (i32, [vp, i32, i32, vp, i32, vp]),
         "hplx_update_stats": (i32, [vp, P(f64), P(i32), P(f64)]),
+        "hplx_update_timing": (i32, [vp, i32, P(f64), P(i32), P(f64)]),
         "hplx_pdgetrf": (i32, [vp, P(PAlg), P(PMat)]),
         "hplx_pdgetrs": (i32, [vp, P(PMat), i32, i32, vp, i32]),
         "hplx_pdgetrs_noalloc": (i32, [P(PMat), i32, i32, vp, i32, vp]),
@@ -396,6 +398,13 @@ class Context:
         ms, cnt, fl = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
         self._chk(lib().hplx_update_stats(self.h, ctypes.byref(ms),
                                           ctypes.byref(cnt), ctypes.byref(fl)))
+        return ms.value, cnt.value, fl.value
+
+    def update_timing(self, kind: int):
+        """(ms, launches, flops) of one kind of update launch (hplx.h)."""
+        ms, cnt, fl = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        self._chk(lib().hplx_update_timing(self.h, kind, ctypes.byref(ms),
+                                           ctypes.byref(cnt), ctypes.byref(fl)))
         return ms.value, cnt.value, fl.value
 
     # ---- mixed precision (FP32/FP16 factors, BF16/FP8 update, FP64 IR) --------

@@ -16,6 +16,7 @@
 // overlap; each matrix element sees the same sequence of operations.  b is
 // column n of the augmented matrix and is carried through 4-6, so after the
 // loop the back-solve U x = b' finishes the solve (HPL_pdtrsv).
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <chrono>
@@ -240,7 +241,8 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   size_t off_stop   = align_up(off_claim + 2 * hplx::kClaimInts * sizeof(int), 256);
   size_t off_vacate = off_stop + 256;
   size_t off_census = align_up(off_vacate + hplx::kVacateSets * 8 * 8, 256);
-  size_t total      = off_census + kCensusGrid * sizeof(int);
+  size_t off_clog   = align_up(off_census + kCensusGrid * sizeof(int), 256);
+  size_t total      = off_clog + 8 * hplx::kMaxClaimLog * sizeof(int);
   auto undo = [&](int code) {
     if(c.small_base) (void)hipFree(c.small_base);
     if(c.ustream) (void)hipStreamDestroy(c.ustream);
@@ -265,6 +267,7 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   c.stop    = reinterpret_cast<int*>(b + off_stop);
   c.vacate  = reinterpret_cast<unsigned long long*>(b + off_vacate);
   int* census = reinterpret_cast<int*>(b + off_census);
+  c.claim_log = reinterpret_cast<int*>(b + off_clog);
   if(hipMemset(c.small_base, 0, total) != hipSuccess ||
      hipDeviceSynchronize() != hipSuccess)
     return undo(HPLX_ERR_HIP);
@@ -505,8 +508,10 @@ struct Fp64Loop {
       HIPCHK(c, hplx::launch_dgemm(s, m - jb, n, jb, -1.0, P + jb, lda, A12,
                                    lda, 1.0, A12 + jb, lda, false, 1));
     HIPCHK(c, hipEventRecord(e1, s));
+    c.ev_kind[c.ev_pairs]  = 1;
+    c.ev_flops[c.ev_pairs] = 2.0 * (m - jb) * static_cast<double>(n) * jb;
+    c.gemm_flops += c.ev_flops[c.ev_pairs];
     c.ev_pairs++;
-    c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(n) * jb;
     return HPLX_SUCCESS;
   }
 };
@@ -559,13 +564,17 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
   Fp64Loop L{c, mat, static_cast<hplx::MatWs*>(mat->ws), mat->A, N, nb, lda,
              palg->nbmin, palg->nbdiv};
   const int npanels = (N + nb - 1) / nb;
-  while(static_cast<int>(c.ev.size()) < 4 * npanels) {
+  while(static_cast<int>(c.ev.size()) < 6 * npanels) {
     hipEvent_t e;
     HIPCHK(c, hipEventCreate(&e));
     c.ev.push_back(e);
   }
   c.ev_pairs   = 0;
   c.gemm_flops = 0.0;
+  c.ev_kind.assign(3 * npanels, 0);
+  c.ev_flops.assign(3 * npanels, 0.0);
+  c.splits.clear();
+  c.claim_host.clear();
   hplx::Progress prog(c, npanels, nb, N);
   HIPCHK(c, prog.init());
 
@@ -627,27 +636,45 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
         if(m - jb > 0) {
           double* Pk  = L.A + j + static_cast<size_t>(j) * lda;
           double* A12 = L.A + j + static_cast<size_t>(j + jb + la_cols) * lda;
-          const int ldu = hplx::ut_ld(rest);
-          hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
-          HIPCHK(c, hipEventRecord(e0, U));
+          const int    ldu   = hplx::ut_ld(rest);
+          const double flops = 2.0 * (m - jb) * static_cast<double>(rest) * jb;
+          auto pair = [&](int kind) {
+            c.ev_kind[c.ev_pairs] = kind;
+            return c.ev_pairs++;
+          };
           if(fast) {
+            // the claimed part beside the panel, then the resume part; each
+            // timed by its own event pair
+            const int pc = pair(2);
+            HIPCHK(c, hipEventRecord(c.ev[2 * pc], U));
             HIPCHK(c, hplx::launch_update_claimed(
                           U, m - jb, rest, jb, Pk + jb, lda, L.ws->W, ldu,
                           A12 + jb, lda, lda - j - jb, ldu, claim,
                           c.vacate + 8 * (rR - 1), stop, 2 * c.num_cus));
+            HIPCHK(c, hipEventRecord(c.ev[2 * pc + 1], U));
+            const int ns = static_cast<int>(c.splits.size());
+            if(ns < hplx::kMaxClaimLog)
+              HIPCHK(c, hplx::launch_claim_log(U, claim, m - jb, rest,
+                                               c.claim_log + 8 * ns));
             HIPCHK(c, hipStreamWaitEvent(U, ev_pan, 0));
+            const int pr = pair(3);
+            HIPCHK(c, hipEventRecord(c.ev[2 * pr], U));
             HIPCHK(c, hplx::launch_update_resume(
                           U, m - jb, rest, jb, Pk + jb, lda, L.ws->W, ldu,
                           A12 + jb, lda, lda - j - jb, ldu, claim));
+            HIPCHK(c, hipEventRecord(c.ev[2 * pr + 1], U));
+            c.splits.push_back({m - jb, rest, jb, pc, pr});
           } else {
             HIPCHK(c, hipStreamWaitEvent(U, ev_pan, 0));
+            const int p1 = pair(1);
+            HIPCHK(c, hipEventRecord(c.ev[2 * p1], U));
             HIPCHK(c, hplx::launch_dgemm(U, m - jb, rest, jb, -1.0, Pk + jb,
                                          lda, A12, lda, 1.0, A12 + jb, lda,
                                          false, 1));
+            HIPCHK(c, hipEventRecord(c.ev[2 * p1 + 1], U));
+            c.ev_flops[p1] = flops;
           }
-          HIPCHK(c, hipEventRecord(e1, U));
-          c.ev_pairs++;
-          c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(rest) * jb;
+          c.gemm_flops += flops;
         } else {
           HIPCHK(c, hipStreamWaitEvent(U, ev_pan, 0));
         }
@@ -670,10 +697,17 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
     HIPCHK(c, hipStreamWaitEvent(c.stream, eu, 0));
     HIPCHK(c, hipStreamWaitEvent(c.stream, ep, 0));
   }
-  // zero pivot or barrier failure?  (one synchronisation)
+  // zero pivot or barrier failure?  (one synchronisation; the claim log of
+  // the split updates comes back with it)
   int flags[2] = {0, 0};
   HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
                            c.stream));
+  const size_t nlog = std::min<size_t>(c.splits.size(), hplx::kMaxClaimLog);
+  c.claim_host.assign(8 * nlog, 0);
+  if(nlog)
+    HIPCHK(c, hipMemcpyAsync(c.claim_host.data(), c.claim_log,
+                             sizeof(int) * 8 * nlog, hipMemcpyDeviceToHost,
+                             c.stream));
   HIPCHK(c, hipStreamSynchronize(c.stream));
   if(flags[1])
     return fail(c, HPLX_ERR_HIP,
@@ -864,6 +898,43 @@ int hplx_pdtrsv(hplx_ctx* ctx, int n, int nb, const double* A, int lda,
                            hipMemcpyDeviceToDevice, c.stream));
   HIPCHK(c, hplx::launch_trsv(c.stream, n, nb, A, lda, x));
   HIPCHK(c, hipStreamSynchronize(c.stream));
+  return HPLX_SUCCESS;
+}
+
+int hplx_update_timing(hplx_ctx* ctx, int kind, double* ms, int* launches,
+                       double* flops) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  if(kind < 0 || kind > 3)
+    return fail(c, HPLX_ERR_INVALID, "update_timing: kind %d", kind);
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  // flops of the split updates' two parts, from the claim log (a split
+  // beyond the log counts wholly as resumed)
+  std::vector<double> fl(c.ev_flops.begin(), c.ev_flops.end());
+  for(size_t i = 0; i < c.splits.size(); ++i) {
+    const auto&  sp    = c.splits[i];
+    const double total = 2.0 * sp.m * static_cast<double>(sp.n) * sp.k;
+    const double took  = 8 * i < c.claim_host.size()
+                             ? hplx::claimed_flops(sp.m, sp.n, sp.k,
+                                                   &c.claim_host[8 * i])
+                             : 0.0;
+    fl[sp.pair_claimed] = took;
+    fl[sp.pair_resume]  = total - took;
+  }
+  double tot = 0.0, f = 0.0;
+  int    cnt = 0;
+  for(int i = 0; i < c.ev_pairs; ++i) {
+    if(kind && c.ev_kind[i] != kind) continue;
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c.ev[2 * i], c.ev[2 * i + 1]));
+    tot += t;
+    f += fl[i];
+    ++cnt;
+  }
+  if(ms) *ms = tot;
+  if(launches) *launches = cnt;
+  if(flops) *flops = f;
   return HPLX_SUCCESS;
 }
 

@@ -70,11 +70,25 @@ struct Ctx {
   // counterpart of the reference's HPL_PROGRESS_REPORT build option)
   int                     progress_every = 0;
   std::vector<hipEvent_t> ev_prog;
-  // timing of the trailing-update GEMM launches of the last solve
+  // timing of the trailing-update GEMM launches of the last solve: one
+  // event pair per launch, its kind (1: a whole update in one launch, 2: the
+  // claimed part of a split update, 3: its resume part) and its flops (for
+  // kinds 2 and 3 filled from the claim log when the stats are read)
   std::vector<hipEvent_t> ev;
   int                     ev_pairs = 0;
   double                  gemm_flops = 0.0;
+  std::vector<int>        ev_kind;
+  std::vector<double>     ev_flops;
+  // split updates of the last solve: shape and the pair indices; the tiles
+  // each XCD share had claimed (8 per split, copied back at the end)
+  struct Split {
+    int m, n, k, pair_claimed, pair_resume;
+  };
+  std::vector<Split> splits;
+  std::vector<int>   claim_host;
+  int*               claim_log = nullptr;  // device, 8 x kMaxClaimLog ints
 };
+constexpr int kMaxClaimLog = 8192;  // split updates whose claims are logged
 
 inline hipStream_t panel_stream(const Ctx& c) {
   return c.panel_stream_ ? c.panel_stream_ : c.stream;
@@ -192,6 +206,13 @@ constexpr int kClaimInts = 8 * 32;  // ints of one set of claim counters
 hipError_t launch_zero_ints(hipStream_t s, int* p, int n);
 // *p = 1 with an agent-scope store (read by running kernels on any XCD).
 hipError_t launch_set_flag(hipStream_t s, int* p);
+// out[x] = min(claim[32 x], per) for the 8 XCD shares of a claimed update of
+// m x n tiles (queued after the claimed launch).
+hipError_t launch_claim_log(hipStream_t s, const int* claim, int m, int n,
+                            int* out);
+// Algorithmic flops of the tiles that the claimed launch of an m x n x k
+// update took, given the 8 counts launch_claim_log recorded.
+double claimed_flops(int m, int n, int k, const int* done);
 // out[b] = __smid() of the CU that ran workgroup b (grid workgroups that
 // each stay ~10 us, so that they spread over the device).
 hipError_t launch_cu_census(hipStream_t s, int* out, int grid);

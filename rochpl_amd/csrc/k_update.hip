@@ -380,6 +380,11 @@ hipError_t update_grid(int m, int n, int k, const double* A, int lda,
   return hipSuccess;
 }
 
+__global__ void claim_log_kernel(const int* claim, int per, int* out) {
+  const int x = threadIdx.x;
+  if(x < kUXcds) out[x] = min(claim[32 * x], per);
+}
+
 // *p = 1, visible to workgroups on every XCD while they run.
 __global__ void set_flag_kernel(int* p) {
   __hip_atomic_store(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -475,6 +480,35 @@ hipError_t launch_update_resume(hipStream_t s, int m, int n, int k,
                      g.tiles_m, g.tiles_n, a_rows, b_cols, claimed,
                      static_cast<int>(per));
   return hipGetLastError();
+}
+
+hipError_t launch_claim_log(hipStream_t s, const int* claim, int m, int n,
+                            int* out) {
+  const long T   = static_cast<long>((m + 127) / 128) * ((n + 127) / 128);
+  const int  per = static_cast<int>((T + kUXcds - 1) / kUXcds);
+  hipLaunchKernelGGL(claim_log_kernel, dim3(1), dim3(64), 0, s, claim, per,
+                     out);
+  return hipGetLastError();
+}
+
+double claimed_flops(int m, int n, int k, const int* done) {
+  const int  tiles_m = (m + 127) / 128, tiles_n = (n + 127) / 128;
+  const long T       = static_cast<long>(tiles_m) * tiles_n;
+  const long per     = (T + kUXcds - 1) / kUXcds;
+  const long width   = static_cast<long>(kUGroupM) * tiles_n;
+  double     f       = 0.0;
+  for(int x = 0; x < kUXcds; ++x)
+    for(long t = x * per; t < x * per + done[x] && t < T; ++t) {
+      // the tile order of update_tile
+      const long group = t / width, firstm = group * kUGroupM;
+      const long gsz   = tiles_m - firstm < kUGroupM ? tiles_m - firstm : kUGroupM;
+      const long within = t - group * width;
+      const long tm = firstm + within % gsz, tn = within / gsz;
+      const long h = m - tm * 128 < 128 ? m - tm * 128 : 128;
+      const long w = n - tn * 128 < 128 ? n - tn * 128 : 128;
+      f += 2.0 * static_cast<double>(h * w) * k;
+    }
+  return f;
 }
 
 hipError_t launch_set_flag(hipStream_t s, int* p) {

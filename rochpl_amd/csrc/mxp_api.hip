@@ -126,6 +126,10 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
   }
   c.ev_pairs   = 0;
   c.gemm_flops = 0.0;
+  c.ev_kind.assign(npanels, 1);
+  c.ev_flops.assign(npanels, 0.0);
+  c.splits.clear();
+  c.claim_host.clear();
   hplx::Progress prog(c, npanels, nb, N);
   HIPCHK(c, prog.init());
 
@@ -192,8 +196,9 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
                                           jb, pk, C22, lda));
           HIPCHK(c, hipEventRecord(e1, s));
         }
+        c.ev_flops[c.ev_pairs] = 2.0 * (m - jb) * static_cast<double>(nt) * jb;
+        c.gemm_flops += c.ev_flops[c.ev_pairs];
         c.ev_pairs++;
-        c.gemm_flops += 2.0 * (m - jb) * static_cast<double>(nt) * jb;
       }
     }
     HIPCHK(c, prog.panel_queued(j / nb, s));
