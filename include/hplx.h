@@ -53,9 +53,12 @@ extern "C" {
 
 /* ---- algorithm parameters: mirrors HPL_T_palg (include/hpl_pgesv.hpp:63-81)
  * The function-pointer plug points pffun/rffun/upfun are selected by the
- * pfact/rfact/L1notran/Unotran values; only the reference's P=Q=1
- * configuration (pfact=rfact=HPLX_RIGHT_LOOKING, L1 and U transposed,
- * depth 1, long swap, equil 0) is implemented, anything else returns
+ * pfact/rfact/L1notran values (src/HPL_pddriver.cpp:164-200): every pfact
+ * and rfact (left-looking, Crout, right-looking) is built, and L1notran 0
+ * or 1 (it only picks how the reference stores its copy of the panel's top
+ * block; this library keeps no such copy, so both run the same kernels).
+ * As in the reference (src/HPL_pdinfo.cpp:925-1004), depth must be 1, the
+ * swap long, U transposed and equil 0: anything else returns
  * HPLX_ERR_UNSUPPORTED from hplx_pdgesv.
  * frac is accepted and not read: at P=1 the reference itself ignores it
  * (HPL_pdpanel_init sets nu0 = JB look-ahead columns, nu1 = 0 and
@@ -66,12 +69,12 @@ typedef struct hplx_palg {
   int    depth;    /* look-ahead depth (must be 1, src/HPL_pdinfo.cpp:930-935) */
   int    nbdiv;    /* recursive division factor (>= 2)            */
   int    nbmin;    /* recursion stopping criterium (>= 1)         */
-  int    pfact;    /* panel fact variant                          */
-  int    rfact;    /* recursive fact variant                      */
+  int    pfact;    /* panel fact variant (HPLX_LEFT_LOOKING ...)  */
+  int    rfact;    /* recursive fact variant (HPLX_LEFT_LOOKING ...) */
   int    fswap;    /* swapping algorithm                          */
   int    fsthr;    /* swapping threshold                          */
   int    equil;    /* equilibration (must be 0)                   */
-  int    L1notran; /* 0: L1 transposed (required)                 */
+  int    L1notran; /* L1 form, 0 or 1 (same kernels either way)   */
   int    Unotran;  /* 0: U transposed (required)                  */
   int    align;    /* memory alignment in doubles                 */
   double frac;     /* update split fraction (accepted, not read at P=1) */
@@ -219,6 +222,17 @@ int hplx_pdtest_check(hplx_ctx* ctx, const hplx_pmat* mat, int seed,
  * receives panel-relative pivot rows plus row_offset. */
 int hplx_pdfact(hplx_ctx* ctx, int nbmin, int nbdiv, double* P, int lda,
                 int m, int jb, int* ipiv, int row_offset);
+
+/* HPL_pdfact with the panel functions palg selects (src/HPL_pddriver.cpp:
+ * 164-200): rffun = HPL_pdrpan{ll,cr,rl}T by palg->rfact
+ * (src/pfact/HPL_pdrpanllT.cpp:116-159, HPL_pdrpancrT.cpp:116-175,
+ * HPL_pdrpanrlT.cpp:116-160), pffun = HPL_pdpan{ll,cr,rl}T by palg->pfact,
+ * nbmin and nbdiv from palg; otherwise as hplx_pdfact.  The three leaf
+ * variants give every element the same products in the same order, so one
+ * leaf kernel computes all three; with nbmin above 32 the 32-column leaves
+ * are blocked in the pfact variant's order. */
+int hplx_pdfact_palg(hplx_ctx* ctx, const hplx_palg* palg, double* P, int lda,
+                     int m, int jb, int* ipiv, int row_offset);
 
 /* HPL_pipid + HPL_pdpanel_swapids P=1 branch (src/pgesv/HPL_pipid.cpp:19-164,
  * src/panel/HPL_pdpanel_swapids.cpp:22-41), on the device: from jb

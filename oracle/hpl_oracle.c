@@ -13,7 +13,11 @@
  *   - panel:   src/pfact/HPL_pdrpanrlT.cpp:93-160 (recursive right-looking
  *              split), src/pfact/HPL_pdpanrlT_device.cpp:147-319 (leaf: max|a|
  *              pivot, full-panel-width row swap, scale by division, rank-1
- *              update of the leaf columns).
+ *              update of the leaf columns); the left-looking and Crout
+ *              variants: src/pfact/HPL_pdrpanllT.cpp:116-159 and
+ *              HPL_pdrpancrT.cpp:116-175 (recursion), HPL_pdpanllT_device.cpp:
+ *              7-376 and HPL_pdpancrT_device.cpp:7-354 (leaves), selected at
+ *              src/HPL_pddriver.cpp:164-200.
  *   - swaps:   src/pgesv/HPL_pipid.cpp:19-164 + src/panel/HPL_pdpanel_swapids.cpp:22-41
  *              (composed permutation), src/pauxil/HPL_dlaswp00N_device.cpp:22-62.
  *   - update:  src/pgesv/HPL_pdupdateTT.cpp:95-144 (dtrsm L,U,T,Unit then
@@ -31,7 +35,8 @@
  * (info = 1-based global column of the first one) and its column is not
  * scaled; the reference divides by zero (HPL_pdpanrlT_device.cpp:232,317).
  *
- * Pinning (see tests/test_oracle.py): pivots equal LAPACK dgetrf's
+ * Pinning (see tests/test_oracle.py, tests/test_oracle_variants.py): pivots
+ * of every (pfact, rfact) variant equal LAPACK dgetrf's
  * (scipy.linalg.lu_factor), x equals scipy.linalg.lu_solve to rounding, the
  * matgen equals the exact-rational closed form and the O(log k) jump equals
  * sequential stepping, pipid equals brute-force application of the
@@ -199,76 +204,155 @@ static void swap_rows(double* P, int lda, int ncols, int r0, int r1) {
 /* panel factorisation                                                 */
 /* ------------------------------------------------------------------ */
 
-/* Leaf (HPL_pdpanrlT_device.cpp:147-319): columns [c0, c0+w) of the panel P
- * (m rows, JB columns, row 0 = panel top).  Row interchanges span the full
- * panel width (the candidate rows are JB wide, :173-184).  ipiv is panel
- * relative.  Returns the 1-based panel column of the first exact zero
- * pivot, or 0. */
-static int leaf(double* P, int lda, int m, int JB, int c0, int w, int* ipiv) {
+/* Variant codes of HPL_T_FACT (include/hpl_pgesv.hpp). */
+#define HPLO_LEFT 301
+#define HPLO_CROUT 302
+#define HPLO_RIGHT 303
+
+/* Pivot step of column c: max |a| over rows c..m-1 (first index on ties),
+ * interchange across the full panel width (the candidate rows are JB wide,
+ * HPL_pdpanrlT_device.cpp:173-184), division of the column below the pivot
+ * (:232).  Returns 0, or 1 when the pivot is an exact zero (the column is
+ * then left unscaled). */
+static int pivot_step(double* P, int lda, int m, int JB, int c, int* ipiv) {
+  int    p    = c;
+  double amax = fabs(AT(P, lda, c, c));
+  for(int r = c + 1; r < m; ++r) {
+    const double v = fabs(AT(P, lda, r, c));
+    if(v > amax) {
+      amax = v;
+      p    = r;
+    }
+  }
+  ipiv[c] = p;
+  swap_rows(P, lda, JB, c, p);
+  const double piv = AT(P, lda, c, c);
+  if(piv == 0.0) return 1;
+  for(int r = c + 1; r < m; ++r) AT(P, lda, r, c) /= piv;
+  return 0;
+}
+
+/* Leaf on columns [c0, c0+w) of the panel P (m rows, JB columns, row 0 =
+ * panel top); ipiv is panel relative.  Returns the 1-based panel column of
+ * the first exact zero pivot, or 0.
+ *   right (HPL_pdpanrlT_device.cpp:147-319): pivot, then a rank-1 update of
+ *     the leaf columns right of the pivot column;
+ *   left (HPL_pdpanllT_device.cpp:115-125,294-345): column c first receives every
+ *     earlier leaf column's update (its rows above c become U by forward
+ *     substitution, its rows from c on are reduced), then the pivot step;
+ *   Crout (HPL_pdpancrT_device.cpp:131-140,311-325): column c's rows from c on receive the
+ *     earlier columns' updates with the final U rows, the pivot step, then
+ *     row c of U right of the pivot is finished (a_cj -= l_ck u_kj).
+ * Every element receives the same products in the same order k = c0, c0+1,
+ * ... in all three, so with these sequential loops the variants agree bit
+ * for bit; only the moment of each update differs. */
+static int leaf_v(int variant, double* P, int lda, int m, int JB, int c0,
+                  int w, int* ipiv) {
   int info = 0;
-  for(int jj = 0; jj < w; ++jj) {
-    const int c = c0 + jj;
-    /* max |a| over rows c..m-1, first index on ties */
-    int    p    = c;
-    double amax = fabs(AT(P, lda, c, c));
-    for(int r = c + 1; r < m; ++r) {
-      const double v = fabs(AT(P, lda, r, c));
-      if(v > amax) {
-        amax = v;
-        p    = r;
+  const int cend = c0 + w;
+  for(int c = c0; c < cend; ++c) {
+    if(variant == HPLO_LEFT) {
+      for(int k = c0; k < c; ++k) {
+        const double u = AT(P, lda, k, c);
+        if(u == 0.0) continue;
+        for(int r = k + 1; r < m; ++r) AT(P, lda, r, c) -= AT(P, lda, r, k) * u;
+      }
+    } else if(variant == HPLO_CROUT) {
+      for(int k = c0; k < c; ++k) {
+        const double u = AT(P, lda, k, c);
+        if(u == 0.0) continue;
+        for(int r = c; r < m; ++r) AT(P, lda, r, c) -= AT(P, lda, r, k) * u;
       }
     }
-    ipiv[c] = p;
-    swap_rows(P, lda, JB, c, p);
-    const double piv = AT(P, lda, c, c);
-    if(piv == 0.0) {
+    if(pivot_step(P, lda, m, JB, c, ipiv)) {
       if(!info) info = c + 1;
       continue;
     }
-    for(int r = c + 1; r < m; ++r) AT(P, lda, r, c) /= piv;
-    /* rank-1 update of the remaining leaf columns */
-    for(int k = c + 1; k < c0 + w; ++k) {
-      const double u = AT(P, lda, c, k);
-      if(u == 0.0) continue;
-      for(int r = c + 1; r < m; ++r) AT(P, lda, r, k) -= AT(P, lda, r, c) * u;
+    if(variant == HPLO_RIGHT) {
+      for(int k = c + 1; k < cend; ++k) {
+        const double u = AT(P, lda, c, k);
+        if(u == 0.0) continue;
+        for(int r = c + 1; r < m; ++r) AT(P, lda, r, k) -= AT(P, lda, r, c) * u;
+      }
+    } else if(variant == HPLO_CROUT) {
+      for(int j = c + 1; j < cend; ++j)
+        for(int k = c0; k < c; ++k) {
+          const double u = AT(P, lda, k, j);
+          if(u == 0.0) continue;
+          AT(P, lda, c, j) -= AT(P, lda, c, k) * u;
+        }
     }
   }
   return info;
 }
 
-/* Recursive right-looking split (HPL_pdrpanrlT.cpp:93-160).  Works on the
- * panel columns [c0, c0+n) with rows [c0, m). */
-static int rpanel(double* P, int lda, int m, int JB, int c0, int n, int nbmin,
-                  int ndiv, int* ipiv) {
-  if(n <= nbmin) return leaf(P, lda, m, JB, c0, n, ipiv);
+/* Recursive split of the panel columns [c0, c0+n), rows [c0, m), in chunks
+ * of nb = ceil(ceil(n / nbmin) / ndiv) * nbmin columns (HPL_pdrpanrlT.cpp:93),
+ * each chunk factored by a recursive call (the leaf once n <= nbmin):
+ *   right (HPL_pdrpanrlT.cpp:116-160): factor the chunk, then its U rows to
+ *     the right (L11^{-1} A12) and the rows below (A22 -= L21 U12);
+ *   left (HPL_pdrpanllT.cpp:116-159): first the chunk's rows above it
+ *     (U = L^{-1} A over the jj columns left of it in this call) and the rows
+ *     from its top down (A -= L U), then factor it;
+ *   Crout (HPL_pdrpancrT.cpp:116-175): the chunk's rows from its top down
+ *     (A -= L U with the final U rows above), factor it, then its own U rows
+ *     right of it (A -= L U over the jj columns, then L11^{-1}). */
+static int rpanel_v(int pfact, int rfact, double* P, int lda, int m, int JB,
+                    int c0, int n, int nbmin, int ndiv, int* ipiv) {
+  if(n <= nbmin) return leaf_v(pfact, P, lda, m, JB, c0, n, ipiv);
   int       info = 0;
   const int nb   = (((n + nbmin - 1) / nbmin + ndiv - 1) / ndiv) * nbmin;
   int       jj = 0, jb = nb, nrem = n;
   do {
     nrem -= jb;
-    const int i1 = rpanel(P, lda, m, JB, c0 + jj, jb, nbmin, ndiv, ipiv);
+    const int top = c0 + jj; /* first row and column of this chunk */
+    if(jj > 0 && rfact == HPLO_LEFT)
+      trsm_llnu(jj, jb, &AT(P, lda, c0, c0), lda, &AT(P, lda, c0, top), lda);
+    if(jj > 0 && rfact != HPLO_RIGHT)
+      gemm_minus(m - top, jb, jj, &AT(P, lda, top, c0), lda,
+                 &AT(P, lda, c0, top), lda, &AT(P, lda, top, top), lda);
+    const int i1 =
+        rpanel_v(pfact, rfact, P, lda, m, JB, top, jb, nbmin, ndiv, ipiv);
     if(i1 && !info) info = i1;
-    const int top = c0 + jj; /* first row and column of this block */
-    /* U12 = L11^{-1} A12 on the block's top jb rows */
-    trsm_llnu(jb, nrem, &AT(P, lda, top, top), lda, &AT(P, lda, top, top + jb),
-              lda);
-    /* A22 -= L21 * U12 on the rows below */
-    gemm_minus(m - top - jb, nrem, jb, &AT(P, lda, top + jb, top), lda,
-               &AT(P, lda, top, top + jb), lda, &AT(P, lda, top + jb, top + jb),
-               lda);
+    if(nrem > 0 && rfact == HPLO_RIGHT) {
+      trsm_llnu(jb, nrem, &AT(P, lda, top, top), lda,
+                &AT(P, lda, top, top + jb), lda);
+      gemm_minus(m - top - jb, nrem, jb, &AT(P, lda, top + jb, top), lda,
+                 &AT(P, lda, top, top + jb), lda,
+                 &AT(P, lda, top + jb, top + jb), lda);
+    } else if(nrem > 0 && rfact == HPLO_CROUT) {
+      gemm_minus(jb, nrem, jj, &AT(P, lda, top, c0), lda,
+                 &AT(P, lda, c0, top + jb), lda, &AT(P, lda, top, top + jb),
+                 lda);
+      trsm_llnu(jb, nrem, &AT(P, lda, top, top), lda,
+                &AT(P, lda, top, top + jb), lda);
+    }
     jj += jb;
     jb = nrem < nb ? nrem : nb;
   } while(nrem > 0);
   return info;
 }
 
-/* HPL_pdfact at P=Q=1: factor the m x jb panel P in place.  ipiv[0:jb] gets
- * panel-relative pivot rows. */
-int hplo_pdfact(double* P, int lda, int m, int jb, int nbmin, int ndiv,
-                int* ipiv) {
+static int valid_fact(int f) {
+  return f == HPLO_LEFT || f == HPLO_CROUT || f == HPLO_RIGHT;
+}
+
+/* HPL_pdfact at P=Q=1 with the panel (pfact) and recursion (rfact)
+ * variants: factor the m x jb panel P in place; ipiv[0:jb] gets
+ * panel-relative pivot rows.  Returns info, or -1 for an unknown variant. */
+int hplo_pdfact_v(double* P, int lda, int m, int jb, int nbmin, int ndiv,
+                  int pfact, int rfact, int* ipiv) {
+  if(!valid_fact(pfact) || !valid_fact(rfact)) return -1;
   if(nbmin < 1) nbmin = 1;
   if(ndiv < 2) ndiv = 2;
-  return rpanel(P, lda, m, jb, 0, jb, nbmin, ndiv, ipiv);
+  return rpanel_v(pfact, rfact, P, lda, m, jb, 0, jb, nbmin, ndiv, ipiv);
+}
+
+/* The right-looking panel and recursion (the reference's default). */
+int hplo_pdfact(double* P, int lda, int m, int jb, int nbmin, int ndiv,
+                int* ipiv) {
+  return hplo_pdfact_v(P, lda, m, jb, nbmin, ndiv, HPLO_RIGHT, HPLO_RIGHT,
+                       ipiv);
 }
 
 /* HPL_pipid + HPL_pdpanel_swapids (P=1 branch): from jb sequential
@@ -358,15 +442,17 @@ void hplo_trsv_upper(int n, const double* U, int ldu, double* x) {
  * ipiv[0:n] gets global 0-based pivot rows (row i <-> ipiv[i]); x[0:n] gets
  * the solution.  Returns info (0, or 1-based column of the first exact zero
  * pivot). */
-int hplo_pdgesv(int n, int nb, int nbmin, int ndiv, double* A, int lda,
-                int* ipiv, double* x) {
+int hplo_pdgesv_v(int n, int nb, int nbmin, int ndiv, int pfact, int rfact,
+                  double* A, int lda, int* ipiv, double* x) {
+  if(!valid_fact(pfact) || !valid_fact(rfact)) return -1;
   int info = 0;
   int* permU = (int*)malloc(sizeof(int) * (size_t)(2 * (nb > 0 ? nb : 1)));
   for(int j = 0; j < n; j += nb) {
     const int jb = (n - j) < nb ? (n - j) : nb;
     const int m  = n - j;
     double*   P  = &AT(A, lda, j, j);
-    const int i1 = hplo_pdfact(P, lda, m, jb, nbmin, ndiv, ipiv + j);
+    const int i1 =
+        hplo_pdfact_v(P, lda, m, jb, nbmin, ndiv, pfact, rfact, ipiv + j);
     if(i1 && !info) info = j + i1;
     /* trailing columns j+jb .. n (b included) */
     const int nt = n + 1 - (j + jb);
@@ -381,6 +467,12 @@ int hplo_pdgesv(int n, int nb, int nbmin, int ndiv, double* A, int lda,
   for(int i = 0; i < n; ++i) x[i] = AT(A, lda, i, n);
   if(!info) hplo_trsv_upper(n, A, lda, x);
   return info;
+}
+
+int hplo_pdgesv(int n, int nb, int nbmin, int ndiv, double* A, int lda,
+                int* ipiv, double* x) {
+  return hplo_pdgesv_v(n, nb, nbmin, ndiv, HPLO_RIGHT, HPLO_RIGHT, A, lda,
+                       ipiv, x);
 }
 
 /* ------------------------------------------------------------------ */

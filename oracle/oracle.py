@@ -50,12 +50,17 @@ def lib():
         L.hplo_matgen.argtypes = [i32, i32, vp, i32, i32, f64]
         L.hplo_pdfact.argtypes = [vp, i32, i32, i32, i32, i32, vp]
         L.hplo_pdfact.restype = i32
+        L.hplo_pdfact_v.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, vp]
+        L.hplo_pdfact_v.restype = i32
         L.hplo_pipid.argtypes = [i32, vp, vp]
         L.hplo_laswp.argtypes = [vp, i32, i32, i32, vp]
         L.hplo_dlaswp00N.argtypes = [i32, i32, vp, i32, vp]
         L.hplo_trsv_upper.argtypes = [i32, vp, i32, vp]
         L.hplo_pdgesv.argtypes = [i32, i32, i32, i32, vp, i32, vp, vp]
         L.hplo_pdgesv.restype = i32
+        L.hplo_pdgesv_v.argtypes = [i32, i32, i32, i32, i32, i32, vp, i32,
+                                    vp, vp]
+        L.hplo_pdgesv_v.restype = i32
         L.hplo_residual.argtypes = [i32, i32, f64, vp, vp]
         L.hplo_flops_ref.argtypes = [i32]
         L.hplo_flops_ref.restype = f64
@@ -107,13 +112,21 @@ def matgen_tile(M: int, i0: int, j0: int, m: int, n: int,
     return A[:m, :]
 
 
-def pdfact(P: np.ndarray, nbmin: int = 32, ndiv: int = 2):
-    """Factor the panel P (m x jb, Fortran order) in place; returns
-    (ipiv panel-relative int32, info)."""
+LEFT_LOOKING, CROUT, RIGHT_LOOKING = 301, 302, 303
+
+
+def pdfact(P: np.ndarray, nbmin: int = 32, ndiv: int = 2,
+           pfact: int = RIGHT_LOOKING, rfact: int = RIGHT_LOOKING):
+    """Factor the panel P (m x jb, Fortran order) in place with the given
+    panel (pfact) and recursion (rfact) variants; returns (ipiv
+    panel-relative int32, info)."""
     assert P.flags.f_contiguous and P.dtype == np.float64
     m, jb = P.shape
     ipiv = np.zeros(max(jb, 1), dtype=np.int32)
-    info = lib().hplo_pdfact(_p(P), max(m, 1), m, jb, nbmin, ndiv, _p(ipiv))
+    info = lib().hplo_pdfact_v(_p(P), max(m, 1), m, jb, nbmin, ndiv, pfact,
+                               rfact, _p(ipiv))
+    if info < 0:
+        raise ValueError("unknown variant pfact=%d rfact=%d" % (pfact, rfact))
     return ipiv[:jb], info
 
 
@@ -142,7 +155,8 @@ def dlaswp00N(A: np.ndarray, permU: np.ndarray) -> None:
 
 def pdgesv(n: int, nb: int, nbmin: int = 32, ndiv: int = 2,
            seed: int = HPL_ISEED, diag_shift: float = 0.0,
-           Ab: np.ndarray | None = None):
+           Ab: np.ndarray | None = None, pfact: int = RIGHT_LOOKING,
+           rfact: int = RIGHT_LOOKING):
     """Full P=Q=1 HPL_pdgesv on the seeded [A|b].  Returns dict with the
     factored augmented matrix, global pivots, x and info."""
     lda = hpl_ld(n)
@@ -154,8 +168,10 @@ def pdgesv(n: int, nb: int, nbmin: int = 32, ndiv: int = 2,
         full[:n, :] = Ab
     ipiv = np.zeros(max(n, 1), dtype=np.int32)
     x = np.zeros(max(n, 1), dtype=np.float64)
-    info = lib().hplo_pdgesv(n, nb, nbmin, ndiv, _p(full), lda, _p(ipiv),
-                             _p(x))
+    info = lib().hplo_pdgesv_v(n, nb, nbmin, ndiv, pfact, rfact, _p(full),
+                               lda, _p(ipiv), _p(x))
+    if info < 0:
+        raise ValueError("unknown variant pfact=%d rfact=%d" % (pfact, rfact))
     return {"LU": full[:n, :], "ipiv": ipiv[:n], "x": x[:n], "info": info,
             "lda": lda}
 

@@ -43,7 +43,7 @@ EXPORTS = [
     "hplx_set_option", "hplx_get_option",
     "hplx_malloc", "hplx_free", "hplx_memcpy_h2d", "hplx_memcpy_d2h",
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
-    "hplx_pdtest_check", "hplx_pdfact", "hplx_pipid", "hplx_dlaswp00N",
+    "hplx_pdtest_check", "hplx_pdfact", "hplx_pdfact_palg", "hplx_pipid", "hplx_dlaswp00N",
     "hplx_pdupdate", "hplx_dgemm", "hplx_pdtrsv", "hplx_update_stats",
     "hplx_update_timing",
     "hplx_pdgetrf", "hplx_pdgetrs", "hplx_pdgetrs_noalloc", "hplx_pdgesv_adjoint",
@@ -158,6 +158,7 @@ def lib():
         "hplx_pdgesv": (i32, [vp, P(PAlg), P(PMat)]),
         "hplx_pdtest_check": (i32, [vp, P(PMat), i32, f64, P(f64)]),
         "hplx_pdfact": (i32, [vp, i32, i32, vp, i32, i32, i32, vp, i32]),
+        "hplx_pdfact_palg": (i32, [vp, vp, vp, i32, i32, i32, vp, i32]),
         "hplx_pipid": (i32, [vp, i32, vp, vp]),
         "hplx_dlaswp00N": (i32, [vp, i32, i32, vp, i32, vp]),
         "hplx_pdupdate": (i32, [vp, vp, i32, i32, i32, i32, vp, vp]),
@@ -370,6 +371,13 @@ class Context:
         return self._chk(lib().hplx_pdfact(self.h, nbmin, nbdiv,
                                            ctypes.c_void_p(P), lda, m, jb,
                                            ctypes.c_void_p(ipiv), row_offset))
+
+    def pdfact_palg(self, palg: "PAlg", P: int, lda: int, m: int, jb: int,
+                    ipiv: int, row_offset: int = 0) -> int:
+        """HPL_pdfact with palg's pfact/rfact variants, nbmin and nbdiv."""
+        return self._chk(lib().hplx_pdfact_palg(
+            self.h, ctypes.byref(palg), ctypes.c_void_p(P), lda, m, jb,
+            ctypes.c_void_p(ipiv), row_offset))
 
     def pipid(self, jb: int, ipiv: int, permU: int) -> None:
         self._chk(lib().hplx_pipid(self.h, jb, ctypes.c_void_p(ipiv),

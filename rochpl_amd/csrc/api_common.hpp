@@ -50,14 +50,21 @@ inline int palg_status(const hplx_palg* p, char* msg, size_t len) {
     std::snprintf(msg, len, "null palg");
     return HPLX_ERR_INVALID;
   }
-  if(p->pfact != HPLX_RIGHT_LOOKING || p->rfact != HPLX_RIGHT_LOOKING) {
-    std::snprintf(msg, len,
-                  "only right-looking panel/recursive factorisation is built "
-                  "(pfact=%d rfact=%d)", p->pfact, p->rfact);
-    return HPLX_ERR_UNSUPPORTED;
+  if(!valid_fact(p->pfact) || !valid_fact(p->rfact)) {
+    std::snprintf(msg, len, "unknown panel factorisation (pfact=%d rfact=%d)",
+                  p->pfact, p->rfact);
+    return HPLX_ERR_INVALID;
   }
-  if(p->L1notran != 0 || p->Unotran != 0) {
-    std::snprintf(msg, len, "L1 and U must be in transposed form");
+  // L1notran picks how the reference stores its copy of the panel's top
+  // block (src/HPL_pddriver.cpp:166-199); the factors here stay in A, so
+  // both forms run the same kernels.  U in non-transposed form is refused,
+  // as the reference refuses it (src/HPL_pdinfo.cpp:979-986).
+  if(p->L1notran != 0 && p->L1notran != 1) {
+    std::snprintf(msg, len, "L1notran=%d (0 or 1)", p->L1notran);
+    return HPLX_ERR_INVALID;
+  }
+  if(p->Unotran != 0) {
+    std::snprintf(msg, len, "U in non-transposed form is not supported");
     return HPLX_ERR_UNSUPPORTED;
   }
   if(p->equil != 0) {

@@ -448,7 +448,8 @@ struct Fp64Loop {
   hplx_pmat*   mat;
   hplx::MatWs* ws;
   double*      A;
-  int          N, nb, lda, nbmin, nbdiv;
+  int          N, nb, lda;
+  hplx::PanelAlgo algo;
 
   double* Linv(int k) const { return (k & 1) ? c.Linv2 : c.Linv; }
   int*    permU(int k) const { return (k & 1) ? c.permU2 : c.permU; }
@@ -461,8 +462,8 @@ struct Fp64Loop {
     double*   P = A + j + static_cast<size_t>(j) * lda;
     c.panel_stream_  = s;
     c.panel_scratch_ = Linv(k);
-    const hipError_t e = hplx::panel_factor(c, nbmin, nbdiv, P, lda, N - j,
-                                            jb, mat->ipiv + j, j);
+    const hipError_t e = hplx::panel_factor(c, algo, P, lda, N - j, jb,
+                                            mat->ipiv + j, j);
     c.panel_stream_  = nullptr;
     c.panel_scratch_ = nullptr;
     HIPCHK(c, e);
@@ -562,7 +563,7 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
     return fail(c, HPLX_ERR_UNSUPPORTED, "N=%d exceeds the panel kernel's %d rows",
                 N, hplx::max_leaf_rows(c));
   Fp64Loop L{c, mat, static_cast<hplx::MatWs*>(mat->ws), mat->A, N, nb, lda,
-             palg->nbmin, palg->nbdiv};
+             hplx::panel_algo(*palg)};
   const int npanels = (N + nb - 1) / nb;
   while(static_cast<int>(c.ev.size()) < 6 * npanels) {
     hipEvent_t e;
@@ -794,8 +795,8 @@ int hplx_pdtest_check(hplx_ctx* ctx, const hplx_pmat* mat, int seed,
   return HPLX_SUCCESS;
 }
 
-int hplx_pdfact(hplx_ctx* ctx, int nbmin, int nbdiv, double* P, int lda, int m,
-                int jb, int* ipiv, int row_offset) {
+static int pdfact_algo(hplx_ctx* ctx, const hplx::PanelAlgo& algo, double* P,
+                       int lda, int m, int jb, int* ipiv, int row_offset) {
   int r = use(ctx);
   if(r) return r;
   Ctx& c = ctx->c;
@@ -804,8 +805,7 @@ int hplx_pdfact(hplx_ctx* ctx, int nbmin, int nbdiv, double* P, int lda, int m,
   if(jb > hplx::kMaxNB || m > hplx::max_leaf_rows(c))
     return fail(c, HPLX_ERR_UNSUPPORTED, "pdfact: m=%d jb=%d too large", m, jb);
   HIPCHK(c, hplx::reset_panel_state(c));
-  HIPCHK(c, hplx::panel_factor(c, nbmin, nbdiv, P, lda, m, jb, ipiv,
-                               row_offset));
+  HIPCHK(c, hplx::panel_factor(c, algo, P, lda, m, jb, ipiv, row_offset));
   int flags[2] = {0, 0};
   HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
                            c.stream));
@@ -813,6 +813,23 @@ int hplx_pdfact(hplx_ctx* ctx, int nbmin, int nbdiv, double* P, int lda, int m,
   if(flags[1])
     return fail(c, HPLX_ERR_HIP, "panel kernel grid barrier timed out");
   return flags[0] ? flags[0] - row_offset : HPLX_SUCCESS;
+}
+
+int hplx_pdfact(hplx_ctx* ctx, int nbmin, int nbdiv, double* P, int lda, int m,
+                int jb, int* ipiv, int row_offset) {
+  return pdfact_algo(ctx, hplx::panel_algo(nbmin, nbdiv), P, lda, m, jb, ipiv,
+                     row_offset);
+}
+
+int hplx_pdfact_palg(hplx_ctx* ctx, const hplx_palg* palg, double* P, int lda,
+                     int m, int jb, int* ipiv, int row_offset) {
+  if(!ctx) return HPLX_ERR_INVALID;
+  if(!palg) return fail(ctx->c, HPLX_ERR_INVALID, "pdfact_palg: null palg");
+  if(!hplx::valid_fact(palg->pfact) || !hplx::valid_fact(palg->rfact))
+    return fail(ctx->c, HPLX_ERR_INVALID, "pdfact_palg: pfact=%d rfact=%d",
+                palg->pfact, palg->rfact);
+  return pdfact_algo(ctx, hplx::panel_algo(*palg), P, lda, m, jb, ipiv,
+                     row_offset);
 }
 
 int hplx_pipid(hplx_ctx* ctx, int jb, const int* ipiv, int* permU) {

@@ -121,13 +121,39 @@ hipError_t leaf_occupancy(int* blocks_per_cu);
 // Zero the flags (queued on c.stream).
 hipError_t reset_panel_state(Ctx& c);
 
-// Panel factorisation of an m x jb panel in place (recursive right-looking,
-// register-resident cooperative leaves).  ipiv receives panel-relative pivot
-// rows + row_offset; zero pivots are reported as row_offset + column + 1.
-hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, double* P, int lda,
+// The panel factorisation's parameters (HPL_T_palg's pfact, rfact, nbmin,
+// nbdiv; pfact and rfact are HPLX_LEFT_LOOKING, HPLX_CROUT or
+// HPLX_RIGHT_LOOKING).
+struct PanelAlgo {
+  int nbmin = 32, nbdiv = 2;
+  int pfact = HPLX_RIGHT_LOOKING, rfact = HPLX_RIGHT_LOOKING;
+};
+inline bool valid_fact(int f) {
+  return f == HPLX_LEFT_LOOKING || f == HPLX_CROUT || f == HPLX_RIGHT_LOOKING;
+}
+inline PanelAlgo panel_algo(const hplx_palg& p) {
+  PanelAlgo a;
+  a.nbmin = p.nbmin;
+  a.nbdiv = p.nbdiv;
+  a.pfact = p.pfact;
+  a.rfact = p.rfact;
+  return a;
+}
+inline PanelAlgo panel_algo(int nbmin, int nbdiv) {
+  PanelAlgo a;
+  a.nbmin = nbmin;
+  a.nbdiv = nbdiv;
+  return a;
+}
+
+// Panel factorisation of an m x jb panel in place (recursive, in the
+// variants' orders, register-resident cooperative leaves).  ipiv receives
+// panel-relative pivot rows + row_offset; zero pivots are reported as
+// row_offset + column + 1.
+hipError_t panel_factor(Ctx& c, const PanelAlgo& a, double* P, int lda,
                         int m, int jb, int* ipiv, int row_offset);
 // The same recursion in FP32 (FP32 leaves, solves and MFMA GEMMs).
-hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, float* P, int lda,
+hipError_t panel_factor(Ctx& c, const PanelAlgo& a, float* P, int lda,
                         int m, int jb, int* ipiv, int row_offset);
 
 hipError_t launch_pipid(hipStream_t s, int jb, const int* ipiv, int ipiv_base,

@@ -298,72 +298,108 @@ inline hipError_t panel_gemm(Ctx& c, int m, int n, int k, const float* A,
                       ldc, false);
 }
 
-// Columns [c0, c0+n) of the panel, rows [c0, m).
-template <typename V>
-hipError_t leaf_or_chunks(Ctx& c, V* P, int lda, int m, int JB, int c0, int n,
-                          int* ipiv, int row_offset, int info_base) {
-  if(n <= kLeafMax)
-    return launch_leaf(c, P, lda, m, JB, c0, n, ipiv, row_offset,
-                       info_base);
-  // nbmin wider than the register leaf: right-looking over 32-column leaves
-  for(int k = 0; k < n; k += kLeafMax) {
-    const int w = (n - k) < kLeafMax ? (n - k) : kLeafMax;
-    hipError_t e = launch_leaf(c, P, lda, m, JB, c0 + k, w, ipiv,
-                               row_offset, info_base);
-    if(e != hipSuccess) return e;
-    const int top  = c0 + k;
-    const int rest = n - k - w;
-    if(rest > 0) {
-      V* L11 = P + top + static_cast<size_t>(top) * lda;
-      V* A12 = P + top + static_cast<size_t>(top + w) * lda;
-      e = launch_trsm_colsweep(panel_stream(c), w, rest, L11, lda, A12, lda);
-      if(e != hipSuccess) return e;
-      e = panel_gemm(c, m - top - w, rest, w, L11 + w, lda, A12, lda,
-                     A12 + w, lda);
+// One level of the blocked panel loop: columns [c0, c0 + n) of the panel,
+// rows [c0, m), in chunks of nb columns, each factored by factor(top, jb)
+// (top = the chunk's first row and column), in the variant's order:
+//   right (src/pfact/HPL_pdrpanrlT.cpp:116-160): factor the chunk, then its
+//     U rows right of it (L11^{-1} A12) and the rows below (A22 -= L21 U12);
+//   left (src/pfact/HPL_pdrpanllT.cpp:116-159): first the chunk's rows above
+//     it (U = L^{-1} A over the jj columns left of it in this level) and its
+//     rows from its top down (A -= L U), then factor it;
+//   Crout (src/pfact/HPL_pdrpancrT.cpp:116-175): the chunk's rows from its
+//     top down (A -= L U with the final U rows above), factor it, then its
+//     own U rows right of it (A -= L U over the jj columns, then L11^{-1}).
+// Row interchanges inside factor() span the whole panel width, so columns
+// left and right of the chunk are permuted before they are used.
+template <typename V, typename F>
+hipError_t chunk_loop(Ctx& c, int variant, V* P, int lda, int m, int c0,
+                      int n, int nb, F&& factor) {
+  auto at = [&](int r, int col) {
+    return P + r + static_cast<size_t>(col) * lda;
+  };
+  hipStream_t s  = panel_stream(c);
+  int         jj = 0, jb = nb < n ? nb : n;
+  while(jj < n) {
+    const int  top  = c0 + jj;
+    const int  nrem = n - jj - jb;
+    hipError_t e;
+    if(jj > 0 && variant == HPLX_LEFT_LOOKING) {
+      e = launch_trsm_colsweep(s, jj, jb, at(c0, c0), lda, at(c0, top), lda);
       if(e != hipSuccess) return e;
     }
+    if(jj > 0 && variant != HPLX_RIGHT_LOOKING) {
+      e = panel_gemm(c, m - top, jb, jj, at(top, c0), lda, at(c0, top), lda,
+                     at(top, top), lda);
+      if(e != hipSuccess) return e;
+    }
+    e = factor(top, jb);
+    if(e != hipSuccess) return e;
+    if(nrem > 0 && variant == HPLX_RIGHT_LOOKING) {
+      e = launch_trsm_colsweep(s, jb, nrem, at(top, top), lda,
+                               at(top, top + jb), lda);
+      if(e != hipSuccess) return e;
+      e = panel_gemm(c, m - top - jb, nrem, jb, at(top + jb, top), lda,
+                     at(top, top + jb), lda, at(top + jb, top + jb), lda);
+      if(e != hipSuccess) return e;
+    } else if(nrem > 0 && variant == HPLX_CROUT) {
+      if(jj > 0) {
+        e = panel_gemm(c, jb, nrem, jj, at(top, c0), lda, at(c0, top + jb),
+                       lda, at(top, top + jb), lda);
+        if(e != hipSuccess) return e;
+      }
+      e = launch_trsm_colsweep(s, jb, nrem, at(top, top), lda,
+                               at(top, top + jb), lda);
+      if(e != hipSuccess) return e;
+    }
+    jj += jb;
+    jb = (n - jj) < nb ? (n - jj) : nb;
   }
   return hipSuccess;
 }
 
+// Columns [c0, c0+n) of the panel, rows [c0, m), n <= nbmin: the panel leaf
+// (HPL_pdpanrlT / HPL_pdpanllT / HPL_pdpancrT).  The register leaf factors up
+// to kLeafMax columns in one kernel; it runs the right-looking schedule for
+// all three pfact variants: within a leaf they give every element the same
+// products in the same order (k ascending) and differ only in when each is
+// applied, so the right-looking leaf computes the left-looking and Crout
+// leaves' results exactly.  Wider leaves (nbmin > kLeafMax) are blocked into
+// kLeafMax-column leaves in the pfact variant's order.
 template <typename V>
-hipError_t rpanel(Ctx& c, int nbmin, int nbdiv, V* P, int lda, int m, int JB,
+hipError_t leaf_or_chunks(Ctx& c, int pfact, V* P, int lda, int m, int JB,
+                          int c0, int n, int* ipiv, int row_offset,
+                          int info_base) {
+  auto leaf = [&](int top, int w) {
+    return launch_leaf(c, P, lda, m, JB, top, w, ipiv, row_offset, info_base);
+  };
+  if(n <= kLeafMax) return leaf(c0, n);
+  return chunk_loop(c, pfact, P, lda, m, c0, n, kLeafMax, leaf);
+}
+
+// The recursive panel factorisation (HPL_pdrpan{rl,ll,cr}T): chunks of
+// nb = ceil(ceil(n / nbmin) / nbdiv) * nbmin columns (HPL_pdrpanrlT.cpp:93),
+// each factored recursively, in the rfact variant's order.
+template <typename V>
+hipError_t rpanel(Ctx& c, const PanelAlgo& a, V* P, int lda, int m, int JB,
                   int c0, int n, int* ipiv, int row_offset, int info_base) {
-  if(n <= nbmin)
-    return leaf_or_chunks(c, P, lda, m, JB, c0, n, ipiv, row_offset,
+  if(n <= a.nbmin)
+    return leaf_or_chunks(c, a.pfact, P, lda, m, JB, c0, n, ipiv, row_offset,
                           info_base);
-  const int nb = (((n + nbmin - 1) / nbmin + nbdiv - 1) / nbdiv) * nbmin;
-  int       jj = 0, jb = nb, nrem = n;
-  do {
-    nrem -= jb;
-    hipError_t e = rpanel(c, nbmin, nbdiv, P, lda, m, JB, c0 + jj, jb,
-                          ipiv, row_offset, info_base);
-    if(e != hipSuccess) return e;
-    const int top = c0 + jj;
-    if(nrem > 0) {
-      V* L11 = P + top + static_cast<size_t>(top) * lda;
-      V* A12 = P + top + static_cast<size_t>(top + jb) * lda;
-      e = launch_trsm_colsweep(panel_stream(c), jb, nrem, L11, lda, A12, lda);
-      if(e != hipSuccess) return e;
-      e = panel_gemm(c, m - top - jb, nrem, jb, L11 + jb, lda, A12, lda,
-                     A12 + jb, lda);
-      if(e != hipSuccess) return e;
-    }
-    jj += jb;
-    jb = nrem < nb ? nrem : nb;
-  } while(nrem > 0);
-  return hipSuccess;
+  const int nb = (((n + a.nbmin - 1) / a.nbmin + a.nbdiv - 1) / a.nbdiv) * a.nbmin;
+  return chunk_loop(c, a.rfact, P, lda, m, c0, n, nb, [&](int top, int jb) {
+    return rpanel(c, a, P, lda, m, JB, top, jb, ipiv, row_offset, info_base);
+  });
 }
 
 template <typename V>
-hipError_t panel_factor_t(Ctx& c, int nbmin, int nbdiv, V* P, int lda, int m,
-                          int jb, int* ipiv, int row_offset) {
-  if(nbmin < 1) nbmin = 1;
-  if(nbdiv < 2) nbdiv = 2;
+hipError_t panel_factor_t(Ctx& c, PanelAlgo a, V* P, int lda, int m, int jb,
+                          int* ipiv, int row_offset) {
+  if(a.nbmin < 1) a.nbmin = 1;
+  if(a.nbdiv < 2) a.nbdiv = 2;
+  if(!valid_fact(a.pfact) || !valid_fact(a.rfact)) return hipErrorInvalidValue;
   if(jb > kMaxNB || jb > m) return hipErrorInvalidValue;
   if(m > max_leaf_rows(c)) return hipErrorInvalidValue;
-  return rpanel(c, nbmin, nbdiv, P, lda, m, jb, 0, jb, ipiv, row_offset,
-                row_offset);
+  return rpanel(c, a, P, lda, m, jb, 0, jb, ipiv, row_offset, row_offset);
 }
 
 }  // namespace
@@ -387,14 +423,14 @@ hipError_t reset_panel_state(Ctx& c) {
   return hipGetLastError();
 }
 
-hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, double* P, int lda,
+hipError_t panel_factor(Ctx& c, const PanelAlgo& a, double* P, int lda,
                         int m, int jb, int* ipiv, int row_offset) {
-  return panel_factor_t(c, nbmin, nbdiv, P, lda, m, jb, ipiv, row_offset);
+  return panel_factor_t(c, a, P, lda, m, jb, ipiv, row_offset);
 }
 
-hipError_t panel_factor(Ctx& c, int nbmin, int nbdiv, float* P, int lda,
+hipError_t panel_factor(Ctx& c, const PanelAlgo& a, float* P, int lda,
                         int m, int jb, int* ipiv, int row_offset) {
-  return panel_factor_t(c, nbmin, nbdiv, P, lda, m, jb, ipiv, row_offset);
+  return panel_factor_t(c, a, P, lda, m, jb, ipiv, row_offset);
 }
 
 hipError_t launch_pipid(hipStream_t s, int jb, const int* ipiv, int ipiv_base,

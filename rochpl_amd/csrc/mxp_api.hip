@@ -149,7 +149,7 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
       P   = Ap;
       ldP = lda;
     }
-    HIPCHK(c, hplx::panel_factor(c, palg->nbmin, palg->nbdiv, P, ldP, m, jb,
+    HIPCHK(c, hplx::panel_factor(c, hplx::panel_algo(*palg), P, ldP, m, jb,
                                  mat->ipiv + j, j));
     if constexpr(kF16)
       HIPCHK(c, hplx::launch_copy_block(s, m, jb, P, ldP, Ap, lda));
@@ -575,7 +575,8 @@ int hplx_spdfact(hplx_ctx* ctx, int nbmin, int nbdiv, float* P, int lda,
   if(m > hplx::max_leaf_rows(c))
     return fail(c, HPLX_ERR_UNSUPPORTED, "spdfact: m=%d too tall", m);
   HIPCHK(c, hplx::reset_panel_state(c));
-  HIPCHK(c, hplx::panel_factor(c, nbmin, nbdiv, P, lda, m, jb, ipiv, row_offset));
+  HIPCHK(c, hplx::panel_factor(c, hplx::panel_algo(nbmin, nbdiv), P, lda, m,
+                               jb, ipiv, row_offset));
   int flags[2] = {0, 0};
   HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
                            c.stream));

@@ -4,15 +4,15 @@
 // (command line -P -Q -p -q -N -NB -f -it -i --version -h, the 31-line
 // HPL.dat, parameter echo) and src/HPL_pdtest.cpp (matgen, timed
 // HPL_pdgesv, W-line, residual check).  One process drives one GPU, so only
-// the P=Q=1 grid runs; other grids, and panel variants the library does not
-// implement, are counted as skipped with a message, as the reference counts
-// tests it cannot run.
+// the P=Q=1 grid runs; other grids are counted as skipped with a message, as
+// the reference counts tests it cannot run.  Every PFACT and RFACT runs.
 //
 // Extra flags (not in the reference): --parse-only echoes the parameters and
 // exits without touching the GPU; --dump DIR writes each solved test's x
 // (N doubles) and pivot vector (N int32, 0-based global rows) to
-// DIR/xhplx_N<N>_NB<NB>_NBMIN<nbmin>_NDIV<ndiv>_it<it>.{x,ipiv}, so that a
-// checker can recompute the residual and compare the pivots.
+// DIR/xhplx_N<N>_NB<NB>_PF<L|C|R>_NBMIN<nbmin>_NDIV<ndiv>_RF<L|C|R>_it<it>
+// .{x,ipiv}, so that a checker can recompute the residual and compare the
+// pivots.
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -281,16 +281,22 @@ double wall() {
 }
 
 // --dump: x and the pivot vector of one solve, raw little-endian arrays
+char fact_letter(int f) {
+  return f == HPLX_LEFT_LOOKING ? 'L' : f == HPLX_CROUT ? 'C' : 'R';
+}
+
 bool dump_solution(hplx_ctx* ctx, const Params& p, const hplx_pmat& mat, int N,
-                   int NB, int nbm, int ndv, int it) {
+                   int NB, int pf, int nbm, int ndv, int rf, int it) {
   std::vector<double> x(N);
   std::vector<int>    piv(N);
   if(hplx_memcpy_d2h(ctx, x.data(), mat.X, sizeof(double) * N) != HPLX_SUCCESS ||
      hplx_memcpy_d2h(ctx, piv.data(), mat.ipiv, sizeof(int) * N) != HPLX_SUCCESS)
     return false;
   char base[512];
-  std::snprintf(base, sizeof(base), "%s/xhplx_N%d_NB%d_NBMIN%d_NDIV%d_it%d",
-                p.dump_dir.c_str(), N, NB, nbm, ndv, it);
+  std::snprintf(base, sizeof(base),
+                "%s/xhplx_N%d_NB%d_PF%c_NBMIN%d_NDIV%d_RF%c_it%d",
+                p.dump_dir.c_str(), N, NB, fact_letter(pf), nbm, ndv,
+                fact_letter(rf), it);
   const std::string names[2] = {std::string(base) + ".x",
                                 std::string(base) + ".ipiv"};
   const void*  data[2]  = {x.data(), piv.data()};
@@ -325,12 +331,6 @@ void run_test(hplx_ctx* ctx, const Params& p, int N, int NB, int pf, int nbm,
   a.align    = p.align;
   a.frac     = p.frac;
   a.its      = p.its;
-  if(pf != HPLX_RIGHT_LOOKING || rf != HPLX_RIGHT_LOOKING || p.L1notran) {
-    warn("skipping N=%d NB=%d: only PFACT=RFACT=Right with L1 transposed is "
-         "built", N, NB);
-    cnt.kskip++;
-    return;
-  }
   hplx_pmat mat;
   if(hplx_pmat_alloc(ctx, N, NB, &mat) != HPLX_SUCCESS) {
     warn("[%d,%d] memory allocation failed for A, x and b: %s -- skip", N, NB,
@@ -359,14 +359,15 @@ void run_test(hplx_ctx* ctx, const Params& p, int N, int NB, int pf, int nbm,
     }
     const double Nd = static_cast<double>(N);
     const double gflops = (Nd / 1.0e9) * (Nd / t) * ((2.0 / 3.0) * Nd + 1.5);
-    const char   rfc = 'R', pfc = 'R';
+    const char   rfc = fact_letter(rf), pfc = fact_letter(pf);
     const char   ctop = static_cast<char>('0' + (tp - HPLX_1RING));
     std::fprintf(g_out, "W%c%1d%c%c%1d%c%1d%12d %5d %5d %5d %18.2f     %18.3e\n",
                  p.pmap == 0 ? 'R' : 'C', dh, ctop, rfc, ndv, pfc, nbm, N, NB,
                  1, 1, t, gflops);
     std::fprintf(g_out, "HPL_pdgesv() start time %s\n", std::ctime(&ts));
     std::fprintf(g_out, "HPL_pdgesv() end time   %s\n", std::ctime(&te));
-    if(!p.dump_dir.empty() && !dump_solution(ctx, p, mat, N, NB, nbm, ndv, it))
+    if(!p.dump_dir.empty() &&
+       !dump_solution(ctx, p, mat, N, NB, pf, nbm, ndv, rf, it))
       warn("could not write the --dump files of N=%d NB=%d", N, NB);
     if(rc > 0) {
       warn("exact zero pivot at column %d", rc);
@@ -531,14 +532,15 @@ int main(int argc, char** argv) {
       cnt.kskip += ntests;
       continue;
     }
+    // the reference's nesting (src/HPL_pddriver.cpp:141-157)
     for(int N : p.N)
       for(int NB : p.NB)
-        for(int pf : p.PF)
-          for(int nbm : p.NBM)
-            for(int ndv : p.NDV)
-              for(int rf : p.RF)
-                for(int tp : p.TP)
-                  for(int dh : p.DH)
+        for(int dh : p.DH)
+          for(int tp : p.TP)
+            for(int rf : p.RF)
+              for(int pf : p.PF)
+                for(int nbm : p.NBM)
+                  for(int ndv : p.NDV)
                     run_test(ctx, p, N, NB, pf, nbm, ndv, rf, tp, dh, cnt, first);
   }
   hplx_ctx_destroy(ctx);

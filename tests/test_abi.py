@@ -114,10 +114,25 @@ def test_palg_settings_the_reference_refuses_are_unsupported(field, value):
     assert rp.lib().hplx_palg_check(ctypes.byref(p)) == rp.HPLX_ERR_UNSUPPORTED
 
 
+@pytest.mark.parametrize("field,value", [
+    ("pfact", rp.HPLX_LEFT_LOOKING), ("pfact", rp.HPLX_CROUT),
+    ("rfact", rp.HPLX_LEFT_LOOKING), ("rfact", rp.HPLX_CROUT),
+    ("L1notran", 1)])
+def test_palg_panel_variants_are_accepted(field, value):
+    """Every PFACT / RFACT of HPL.dat and L1 in either form
+    (src/HPL_pddriver.cpp:164-200) are built."""
+    if not os.path.exists(rp.LIB_PATH):
+        pytest.skip("libhplx.so not built")
+    p = rp.palg_default()
+    setattr(p, field, value)
+    assert rp.lib().hplx_palg_check(ctypes.byref(p)) == rp.HPLX_SUCCESS
+
+
 def test_palg_invalid_recursion_and_null():
     if not os.path.exists(rp.LIB_PATH):
         pytest.skip("libhplx.so not built")
-    for field, value in (("nbmin", 0), ("nbdiv", 1)):
+    for field, value in (("nbmin", 0), ("nbdiv", 1), ("pfact", 300),
+                         ("rfact", 304), ("L1notran", 2)):
         p = rp.palg_default()
         setattr(p, field, value)
         assert rp.lib().hplx_palg_check(ctypes.byref(p)) == rp.HPLX_ERR_INVALID

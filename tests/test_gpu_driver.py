@@ -1,15 +1,14 @@
 """End-to-end runs of the xhplx driver.
 
 config 1 (N=4096, NB=256, P=Q=1): the reference's own acceptance check, the
-PASSED line.  A multi-value HPL.dat (src/HPL_pddriver.cpp:141-220 loops over
-N x NB x PFACT x NBMIN and the grids): every test it runs is checked against
+PASSED line.  A multi-value HPL.dat (src/HPL_pddriver.cpp:130-220 loops over
+the grids, N, NB, RFACT, PFACT and NBMIN): every test it runs is checked against
 the oracle (pivots equal to LAPACK dgetrf's, x equal to LAPACK's solution to
 the accuracy the residual implies, the printed residuals equal to the
 library's residual check of the dumped x and within rounding of the
 oracle's), every line it prints is re-formatted with
 the reference's printf formats and must come out byte for byte the same, and
-the summary counts the tests it cannot run (Crout panels, a 1x2 grid) as
-skipped."""
+the summary counts the tests it cannot run (a 1x2 grid) as skipped."""
 import os
 import re
 import subprocess
@@ -61,13 +60,13 @@ def test_xhplx_multi_value_loops_match_the_oracle(ctx, tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.splitlines()
     # the tests xhplx runs, in the reference's loop order (grid, N, NB,
-    # PFACT, NBMIN, NDIV, RFACT, BCAST, DEPTH); Crout panels are skipped
-    expect = [(n, nb, nbmin) for n in (1000, 4096) for nb in (64, 256)
-              for nbmin in (32, 16)]
+    # DEPTH, BCAST, RFACT, PFACT, NBMIN, NDIV; src/HPL_pddriver.cpp:130-157)
+    expect = [(n, nb, rf, pf, nbmin) for n in (1000, 4096) for nb in (64, 256)
+              for rf in ("R", "L") for pf in ("R", "C") for nbmin in (32, 16)]
     wl = [i for i, ln in enumerate(lines) if ln.startswith("W")]
     assert len(wl) == len(expect), r.stdout
     lu_cache = {}
-    for (n, nb, nbmin), i in zip(expect, wl):
+    for (n, nb, rf, pf, nbmin), i in zip(expect, wl):
         line = lines[i]
         m = W_RE.match(line)
         assert m, line
@@ -76,7 +75,7 @@ def test_xhplx_multi_value_loops_match_the_oracle(ctx, tmp_path):
                 int(g[7]), int(g[8]), int(g[9]), int(g[10]), float(g[11]),
                 float(g[12]))
         assert W_FMT % vals == line, (W_FMT % vals, line)
-        assert vals[:11] == ("C", 1, "0", "R", 2, "R", nbmin, n, nb, 1, 1)
+        assert vals[:11] == ("C", 1, "0", rf, 2, pf, nbmin, n, nb, 1, 1)
         assert vals[11] > 0 and vals[12] > 0
         # the residual line that follows the W-line and the time stamps
         k = next(j for j in range(i, len(lines))
@@ -88,7 +87,8 @@ def test_xhplx_multi_value_loops_match_the_oracle(ctx, tmp_path):
                       lines[k + 1])
         assert r0, lines[k + 1]
         # against the oracle: the dumped x and pivots of this test
-        base = dump / ("xhplx_N%d_NB%d_NBMIN%d_NDIV2_it0" % (n, nb, nbmin))
+        base = dump / ("xhplx_N%d_NB%d_PF%s_NBMIN%d_NDIV2_RF%s_it0"
+                       % (n, nb, pf, nbmin, rf))
         x = np.fromfile(str(base) + ".x", dtype=np.float64)
         ipiv = np.fromfile(str(base) + ".ipiv", dtype=np.int32)
         assert x.shape == (n,) and ipiv.shape == (n,)
@@ -115,18 +115,18 @@ def test_xhplx_multi_value_loops_match_the_oracle(ctx, tmp_path):
         if n not in lu_cache:
             A = o.matgen(n, n)
             lu_cache[n] = sl.lu_factor(A)
-        assert np.array_equal(ipiv, lu_cache[n][1]), (n, nb, nbmin)
+        assert np.array_equal(ipiv, lu_cache[n][1]), (n, nb, rf, pf, nbmin)
         # x solves the system to the accuracy the residual states
         xr = sl.lu_solve(lu_cache[n], o.matgen(n, n + 1)[:, n])
         assert np.max(np.abs(x - xr)) <= 1e-8 * np.max(np.abs(xr))
-    # summary: 2 grids x 2 N x 2 NB x 2 PFACT x 2 NBMIN = 32 tests; the 8
-    # P=Q=1 Right-looking ones ran, the Crout ones and the 1x2 grid skipped
+    # summary: 2 grids x 2 N x 2 NB x 2 RFACT x 2 PFACT x 2 NBMIN = 64
+    # tests; the 32 of the P=Q=1 grid ran, the 1x2 grid's were skipped
     text = r.stdout
-    assert "Finished %6d tests with the following results:" % 32 in text
-    assert SUMMARY_FMT % (8, "tests completed and passed residual checks,") \
+    assert "Finished %6d tests with the following results:" % 64 in text
+    assert SUMMARY_FMT % (32, "tests completed and passed residual checks,") \
         in lines
     assert SUMMARY_FMT % (0, "tests completed and failed residual checks,") \
         in lines
-    assert SUMMARY_FMT % (24, "tests skipped because of illegal input "
+    assert SUMMARY_FMT % (32, "tests skipped because of illegal input "
                               "values.") in lines
     assert "End of Tests." in lines

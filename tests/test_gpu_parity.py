@@ -121,6 +121,68 @@ def test_pdfact_matches_oracle(ctx, m, jb, nbmin, ndiv):
     assert np.max(np.abs(got - P)) < 1e-11
 
 
+_FACT = {"L": rp.HPLX_LEFT_LOOKING, "C": rp.HPLX_CROUT,
+         "R": rp.HPLX_RIGHT_LOOKING}
+
+
+# every (pfact, rfact) pair, with nbmin under and over the 32-column register
+# leaf (48: the leaf level is blocked in the pfact variant's order), ndiv 2
+# and 3, and an odd panel height
+@pytest.mark.parametrize("pf", ["L", "C", "R"])
+@pytest.mark.parametrize("rf", ["L", "C", "R"])
+@pytest.mark.parametrize("m,jb,nbmin,ndiv", [
+    (3001, 256, 16, 2), (2000, 192, 48, 3), (700, 96, 32, 2)])
+def test_pdfact_variants_match_oracle(ctx, pf, rf, m, jb, nbmin, ndiv):
+    """HPL_pdfact with rffun = HPL_pdrpan{ll,cr,rl}T and pffun =
+    HPL_pdpan{ll,cr,rl}T (src/HPL_pddriver.cpp:164-200) against the oracle's
+    restatement of the same variant: the pivots are equal and the factors
+    agree to rounding (the GPU groups each variant's sums in MFMA tiles)."""
+    say("pdfact variant pfact=%s rfact=%s m=%d jb=%d nbmin=%d ndiv=%d"
+        % (pf, rf, m, jb, nbmin, ndiv))
+    P0 = np.asfortranarray(o.matgen_tile(m + 11, 0, 0, m, jb, seed=m + jb))
+    dP, lda = upload_matrix(ctx, P0, m + 11)
+    dpiv = ctx.alloc(4 * jb)
+    a = rp.palg_default()
+    a.pfact, a.rfact, a.nbmin, a.nbdiv = _FACT[pf], _FACT[rf], nbmin, ndiv
+    info = ctx.pdfact_palg(a, dP.ptr, lda, m, jb, dpiv.ptr, 0)
+    got = dP.download((lda, jb), np.float64)[:m]
+    gpiv = dpiv.download(jb, np.int32)
+    P = P0.copy(order="F")
+    wpiv, winfo = o.pdfact(P, nbmin=nbmin, ndiv=ndiv, pfact=_FACT[pf],
+                           rfact=_FACT[rf])
+    assert info == winfo == 0
+    assert np.array_equal(gpiv, wpiv)
+    assert np.max(np.abs(got - P)) < 1e-11
+
+
+@pytest.mark.parametrize("pf,rf,l1", [
+    ("L", "L", 0), ("C", "C", 0), ("L", "C", 1), ("C", "R", 0), ("R", "L", 1)])
+def test_pdgesv_variants_match_oracle(ctx, pf, rf, l1):
+    """The whole solve with a panel variant (and L1 in either form, which
+    selects no different kernel here): pivots equal the oracle's (and so
+    LAPACK's), x agrees to the accuracy the residual implies, and the
+    residual passes."""
+    n, nb = 3000, 192
+    say("pdgesv variant pfact=%s rfact=%s L1notran=%d n=%d nb=%d"
+        % (pf, rf, l1, n, nb))
+    a = rp.palg_default()
+    a.pfact, a.rfact, a.nbmin, a.L1notran = _FACT[pf], _FACT[rf], 16, l1
+    mat = ctx.pmat_alloc(n, nb)
+    try:
+        ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+        info = ctx.pdgesv(a, mat)
+        x = ctx.download_vec(mat.X, n)
+        ipiv = ctx.download_vec(mat.ipiv, n, np.int32)
+        chk = ctx.pdtest_check(mat)
+    finally:
+        ctx.pmat_free(mat)
+    w = o.pdgesv(n, nb, nbmin=16, pfact=_FACT[pf], rfact=_FACT[rf])
+    assert info == w["info"] == 0
+    assert np.array_equal(ipiv, w["ipiv"])
+    assert chk["resid"] < 16.0
+    assert np.max(np.abs(x - w["x"])) < 1e-8 * np.max(np.abs(w["x"]))
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_pdfact_exact_ties_take_the_first_row(ctx, precision):
     """Exact ties of max |a| go to the first row, as in the oracle and
@@ -335,17 +397,20 @@ def test_pdgesv_zero_pivot_returns_info(ctx):
 
 
 def test_pdgesv_rejects_unsupported_variants(ctx):
+    """U in non-transposed form is refused, as the reference refuses it
+    (src/HPL_pdinfo.cpp:979-986); an unknown panel variant is invalid."""
     mat = ctx.pmat_alloc(64, 32)
     try:
         p = rp.palg_default()
-        p.pfact = rp.HPLX_CROUT
+        p.Unotran = 1
         with pytest.raises(rp.HplxError) as e:
             ctx.pdgesv(p, mat)
         assert e.value.code == rp.HPLX_ERR_UNSUPPORTED
         p = rp.palg_default()
-        p.L1notran = 1
-        with pytest.raises(rp.HplxError):
+        p.pfact = 300
+        with pytest.raises(rp.HplxError) as e:
             ctx.pdgesv(p, mat)
+        assert e.value.code == rp.HPLX_ERR_INVALID
     finally:
         ctx.pmat_free(mat)
 
