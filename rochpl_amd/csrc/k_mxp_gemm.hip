@@ -37,6 +37,8 @@
 // The next stage's global loads are in flight while the current stage's
 // MFMAs run; LDS is single-buffered (two barriers per stage).  Workgroup ids
 // are remapped XCD-aware as in the FP64 GEMM.
+#include <type_traits>
+
 #include "hplx_internal.hpp"
 #include "mxp.hpp"
 
@@ -327,9 +329,19 @@ typedef _Float16 f16x4v __attribute__((ext_vector_type(4)));
 template <typename S>
 __device__ inline void epilogue_vec(f32x4 (&acc)[4][4], S* __restrict__ C,
                                     int ldc, int r0, int c0, float* E) {
+  typedef typename std::conditional<sizeof(S) == 4, f32x4v, f16x4v>::type CV;
   const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+  const int rl   = 4 * l16;  // local rows rl .. rl+3
 #pragma unroll
   for(int p = 0; p < 2; ++p) {
+    // the pass's 8 C vectors in flight at once, before the LDS transpose:
+    // one memory latency per pass instead of one per vector (a load issued
+    // after a store to C would wait for it)
+    CV x[8];
+#pragma unroll
+    for(int q = 0; q < 8; ++q)
+      x[q] = *reinterpret_cast<const CV*>(
+          C + (r0 + rl) + static_cast<size_t>(c0 + 32 * p + 4 * q + lk) * ldc);
 #pragma unroll
     for(int jj = 0; jj < 2; ++jj)
 #pragma unroll
@@ -342,18 +354,15 @@ __device__ inline void epilogue_vec(f32x4 (&acc)[4][4], S* __restrict__ C,
 #pragma unroll
     for(int q = 0; q < 8; ++q) {
       const int    cl = 4 * q + lk;  // local column 0..31
-      const int    rl = 4 * l16;     // local rows rl .. rl+3
       const f32x4v v  = *reinterpret_cast<const f32x4v*>(&E[cl * kEpiPitch + rl]);
       S* c = C + (r0 + rl) + static_cast<size_t>(c0 + 32 * p + cl) * ldc;
       if constexpr(sizeof(S) == 4) {
-        f32x4v x = *reinterpret_cast<const f32x4v*>(c);
-        *reinterpret_cast<f32x4v*>(c) = x - v;
+        *reinterpret_cast<f32x4v*>(c) = x[q] - v;
       } else {
-        const f16x4v x = *reinterpret_cast<const f16x4v*>(c);
-        f16x4v       y;
+        f16x4v y;
 #pragma unroll
         for(int e = 0; e < 4; ++e)
-          y[e] = static_cast<_Float16>(static_cast<float>(x[e]) - v[e]);
+          y[e] = static_cast<_Float16>(static_cast<float>(x[q][e]) - v[e]);
         *reinterpret_cast<f16x4v*>(c) = y;
       }
     }
@@ -478,7 +487,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
 }
 
 template <int WM, int WN, int KS, typename S>
-__global__ __launch_bounds__(64 * WM * WN) void gemm_fp8_kernel(
+__global__ __launch_bounds__(64 * WM * WN, 3) void gemm_fp8_kernel(
     int M, int N, int nks, const uint8_t* __restrict__ Ap, int m_pad,
     const int* __restrict__ ea, const uint8_t* __restrict__ Bp, int n_pad,
     const int* __restrict__ eb, int ntn128, S* __restrict__ C, int ldc,
