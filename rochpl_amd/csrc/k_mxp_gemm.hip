@@ -29,11 +29,13 @@
 // FP32 accumulators: the MFMA applies the factor itself through its E8M0
 // operand scales (127 - eA, 127 - eB) and accumulates in place.
 //
-// Tile: BM x BN = 64*WM x 64*WN with WM x WN waves, each wave 64 x 64 as
-// 4 x 4 blocks of 16 x 16.  The MFMA computes the transposed block
+// Tile: WM x WN waves, each wave 64 rows x 16*NI columns as 4 x NI blocks of
+// 16 x 16 (BF16: 128 x 128 workgroup tiles of 64 x 64 waves; FP8: 256 x 128
+// tiles of four 64 x 128 waves, whose stages carry 25 % fewer operand bytes
+// per flop).  The MFMA computes the transposed block
 // (D = B^T A^T) so that 16 lanes hold 16 consecutive rows of one C column;
 // the C update of full tiles goes through LDS as 4-row vectors per lane
-// (epilogue_vec).
+// (epilogue_vec), with non-temporal C accesses.
 // The next stage's global loads are in flight while the current stage's
 // MFMAs run; LDS is single-buffered (two barriers per stage).  Workgroup ids
 // are remapped XCD-aware as in the FP64 GEMM.
@@ -326,22 +328,24 @@ constexpr int epi_bytes(int waves) { return waves * kEpiWaveFloats * 4; }
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4v __attribute__((ext_vector_type(4)));
 
-template <typename S>
-__device__ inline void epilogue_vec(f32x4 (&acc)[4][4], S* __restrict__ C,
+template <int NI, typename S>
+__device__ inline void epilogue_vec(f32x4 (&acc)[4][NI], S* __restrict__ C,
                                     int ldc, int r0, int c0, float* E) {
   typedef typename std::conditional<sizeof(S) == 4, f32x4v, f16x4v>::type CV;
   const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
   const int rl   = 4 * l16;  // local rows rl .. rl+3
 #pragma unroll
-  for(int p = 0; p < 2; ++p) {
+  for(int p = 0; p < NI / 2; ++p) {
     // the pass's 8 C vectors in flight at once, before the LDS transpose:
     // one memory latency per pass instead of one per vector (a load issued
-    // after a store to C would wait for it)
+    // after a store to C would wait for it).  C is read and written once per
+    // update, so its accesses are non-temporal: streamed through L2 without
+    // evicting the operand stages the other workgroups of the XCD reuse
     CV x[8];
 #pragma unroll
     for(int q = 0; q < 8; ++q)
-      x[q] = *reinterpret_cast<const CV*>(
-          C + (r0 + rl) + static_cast<size_t>(c0 + 32 * p + 4 * q + lk) * ldc);
+      x[q] = __builtin_nontemporal_load(reinterpret_cast<const CV*>(
+          C + (r0 + rl) + static_cast<size_t>(c0 + 32 * p + 4 * q + lk) * ldc));
 #pragma unroll
     for(int jj = 0; jj < 2; ++jj)
 #pragma unroll
@@ -357,13 +361,13 @@ __device__ inline void epilogue_vec(f32x4 (&acc)[4][4], S* __restrict__ C,
       const f32x4v v  = *reinterpret_cast<const f32x4v*>(&E[cl * kEpiPitch + rl]);
       S* c = C + (r0 + rl) + static_cast<size_t>(c0 + 32 * p + cl) * ldc;
       if constexpr(sizeof(S) == 4) {
-        *reinterpret_cast<f32x4v*>(c) = x[q] - v;
+        __builtin_nontemporal_store(x[q] - v, reinterpret_cast<f32x4v*>(c));
       } else {
         f16x4v y;
 #pragma unroll
         for(int e = 0; e < 4; ++e)
           y[e] = static_cast<_Float16>(static_cast<float>(x[q][e]) - v[e]);
-        *reinterpret_cast<f16x4v*>(c) = y;
+        __builtin_nontemporal_store(y, reinterpret_cast<f16x4v*>(c));
       }
     }
     __syncthreads();
@@ -486,15 +490,19 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
   }
 }
 
-template <int WM, int WN, int KS, typename S>
-__global__ __launch_bounds__(64 * WM * WN, 3) void gemm_fp8_kernel(
+// WM x WN waves, each 64 rows x 16*NI columns (4 x NI MFMA blocks); the
+// workgroup tile is 64*WM x 16*NI*WN.  MINW: waves per SIMD the register
+// allocation must allow.
+template <int WM, int WN, int NI, int KS, int MINW, typename S>
+__global__ __launch_bounds__(64 * WM * WN, MINW) void gemm_fp8_kernel(
     int M, int N, int nks, const uint8_t* __restrict__ Ap, int m_pad,
     const int* __restrict__ ea, const uint8_t* __restrict__ Bp, int n_pad,
     const int* __restrict__ eb, int ntn128, S* __restrict__ C, int ldc,
     int tiles_m, int tiles_n, int vec_ok) {
   constexpr int NT  = 64 * WM * WN;
   constexpr int BM  = 64 * WM;
-  constexpr int BN  = 64 * WN;
+  constexpr int WNC = 16 * NI;  // columns per wave
+  constexpr int BN  = WNC * WN;
   // LDS row: KS stages of 128 bytes, padded by 16 (68 dwords for KS = 2: the
   // 16 rows of a fragment read start 4 banks apart)
   constexpr int RS  = KS * 128 + 16;
@@ -510,7 +518,7 @@ __global__ __launch_bounds__(64 * WM * WN, 3) void gemm_fp8_kernel(
   if(!map_tile(tiles_m, tiles_n, tm, tn)) return;
   const int m0 = tm * BM, n0 = tn * BN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm0 = (wave % WM) * 64, wn0 = (wave / WM) * 64;
+  const int wm0 = (wave % WM) * 64, wn0 = (wave / WM) * WNC;
   const int l16 = lane & 15, lk = lane >> 4;
   const int atile = (m0 + wm0) / 128;  // this wave's scale tiles
   const int btile = (n0 + wn0) / 128;
@@ -548,11 +556,11 @@ __global__ __launch_bounds__(64 * WM * WN, 3) void gemm_fp8_kernel(
     }
   };
 
-  f32x4 acc[4][4];
+  f32x4 acc[4][NI];
 #pragma unroll
   for(int i = 0; i < 4; ++i)
 #pragma unroll
-    for(int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for(int j = 0; j < NI; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   load(0);
   for(int it = 0; it < nit; ++it) {
@@ -567,28 +575,26 @@ __global__ __launch_bounds__(64 * WM * WN, 3) void gemm_fp8_kernel(
       // (2^(e - 127) per operand): 2^-(eA + eB) on every product
       const int sa = 127 - ea[atile * nks + ks];
       const int sb = 127 - eb[ks * ntn128 + btile];
-      i32x8 fa[4], fb[4];
-#pragma unroll
-      for(int i = 0; i < 4; ++i) {
-        const uint8_t* p = &sA[(wm0 + i * 16 + l16) * RS + h * 128 + lk * 32];
+      auto frag = [&](const uint8_t* base, int row) {
+        const uint8_t* p  = base + (row + l16) * RS + h * 128 + lk * 32;
         const i32x4    lo = *reinterpret_cast<const i32x4*>(p);
         const i32x4    hi = *reinterpret_cast<const i32x4*>(p + 16);
-        fa[i] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-      }
+        return i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      };
+      i32x8 fa[4];
 #pragma unroll
-      for(int j = 0; j < 4; ++j) {
-        const uint8_t* p = &sB[(wn0 + j * 16 + l16) * RS + h * 128 + lk * 32];
-        const i32x4    lo = *reinterpret_cast<const i32x4*>(p);
-        const i32x4    hi = *reinterpret_cast<const i32x4*>(p + 16);
-        fb[j] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-      }
+      for(int i = 0; i < 4; ++i) fa[i] = frag(sA, wm0 + i * 16);
+      // B fragments one block column at a time (a wave of NI = 8 cannot
+      // hold all of them beside 4 x 8 accumulators)
 #pragma unroll
-      for(int i = 0; i < 4; ++i)
+      for(int j = 0; j < NI; ++j) {
+        const i32x8 fb = frag(sB, wn0 + j * 16);
 #pragma unroll
-        for(int j = 0; j < 4; ++j)
+        for(int i = 0; i < 4; ++i)
           // format 0 = e4m3 for both operands
           acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-              fb[j], fa[i], acc[i][j], 0, 0, 0, sb, 0, sa);
+              fb, fa[i], acc[i][j], 0, 0, 0, sb, 0, sa);
+      }
     }
     __syncthreads();
   }
@@ -603,7 +609,7 @@ __global__ __launch_bounds__(64 * WM * WN, 3) void gemm_fp8_kernel(
     const int row = m0 + wm0 + i * 16 + l16;
     if(row >= M) continue;
 #pragma unroll
-    for(int j = 0; j < 4; ++j)
+    for(int j = 0; j < NI; ++j)
 #pragma unroll
       for(int r = 0; r < 4; ++r) {
         const int col = n0 + wn0 + j * 16 + lk * 4 + r;
@@ -615,9 +621,10 @@ __global__ __launch_bounds__(64 * WM * WN, 3) void gemm_fp8_kernel(
   }
 }
 
-constexpr int kWM = 2, kWN = 2;  // 128 x 128 workgroup tiles
-// 128-deep FP8 stages per LDS fill: one (2.36 ms at 32768 x 32768 x 1024
-// with FP16 C; two stages per fill took 3.06 ms, four 4.71 ms)
+constexpr int kWM = 2, kWN = 2;  // 128 x 128 workgroup tiles (BF16)
+// FP8: 256 x 128 workgroup tiles of 4 waves, each 64 x 128 (two workgroups
+// per CU); 128-deep FP8 stages, one per LDS fill
+constexpr int kFp8WM = 4, kFp8WN = 1, kFp8NI = 8, kFp8MinW = 2;
 constexpr int kFp8Stages = 1;
 
 inline long tile_grid(int tiles_m, int tiles_n) {
@@ -667,13 +674,16 @@ template <typename S>
 hipError_t gemm_fp8_t(hipStream_t s, int m, int n, int K, const MxpPacked& P,
                       S* C, int ldc) {
   if(m <= 0 || n <= 0 || K <= 0) return hipSuccess;
-  constexpr int BM = 64 * kWM, BN = 64 * kWN;
+  constexpr int BM = 64 * kFp8WM, BN = 16 * kFp8NI * kFp8WN;
+  static_assert(kMxpRowPad % BM == 0 && kMxpRowPad % BN == 0,
+                "packed operands are padded to whole tiles");
   const int  tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
   const long grid    = tile_grid(tiles_m, tiles_n);
   if(grid > 0x7fffffffL) return hipErrorInvalidValue;
   const int nks = (K + 127) / 128;
-  const dim3 g(static_cast<unsigned>(grid)), b(64 * kWM * kWN);
-  hipLaunchKernelGGL((gemm_fp8_kernel<kWM, kWN, kFp8Stages, S>), g, b, 0, s, m, n, nks,
+  const dim3 g(static_cast<unsigned>(grid)), b(64 * kFp8WM * kFp8WN);
+  hipLaunchKernelGGL((gemm_fp8_kernel<kFp8WM, kFp8WN, kFp8NI, kFp8Stages, kFp8MinW, S>),
+                     g, b, 0, s, m, n, nks,
                      static_cast<const uint8_t*>(P.a), P.m_pad, P.ea,
                      static_cast<const uint8_t*>(P.b), P.n_pad, P.eb,
                      P.n_pad / 128, C, ldc, tiles_m, tiles_n,
