@@ -298,8 +298,9 @@ int hplx_update_timing(hplx_ctx* ctx, int kind, double* ms, int* launches,
  *     FP8 e4m3, on MFMA with FP32 accumulation,
  * followed by FP64 iterative refinement: r = b - A x with A and b
  * regenerated tile by tile in FP64 from the generator's seed (never stored),
- * the correction solved with the low-precision factors on FP64 vectors, and
- * x += d, until the HPL scaled residual is below the threshold.
+ * the correction d = LU \ r solved with the low-precision factors in FP32
+ * (solve_fp64 = 1: in FP64), and x += d in FP64, until the HPL scaled
+ * residual is below the threshold.
  * ==================================================================== */
 
 /* storage of the factors */
@@ -314,7 +315,8 @@ typedef struct hplx_mxp_opts {
   int    update;    /* HPLX_UPDATE_*                                        */
   int    terms;     /* BF16: 1 (plain), 2 (3 products) or 3 (6 products)  */
   int    max_its;   /* refinement solves allowed after the first (<= 64)    */
-  int    reserved;  /* 0                                                    */
+  int    solve_fp64; /* 0: solves with the factors in FP32 (FP64 only for
+                        r = b - A x, x += d and the norms); 1: in FP64      */
   double threshold; /* target HPL scaled residual (16.0)                    */
 } hplx_mxp_opts;
 
@@ -362,7 +364,8 @@ int hplx_mxp_pdgetrf(hplx_ctx* ctx, const hplx_palg* palg,
                      const hplx_mxp_opts* opts, hplx_mxp_pmat* mat);
 
 /* x := A^{-1} x with the stored low-precision factors, FP64 vector x
- * (device, n). */
+ * (device, n), solved in the precision the factorisation's opts chose
+ * (FP32: x is rounded to FP32, solved, and widened back). */
 int hplx_mxp_pdgetrs(hplx_ctx* ctx, const hplx_mxp_pmat* mat, double* x);
 
 /* The mixed-precision counterpart of hplx_pdgesv: factor, solve for the

@@ -82,7 +82,7 @@ class PMat(ctypes.Structure):
 class MxpOpts(ctypes.Structure):
     """hplx_mxp_opts (mixed-precision options)."""
     _fields_ = [("update", ctypes.c_int), ("terms", ctypes.c_int),
-                ("max_its", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("max_its", ctypes.c_int), ("solve_fp64", ctypes.c_int),
                 ("threshold", ctypes.c_double)]
 
 

@@ -1,7 +1,7 @@
 // Small kernels of the mixed-precision path: storage <-> FP32 block copies,
-// FP64 vector row interchanges (a panel's interchanges as net moves),
-// x += d, and the right-hand side b generated in FP64 from the matrix
-// generator (column n of [A|b], state i + n*n).
+// vector row interchanges (a panel's interchanges as net moves), FP64 <->
+// FP32 vector conversions, x += d, and the right-hand side b generated in
+// FP64 from the matrix generator (column n of [A|b], state i + n*n).
 #include "lcg.hpp"
 #include "mxp.hpp"
 
@@ -33,10 +33,11 @@ __global__ __launch_bounds__(256) void copy_block_kernel(
 
 // y[dst_i] = y[src_i] for the cnt pairs (one workgroup; every source is read
 // before any destination is written).
+template <typename V>
 __global__ __launch_bounds__(1024) void apply_swaps_kernel(
-    int cnt, const int* __restrict__ pairs, double* __restrict__ y) {
+    int cnt, const int* __restrict__ pairs, V* __restrict__ y) {
   constexpr int kPer = (kMaxSwapPairs + 1023) / 1024;
-  double        v[kPer];
+  V             v[kPer];
 #pragma unroll
   for(int q = 0; q < kPer; ++q) {
     const int i = threadIdx.x + 1024 * q;
@@ -50,11 +51,20 @@ __global__ __launch_bounds__(1024) void apply_swaps_kernel(
   }
 }
 
+template <typename D>
 __global__ __launch_bounds__(256) void axpy_kernel(int n,
-                                                   const double* __restrict__ d,
+                                                   const D* __restrict__ d,
                                                    double* __restrict__ x) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if(i < n) x[i] += d[i];
+  if(i < n) x[i] += static_cast<double>(d[i]);
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void convert_kernel(int n,
+                                                      const S* __restrict__ x,
+                                                      D* __restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i < n) y[i] = static_cast<D>(x[i]);
 }
 
 __global__ __launch_bounds__(256) void gen_b_kernel(int n, uint64_t seed,
@@ -97,20 +107,61 @@ hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
   return copy_t(s, m, n, A, lda, B, ldb);
 }
 
-hipError_t launch_apply_swaps(hipStream_t s, int cnt, const int* pairs,
-                              double* y) {
+namespace {
+template <typename V>
+hipError_t apply_swaps_t(hipStream_t s, int cnt, const int* pairs, V* y) {
   if(cnt <= 0) return hipSuccess;
   if(cnt > kMaxSwapPairs) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(apply_swaps_kernel, dim3(1), dim3(1024), 0, s, cnt, pairs,
-                     y);
+  hipLaunchKernelGGL(apply_swaps_kernel<V>, dim3(1), dim3(1024), 0, s, cnt,
+                     pairs, y);
   return hipGetLastError();
 }
+}  // namespace
+
+hipError_t launch_apply_swaps(hipStream_t s, int cnt, const int* pairs,
+                              double* y) {
+  return apply_swaps_t(s, cnt, pairs, y);
+}
+
+hipError_t launch_apply_swaps(hipStream_t s, int cnt, const int* pairs,
+                              float* y) {
+  return apply_swaps_t(s, cnt, pairs, y);
+}
+
+namespace {
+template <typename D>
+hipError_t axpy_t(hipStream_t s, int n, const D* d, double* x) {
+  if(n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(axpy_kernel<D>, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                     d, x);
+  return hipGetLastError();
+}
+}  // namespace
 
 hipError_t launch_axpy(hipStream_t s, int n, const double* d, double* x) {
+  return axpy_t(s, n, d, x);
+}
+
+hipError_t launch_axpy(hipStream_t s, int n, const float* d, double* x) {
+  return axpy_t(s, n, d, x);
+}
+
+namespace {
+template <typename S, typename D>
+hipError_t convert_t(hipStream_t s, int n, const S* x, D* y) {
   if(n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d,
-                     x);
+  hipLaunchKernelGGL((convert_kernel<S, D>), dim3((n + 255) / 256), dim3(256),
+                     0, s, n, x, y);
   return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_to_f32(hipStream_t s, int n, const double* x, float* y) {
+  return convert_t(s, n, x, y);
+}
+
+hipError_t launch_to_f64(hipStream_t s, int n, const float* y, double* x) {
+  return convert_t(s, n, y, x);
 }
 
 hipError_t launch_gen_b(hipStream_t s, int n, int seed, double* y) {

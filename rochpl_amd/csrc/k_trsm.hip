@@ -127,17 +127,18 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(
       k, T, ldt, X + static_cast<size_t>(blockIdx.x) * ldx, blockIdx.x);
 }
 
-// FP64 inverses of consecutive kb x kb diagonal blocks of T: block J
-// (blockIdx.y) starts at T[J kb, J kb]; column blockIdx.x of its inverse goes
-// to Inv + (J kb + col) kb (each inverse column-major with leading dimension
-// kb).  LOWER: unit lower triangle; else the upper triangle.
-template <typename MT, bool LOWER>
+// Inverses (in V: FP64 or FP32) of consecutive kb x kb diagonal blocks of T:
+// block J (blockIdx.y) starts at T[J kb, J kb]; column blockIdx.x of its
+// inverse goes to Inv + (J kb + col) kb (each inverse column-major with
+// leading dimension kb).  LOWER: unit lower triangle; else the upper
+// triangle.
+template <typename MT, typename V, bool LOWER>
 __global__ __launch_bounds__(256) void diag_inv_kernel(int kb,
                                                        const MT* __restrict__ T,
                                                        int ldt,
-                                                       double* __restrict__ Inv) {
+                                                       V* __restrict__ Inv) {
   const size_t J = blockIdx.y;
-  tri_solve_col<MT, double, LOWER, false, LOWER, true>(
+  tri_solve_col<MT, V, LOWER, false, LOWER, true>(
       kb, T + J * kb * (1 + static_cast<size_t>(ldt)), ldt,
       Inv + (J * kb + blockIdx.x) * kb, blockIdx.x);
 }
@@ -195,16 +196,16 @@ hipError_t launch_linv(hipStream_t s, int jb, const float* L, int ldl,
   return linv_t(s, jb, L, ldl, Linv, ldi);
 }
 
-template <typename MT>
+template <typename MT, typename V>
 hipError_t diag_inverse_t(hipStream_t s, bool lower, int nblk, int kb,
-                          const MT* T, int ldt, double* Inv) {
+                          const MT* T, int ldt, V* Inv) {
   if(nblk <= 0 || kb <= 0) return hipSuccess;
   if(kb > kMaxNB || nblk > 65535) return hipErrorInvalidValue;
   const dim3 g(kb, nblk);
   if(lower)
-    hipLaunchKernelGGL((diag_inv_kernel<MT, true>), g, dim3(256), 0, s, kb, T, ldt, Inv);
+    hipLaunchKernelGGL((diag_inv_kernel<MT, V, true>), g, dim3(256), 0, s, kb, T, ldt, Inv);
   else
-    hipLaunchKernelGGL((diag_inv_kernel<MT, false>), g, dim3(256), 0, s, kb, T, ldt, Inv);
+    hipLaunchKernelGGL((diag_inv_kernel<MT, V, false>), g, dim3(256), 0, s, kb, T, ldt, Inv);
   return hipGetLastError();
 }
 
@@ -215,6 +216,16 @@ hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
 
 hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
                                const __half* T, int ldt, double* Inv) {
+  return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
+}
+
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const float* T, int ldt, float* Inv) {
+  return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
+}
+
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const __half* T, int ldt, float* Inv) {
   return diag_inverse_t(s, lower, nblk, kb, T, ldt, Inv);
 }
 

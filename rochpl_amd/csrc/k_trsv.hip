@@ -15,25 +15,26 @@ namespace {
 
 typedef double ud4 __attribute__((ext_vector_type(4)));
 
-template <typename MT>
-__device__ inline double to_d(MT x) {
+template <typename V, typename MT>
+__device__ inline V to_v(MT x) {
   if constexpr(sizeof(MT) == 2)
-    return static_cast<double>(__half2float(x));
+    return static_cast<V>(__half2float(x));
   else
-    return static_cast<double>(x);
+    return static_cast<V>(x);
 }
 
-// y[0:m] -= U[0:m, 0:kb] * xs[0:kb] in FP64, U of element type MT (double, or
-// the FP32 / FP16 factors of the mixed-precision solves).  Workgroup = 64
-// rows; wave w takes a quarter of the columns; partial sums meet in LDS.
-// xback (optional): workgroup 0 also copies xs[0:kb] there.
-template <typename MT>
+// y[0:m] -= U[0:m, 0:kb] * xs[0:kb] in the vectors' type V (FP64, or FP32 in
+// the mixed-precision correction solves), U of element type MT (double, or
+// the FP32 / FP16 factors of the mixed forms).  Workgroup = 64 rows; wave w
+// takes a quarter of the columns; partial sums meet in LDS.  xback
+// (optional): workgroup 0 also copies xs[0:kb] there.
+template <typename MT, typename V>
 __global__ __launch_bounds__(256) void gemv_sub_kernel(
     int m, int kb, const MT* __restrict__ U, int ldu,
-    const double* __restrict__ xs, double* __restrict__ y,
-    double* __restrict__ xback) {
-  __shared__ double sxs[kMaxNB];
-  __shared__ double part[4][64];
+    const V* __restrict__ xs, V* __restrict__ y,
+    V* __restrict__ xback) {
+  __shared__ V sxs[kMaxNB];
+  __shared__ V part[4][64];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   for(int i = t; i < kb; i += blockDim.x) sxs[i] = xs[i];
   __syncthreads();
@@ -43,15 +44,15 @@ __global__ __launch_bounds__(256) void gemv_sub_kernel(
   const int ck  = (kb + 3) / 4;
   const int c0  = wave * ck;
   const int c1  = min(kb, c0 + ck);
-  double    acc = 0.0;
+  V         acc = 0;
   if(row < m) {
     const MT* u = U + row + static_cast<size_t>(c0) * ldu;
     int c = c0;
     for(; c + 4 <= c1; c += 4) {
-      const double u0 = to_d(u[0]);
-      const double u1 = to_d(u[static_cast<size_t>(ldu)]);
-      const double u2 = to_d(u[2 * static_cast<size_t>(ldu)]);
-      const double u3 = to_d(u[3 * static_cast<size_t>(ldu)]);
+      const V u0 = to_v<V>(u[0]);
+      const V u1 = to_v<V>(u[static_cast<size_t>(ldu)]);
+      const V u2 = to_v<V>(u[2 * static_cast<size_t>(ldu)]);
+      const V u3 = to_v<V>(u[3 * static_cast<size_t>(ldu)]);
       acc = fma(u0, sxs[c], acc);
       acc = fma(u1, sxs[c + 1], acc);
       acc = fma(u2, sxs[c + 2], acc);
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(256) void gemv_sub_kernel(
       u += 4 * static_cast<size_t>(ldu);
     }
     for(; c < c1; ++c) {
-      acc = fma(to_d(u[0]), sxs[c], acc);
+      acc = fma(to_v<V>(u[0]), sxs[c], acc);
       u += ldu;
     }
   }
@@ -69,15 +70,17 @@ __global__ __launch_bounds__(256) void gemv_sub_kernel(
     y[row] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
-// y[0:k] = Inv * x[0:k] for a k x k FP64 matrix (column-major, leading
-// dimension k), k <= kMaxNB.  Workgroup = 64 rows with 16 waves, each wave a
-// sixteenth of the columns in chunks of 32 whose loads are all in flight at
-// once (the matrix is small: the kernel is latency bound).
+// y[0:k] = Inv * x[0:k] for a k x k matrix of type V (FP64 or FP32;
+// column-major, leading dimension k), k <= kMaxNB.  Workgroup = 64 rows with
+// 16 waves, each wave a sixteenth of the columns in chunks of 32 whose loads
+// are all in flight at once (the matrix is small: the kernel is latency
+// bound).
+template <typename V>
 __global__ __launch_bounds__(1024) void inv_gemv_kernel(
-    int k, const double* __restrict__ Inv, const double* __restrict__ x,
-    double* __restrict__ y) {
-  __shared__ double sx[kMaxNB];
-  __shared__ double part[16][64];
+    int k, const V* __restrict__ Inv, const V* __restrict__ x,
+    V* __restrict__ y) {
+  __shared__ V sx[kMaxNB];
+  __shared__ V part[16][64];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   for(int i = t; i < k; i += blockDim.x) sx[i] = x[i];
   __syncthreads();
@@ -85,14 +88,14 @@ __global__ __launch_bounds__(1024) void inv_gemv_kernel(
   const int ck  = (k + 15) / 16;
   const int c0  = wave * ck;
   const int c1  = min(k, c0 + ck);
-  double    acc = 0.0;
+  V         acc = 0;
   if(row < k) {
     for(int cc = c0; cc < c1; cc += 32) {
-      const double* p = Inv + row + static_cast<size_t>(cc) * k;
-      double        v[32];
+      const V* p = Inv + row + static_cast<size_t>(cc) * k;
+      V             v[32];
 #pragma unroll
       for(int j = 0; j < 32; ++j)
-        v[j] = cc + j < c1 ? p[static_cast<size_t>(j) * k] : 0.0;
+        v[j] = cc + j < c1 ? p[static_cast<size_t>(j) * k] : V(0);
 #pragma unroll
       for(int j = 0; j < 32; ++j)
         if(cc + j < c1) acc = fma(v[j], sx[cc + j], acc);
@@ -101,7 +104,7 @@ __global__ __launch_bounds__(1024) void inv_gemv_kernel(
   part[wave][lane] = acc;
   __syncthreads();
   if(wave == 0 && row < k) {
-    double sum = 0.0;
+    V sum = 0;
 #pragma unroll
     for(int w = 0; w < 16; ++w) sum += part[w][lane];
     y[row] = sum;
@@ -223,12 +226,12 @@ __global__ __launch_bounds__(256) void tn_reduce_kernel(
   C[i + static_cast<size_t>(j) * ldc] -= sum;
 }
 
-template <typename MT>
+template <typename MT, typename V>
 hipError_t gemv_sub_t(hipStream_t s, int m, int k, const MT* M, int ldm,
-                      const double* x, double* y, double* xback) {
+                      const V* x, V* y, V* xback) {
   if(m <= 0 || k <= 0) return hipSuccess;
   if(k > kMaxNB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((gemv_sub_kernel<MT>), dim3((m + 63) / 64), dim3(256), 0,
+  hipLaunchKernelGGL((gemv_sub_kernel<MT, V>), dim3((m + 63) / 64), dim3(256), 0,
                      s, m, k, M, ldm, x, y, xback);
   return hipGetLastError();
 }
@@ -238,6 +241,16 @@ hipError_t gemv_sub_t(hipStream_t s, int m, int k, const MT* M, int ldm,
 hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const float* M,
                            int ldm, const double* x, double* y,
                            double* xback) {
+  return gemv_sub_t(s, m, k, M, ldm, x, y, xback);
+}
+
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const float* M,
+                           int ldm, const float* x, float* y, float* xback) {
+  return gemv_sub_t(s, m, k, M, ldm, x, y, xback);
+}
+
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const __half* M,
+                           int ldm, const float* x, float* y, float* xback) {
   return gemv_sub_t(s, m, k, M, ldm, x, y, xback);
 }
 
@@ -253,13 +266,23 @@ hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const double* M,
   return gemv_sub_t(s, m, k, M, ldm, x, y, xback);
 }
 
-hipError_t launch_inv_gemv(hipStream_t s, int k, const double* Inv,
-                           const double* x, double* y) {
+template <typename V>
+hipError_t inv_gemv_t(hipStream_t s, int k, const V* Inv, const V* x, V* y) {
   if(k <= 0) return hipSuccess;
   if(k > kMaxNB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(inv_gemv_kernel, dim3((k + 63) / 64), dim3(1024), 0, s, k,
-                     Inv, x, y);
+  hipLaunchKernelGGL(inv_gemv_kernel<V>, dim3((k + 63) / 64), dim3(1024), 0, s,
+                     k, Inv, x, y);
   return hipGetLastError();
+}
+
+hipError_t launch_inv_gemv(hipStream_t s, int k, const double* Inv,
+                           const double* x, double* y) {
+  return inv_gemv_t(s, k, Inv, x, y);
+}
+
+hipError_t launch_inv_gemv(hipStream_t s, int k, const float* Inv,
+                           const float* x, float* y) {
+  return inv_gemv_t(s, k, Inv, x, y);
 }
 
 hipError_t launch_tn_sub(hipStream_t s, int K, int m, int n, const double* A,
@@ -332,7 +355,7 @@ hipError_t launch_trsv(hipStream_t s, int n, int nb, const double* A, int lda,
                                     lda, x + s0, n);
     if(e != hipSuccess) return e;
     if(s0 > 0)
-      hipLaunchKernelGGL(gemv_sub_kernel<double>, dim3((s0 + 63) / 64),
+      hipLaunchKernelGGL((gemv_sub_kernel<double, double>), dim3((s0 + 63) / 64),
                          dim3(256), 0, s, s0, kb,
                          A + static_cast<size_t>(s0) * lda, lda, x + s0, x,
                          nullptr);

@@ -58,8 +58,10 @@ hipError_t launch_copy_block(hipStream_t s, int m, int n, const double* A,
 constexpr int kMaxSwapPairs = 2 * kMaxNB;
 hipError_t launch_apply_swaps(hipStream_t s, int cnt, const int* pairs,
                               double* y);
-// y[0:m] -= M[0:m, 0:k] * x[0:k], M in storage type, FP64 accumulation;
-// if xback is not null, x[0:k] is also copied there.
+hipError_t launch_apply_swaps(hipStream_t s, int cnt, const int* pairs,
+                              float* y);
+// y[0:m] -= M[0:m, 0:k] * x[0:k], M in storage type, accumulation in the
+// vectors' type; if xback is not null, x[0:k] is also copied there.
 hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const float* M,
                            int ldm, const double* x, double* y,
                            double* xback = nullptr);
@@ -69,15 +71,28 @@ hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const __half* M,
 hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const double* M,
                            int ldm, const double* x, double* y,
                            double* xback = nullptr);
-// FP64 inverses of nblk consecutive kb x kb diagonal blocks of T (unit lower
-// triangle or upper triangle), block J at Inv + J kb kb, leading dimension kb.
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const float* M,
+                           int ldm, const float* x, float* y,
+                           float* xback = nullptr);
+hipError_t launch_gemv_sub(hipStream_t s, int m, int k, const __half* M,
+                           int ldm, const float* x, float* y,
+                           float* xback = nullptr);
+// Inverses (FP64 or FP32) of nblk consecutive kb x kb diagonal blocks of T
+// (unit lower triangle or upper triangle), block J at Inv + J kb kb, leading
+// dimension kb.
 hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
                                const float* T, int ldt, double* Inv);
 hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
                                const __half* T, int ldt, double* Inv);
-// y[0:k] = Inv x[0:k], Inv k x k FP64 (leading dimension k).
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const float* T, int ldt, float* Inv);
+hipError_t launch_diag_inverse(hipStream_t s, bool lower, int nblk, int kb,
+                               const __half* T, int ldt, float* Inv);
+// y[0:k] = Inv x[0:k], Inv k x k (leading dimension k).
 hipError_t launch_inv_gemv(hipStream_t s, int k, const double* Inv,
                            const double* x, double* y);
+hipError_t launch_inv_gemv(hipStream_t s, int k, const float* Inv,
+                           const float* x, float* y);
 
 // The generator's matrix rounded to FP32 / FP16 storage.
 hipError_t launch_matgen(hipStream_t s, int m, int n, float* A, int lda,
@@ -96,5 +111,11 @@ hipError_t launch_ir_residual(hipStream_t s, int n, int seed,
                               double* part, double* out);
 // x += d
 hipError_t launch_axpy(hipStream_t s, int n, const double* d, double* x);
+// x += d with an FP32 correction d (the sum in FP64)
+hipError_t launch_axpy(hipStream_t s, int n, const float* d, double* x);
+// y = (FP32) x, rounded to nearest
+hipError_t launch_to_f32(hipStream_t s, int n, const double* x, float* y);
+// x = (FP64) y
+hipError_t launch_to_f64(hipStream_t s, int n, const float* y, double* x);
 
 }  // namespace hplx

@@ -11,10 +11,12 @@
 //   5. U12 = L11^{-1} W in FP32, stored back in S
 //   6. A22 -= L21 U12 in BF16 / FP8 (or FP32) on MFMA, FP32 accumulation
 // b is not carried through the loop.  After the loop the diagonal blocks of
-// L and U are inverted in FP64.  The solve then runs in FP64 on the
-// device: x = U^{-1} L^{-1} P b with b generated in FP64 from the seed, and
-// refinement r = b - A x (A regenerated in FP64 tile by tile), d = LU \ r,
-// x += d until the HPL scaled residual is below the threshold.
+// L and U are inverted in FP32 (FP64 when the caller asks for an FP64
+// correction solve).  The solves run on the device in that precision with
+// the stored factors: x = U^{-1} L^{-1} P b with b generated in FP64 from
+// the seed and rounded to FP32, then refinement in FP64 -- r = b - A x (A
+// regenerated in FP64 tile by tile) and the residual norm -- with the
+// correction d = LU \ r solved in FP32 and x += d accumulated in FP64.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -47,11 +49,14 @@ struct MxpWs {
   int*             swp = nullptr;  // device, up to 2n pairs
   std::vector<int> swp_off;
   bool             swp_ready = false;
-  // FP64 inverses of the diagonal blocks of L (unit lower) and U, block J at
-  // J nb nb (leading dimension = the block's size), and a block-sized vector
-  double* dinv_l = nullptr;
-  double* dinv_u = nullptr;
-  double* ytmp   = nullptr;
+  // inverses of the diagonal blocks of L (unit lower) and U, block J at
+  // J nb nb (leading dimension = the block's size), and a block-sized
+  // vector: FP32 values, or FP64 with solve_fp64 (room for FP64 either way)
+  void* dinv_l = nullptr;
+  void* dinv_u = nullptr;
+  void* ytmp   = nullptr;
+  float* x32   = nullptr;  // the FP32 vector of an FP32 solve, n
+  bool  solve_fp64 = false;  // precision of the solves (set by the factorisation)
 };
 
 int hpl_ld(int n) { return ((n > 0 ? n : 0) + 95) / 128 * 128 + 32; }
@@ -72,6 +77,8 @@ int check_opts(Ctx& c, const hplx_mxp_opts* o, int storage) {
   }
   if(o->max_its < 0 || o->max_its > 64)
     return fail(c, HPLX_ERR_INVALID, "max_its %d outside 0..64", o->max_its);
+  if(o->solve_fp64 != 0 && o->solve_fp64 != 1)
+    return fail(c, HPLX_ERR_INVALID, "solve_fp64 %d (0 or 1)", o->solve_fp64);
   if(!(o->threshold > 0.0)) return fail(c, HPLX_ERR_INVALID, "threshold <= 0");
   return HPLX_SUCCESS;
 }
@@ -87,15 +94,15 @@ int build_swaps(Ctx& c, const hplx_mxp_pmat* mat);
 
 // Inverses of the diagonal blocks of the stored factors (all final once the
 // factorisation is done: later interchanges do not touch earlier columns).
-template <typename S>
-int build_inverses(Ctx& c, const hplx_mxp_pmat* mat) {
+template <typename S, typename V>
+int build_inverses_t(Ctx& c, const hplx_mxp_pmat* mat) {
   MxpWs*    ws = static_cast<MxpWs*>(mat->ws);
   const int N = mat->n, nb = mat->nb, lda = mat->ld;
   const S*  A     = static_cast<const S*>(mat->A);
   const int nfull = N / nb, rest = N - nfull * nb;
   const size_t tail = static_cast<size_t>(nfull) * nb * nb;
   for(const bool lower : {true, false}) {
-    double* inv = lower ? ws->dinv_l : ws->dinv_u;
+    V* inv = static_cast<V*>(lower ? ws->dinv_l : ws->dinv_u);
     HIPCHK(c, hplx::launch_diag_inverse(c.stream, lower, nfull, nb, A, lda, inv));
     if(rest > 0)
       HIPCHK(c, hplx::launch_diag_inverse(
@@ -104,6 +111,13 @@ int build_inverses(Ctx& c, const hplx_mxp_pmat* mat) {
                     lda, inv + tail));
   }
   return HPLX_SUCCESS;
+}
+
+template <typename S>
+int build_inverses(Ctx& c, const hplx_mxp_pmat* mat) {
+  return static_cast<MxpWs*>(mat->ws)->solve_fp64
+             ? build_inverses_t<S, double>(c, mat)
+             : build_inverses_t<S, float>(c, mat);
 }
 
 template <typename S>
@@ -254,12 +268,13 @@ int build_swaps(Ctx& c, const hplx_mxp_pmat* mat) {
   return HPLX_SUCCESS;
 }
 
-// x = U^{-1} L^{-1} P x in FP64 with the stored factors.  Each diagonal
-// block is applied as its FP64 inverse (one bandwidth-bound product) into
-// ytmp, then the block's columns below (L) or above (U) are applied from
-// ytmp, the same launch copying ytmp back into x.
-template <typename S>
-int solve(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
+// x = U^{-1} L^{-1} P x with the stored factors, in the vectors' precision
+// V (FP32, or FP64 with solve_fp64).  Each diagonal block is applied as its
+// inverse (one bandwidth-bound product) into ytmp, then the block's columns
+// below (L) or above (U) are applied from ytmp, the same launch copying
+// ytmp back into x.
+template <typename S, typename V>
+int solve(Ctx& c, const hplx_mxp_pmat* mat, V* x) {
   const int   N = mat->n, nb = mat->nb, lda = mat->ld;
   const S*    A = static_cast<const S*>(mat->A);
   hipStream_t s = c.stream;
@@ -268,21 +283,23 @@ int solve(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
     const int r = build_swaps(c, mat);
     if(r) return r;
   }
-  double* y = ws->ytmp;
+  V*       y    = static_cast<V*>(ws->ytmp);
+  const V* invl = static_cast<const V*>(ws->dinv_l);
+  const V* invu = static_cast<const V*>(ws->dinv_u);
   // L y = P b, panel by panel (each panel's interchanges, then its L11 and
   // L21; L is left unpivoted by later panels)
   for(int j = 0; j < N; j += nb) {
     const int jb = (N - j) < nb ? (N - j) : nb;
     const int p0 = ws->swp_off[j / nb], p1 = ws->swp_off[j / nb + 1];
     HIPCHK(c, hplx::launch_apply_swaps(s, p1 - p0, ws->swp + 2 * p0, x));
-    HIPCHK(c, hplx::launch_inv_gemv(s, jb, ws->dinv_l + static_cast<size_t>(j) * nb,
+    HIPCHK(c, hplx::launch_inv_gemv(s, jb, invl + static_cast<size_t>(j) * nb,
                                     x + j, y));
     if(N - j - jb > 0)
       HIPCHK(c, hplx::launch_gemv_sub(s, N - j - jb, jb,
                                       A + j + jb + static_cast<size_t>(j) * lda,
                                       lda, y, x + j + jb, x + j));
     else
-      HIPCHK(c, hipMemcpyAsync(x + j, y, sizeof(double) * jb,
+      HIPCHK(c, hipMemcpyAsync(x + j, y, sizeof(V) * jb,
                                hipMemcpyDeviceToDevice, s));
   }
   // U x = y, blocks from the bottom
@@ -290,21 +307,33 @@ int solve(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
   for(int J = nblk - 1; J >= 0; --J) {
     const int s0 = J * nb;
     const int kb = (N - s0) < nb ? (N - s0) : nb;
-    HIPCHK(c, hplx::launch_inv_gemv(s, kb, ws->dinv_u + static_cast<size_t>(s0) * nb,
+    HIPCHK(c, hplx::launch_inv_gemv(s, kb, invu + static_cast<size_t>(s0) * nb,
                                     x + s0, y));
     if(s0 > 0)
       HIPCHK(c, hplx::launch_gemv_sub(s, s0, kb, A + static_cast<size_t>(s0) * lda,
                                       lda, y, x, x + s0));
     else
-      HIPCHK(c, hipMemcpyAsync(x + s0, y, sizeof(double) * kb,
+      HIPCHK(c, hipMemcpyAsync(x + s0, y, sizeof(V) * kb,
                                hipMemcpyDeviceToDevice, s));
   }
   return HPLX_SUCCESS;
 }
 
+// x (FP64, in and out) = (LU)^{-1} P x: in FP64, or rounded to FP32, solved
+// in FP32 and widened back.
+template <typename S>
+int solve_vec(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
+  MxpWs* ws = static_cast<MxpWs*>(mat->ws);
+  if(ws->solve_fp64) return solve<S, double>(c, mat, x);
+  HIPCHK(c, hplx::launch_to_f32(c.stream, mat->n, x, ws->x32));
+  if(const int r = solve<S, float>(c, mat, ws->x32)) return r;
+  HIPCHK(c, hplx::launch_to_f64(c.stream, mat->n, ws->x32, x));
+  return HPLX_SUCCESS;
+}
+
 int solve_any(Ctx& c, const hplx_mxp_pmat* mat, double* x) {
-  return mat->storage == HPLX_STORE_FP16 ? solve<__half>(c, mat, x)
-                                         : solve<float>(c, mat, x);
+  return mat->storage == HPLX_STORE_FP16 ? solve_vec<__half>(c, mat, x)
+                                         : solve_vec<float>(c, mat, x);
 }
 
 int factor_any(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
@@ -313,6 +342,7 @@ int factor_any(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
   if((r = hplx::check_palg(c, palg))) return r;
   if((r = check_mat(c, mat))) return r;
   if((r = check_opts(c, o, mat->storage))) return r;
+  static_cast<MxpWs*>(mat->ws)->solve_fp64 = o->solve_fp64 != 0;
   return mat->storage == HPLX_STORE_FP16 ? factor_loop<__half>(c, palg, o, mat)
                                          : factor_loop<float>(c, palg, o, mat);
 }
@@ -335,7 +365,7 @@ void hplx_mxp_opts_default(hplx_mxp_opts* o) {
   o->update    = HPLX_UPDATE_BF16;
   o->terms     = 3;
   o->max_its   = 50;
-  o->reserved  = 0;
+  o->solve_fp64 = 0;
   o->threshold = 16.0;
 }
 
@@ -374,9 +404,10 @@ int hplx_mxp_alloc(hplx_ctx* ctx, int n, int nb, int storage,
             hipMalloc(reinterpret_cast<void**>(&ws->part), 8 * hplx::ir_part_len(n)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&ws->out), 16 * 8) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&ws->swp), 16 * static_cast<size_t>(n)) == hipSuccess &&
-            hipMalloc(reinterpret_cast<void**>(&ws->dinv_l), 8 * dinv) == hipSuccess &&
-            hipMalloc(reinterpret_cast<void**>(&ws->dinv_u), 8 * dinv) == hipSuccess &&
-            hipMalloc(reinterpret_cast<void**>(&ws->ytmp), 8 * static_cast<size_t>(nb)) == hipSuccess;
+            hipMalloc(&ws->dinv_l, 8 * dinv) == hipSuccess &&
+            hipMalloc(&ws->dinv_u, 8 * dinv) == hipSuccess &&
+            hipMalloc(&ws->ytmp, 8 * static_cast<size_t>(nb)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&ws->x32), 4 * static_cast<size_t>(n)) == hipSuccess;
   if(ok && storage == HPLX_STORE_FP16) {
     ws->ldp = static_cast<int>(hplx::align_up(n, 64));
     ok = hipMalloc(reinterpret_cast<void**>(&ws->P32), 4 * static_cast<size_t>(ws->ldp) * nb) == hipSuccess &&
@@ -400,8 +431,8 @@ int hplx_mxp_free(hplx_ctx* ctx, hplx_mxp_pmat* mat) {
                    ws->pb, static_cast<void*>(ws->ea), static_cast<void*>(ws->eb),
                    static_cast<void*>(ws->r), static_cast<void*>(ws->part),
                    static_cast<void*>(ws->out), static_cast<void*>(ws->swp),
-                   static_cast<void*>(ws->dinv_l), static_cast<void*>(ws->dinv_u),
-                   static_cast<void*>(ws->ytmp)})
+                   ws->dinv_l, ws->dinv_u, ws->ytmp,
+                   static_cast<void*>(ws->x32)})
       if(p) (void)hipFree(p);
     delete ws;
   }

@@ -204,6 +204,7 @@ def test_spdfact_is_partial_pivoting_lu(ctx, m, jb, nbmin):
 # ---------------------------------------------------------------------------
 # solves with the stored low-precision factors, refinement residual
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("solve_fp64", [1, 0])
 @pytest.mark.parametrize("storage,update,n,nb,shift", [
     (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_FP32, 2048, 256, 0.0),
     (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1536, 512, 384.0),
@@ -211,10 +212,18 @@ def test_spdfact_is_partial_pivoting_lu(ctx, m, jb, nbmin):
     (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_FP32, 1000, 128, 0.0),
     (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1300, 256, 325.0),
 ])
-def test_mxp_getrs_and_residual(ctx, storage, update, n, nb, shift):
-    say("mxp getrs storage=%d n=%d nb=%d" % (storage, n, nb))
+def test_mxp_getrs_and_residual(ctx, storage, update, n, nb, shift,
+                                solve_fp64):
+    """The solve with the stored factors: in FP64 it equals the same solve on
+    the CPU to rounding; in FP32 (the default) it is backward stable, its x
+    solving L U x = P b to within a multiple of n * eps32 relative to
+    |L| |U| |x|, a bound an indexing or interchange mistake exceeds by
+    orders of magnitude."""
+    say("mxp getrs storage=%d n=%d nb=%d solve_fp64=%d"
+        % (storage, n, nb, solve_fp64))
     mat = ctx.mxp_alloc(n, nb, storage)
     opts = rp.mxp_opts_default(update, 1)
+    opts.solve_fp64 = solve_fp64
     try:
         ctx.mxp_matgen(mat, rp.HPL_ISEED, shift)
         # the stored matrix is the generator's, rounded to the storage type
@@ -243,9 +252,27 @@ def test_mxp_getrs_and_residual(ctx, storage, update, n, nb, shift):
         d = upload(ctx, b)
         ctx.mxp_pdgetrs(mat, d.ptr)
         x = d.download((n,), np.float64)
-        # the same solve on the CPU in FP64 with the downloaded factors
-        xr = hpl_lu_solve(F, ipiv, nb, b)
-        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+        if solve_fp64:
+            # the same solve on the CPU in FP64 with the downloaded factors
+            xr = hpl_lu_solve(F, ipiv, nb, b)
+            assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+        else:
+            # P b with HPL's panel-wise interchanges, then the backward error
+            # of the FP32 solve, all in FP64 on the CPU
+            pb = np.array(b, dtype=np.float64)
+            for i in range(n):
+                pb[i], pb[ipiv[i]] = pb[ipiv[i]], pb[i]
+            L = np.tril(F, -1) + np.eye(n)
+            U = np.triu(F)
+            res_b = np.abs(pb - L @ (U @ x))
+            scale = (np.abs(L) @ (np.abs(U) @ np.abs(x))) + np.abs(pb)
+            rel = np.max(res_b / scale)
+            eps32 = np.finfo(np.float32).eps
+            print("[mxp] fp32 solve backward error %.3e (n eps32 %.3e)"
+                  % (rel, n * eps32), flush=True)
+            assert rel <= n * eps32
+            x32 = x.astype(np.float32).astype(np.float64)
+            assert np.array_equal(x, x32)  # an FP32 solution, widened
         # refinement residual against the FP64 oracle's
         res = ctx.mxp_residual(mat, d.ptr)
         ref = o.residual(n, x, diag_shift=shift)
@@ -272,11 +299,14 @@ def test_mxp_getrs_and_residual(ctx, storage, update, n, nb, shift):
     (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1, 4096, 1024, 1024.0),
     (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_BF16, 2, 2500, 256, 625.0),
 ])
-def test_mxp_pdgesv_converges(ctx, storage, update, terms, n, nb, shift):
-    say("mxp_pdgesv storage=%d update=%d terms=%d n=%d nb=%d shift=%g" %
-        (storage, update, terms, n, nb, shift))
+@pytest.mark.parametrize("solve_fp64", [0, 1])
+def test_mxp_pdgesv_converges(ctx, storage, update, terms, n, nb, shift,
+                              solve_fp64):
+    say("mxp_pdgesv storage=%d update=%d terms=%d n=%d nb=%d shift=%g "
+        "solve_fp64=%d" % (storage, update, terms, n, nb, shift, solve_fp64))
     mat = ctx.mxp_alloc(n, nb, storage)
     opts = rp.mxp_opts_default(update, terms)
+    opts.solve_fp64 = solve_fp64
     try:
         ctx.mxp_matgen(mat, rp.HPL_ISEED, shift)
         rc, res = ctx.mxp_pdgesv(rp.palg_default(), opts, mat)
@@ -318,5 +348,9 @@ def test_mxp_rejects_bad_options(ctx):
         with pytest.raises(rp.HplxError):
             ctx.mxp_pdgetrf(rp.palg_default(),
                             rp.mxp_opts_default(rp.HPLX_UPDATE_BF16, 4), mat)
+        bad = rp.mxp_opts_default(rp.HPLX_UPDATE_BF16, 2)
+        bad.solve_fp64 = 2
+        with pytest.raises(rp.HplxError):
+            ctx.mxp_pdgetrf(rp.palg_default(), bad, mat)
     finally:
         ctx.mxp_free(mat)
