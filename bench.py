@@ -417,7 +417,7 @@ def run_mixed(ctx, rp, form, args):
         "value": round(hpl_flops(n) / t / 1e12, 3), "ms": round(t * 1e3, 1),
         "config": {"workload": "hpl_mxp_%s_p1q1" % form, "N": n, "NB": nb,
                    "storage": storage, "update": update, "terms": terms,
-                   "panel": "fp32", "refinement": "fp64 classic",
+                   "panel": "fp32", "refinement": "fp64 residual and update, fp32 solves",
                    "diag_shift": shift},
         "refinement_its": res["its"], "residual": round(res["resid"], 6),
         "residual_history": [float("%.4g" % h) for h in res["history"]],
