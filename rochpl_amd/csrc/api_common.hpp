@@ -35,6 +35,21 @@ inline int fail(Ctx& c, int code, const char* fmt, ...) {
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Timing-disabled event i of the context's cross-stream pool (created on
+// first use).
+inline int sync_event(Ctx& c, size_t i, hipEvent_t* e) {
+  while(c.ev_sync.size() <= i) {
+    hipEvent_t x;
+    const hipError_t r = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+    if(r != hipSuccess)
+      return fail(c, HPLX_ERR_HIP, "hipEventCreateWithFlags: %s",
+                  hipGetErrorString(r));
+    c.ev_sync.push_back(x);
+  }
+  *e = c.ev_sync[i];
+  return HPLX_SUCCESS;
+}
+
 inline int use(hplx_ctx* ctx) {
   if(!ctx) return HPLX_ERR_INVALID;
   hipError_t e = hipSetDevice(ctx->c.device);

@@ -518,19 +518,8 @@ struct Fp64Loop {
 };
 
 // Event from the context's pool (timing disabled), for cross-stream order.
-static int sync_event(Ctx& c, size_t i, hipEvent_t* e) {
-  while(c.ev_sync.size() <= i) {
-    hipEvent_t x;
-    HIPCHK(c, hipEventCreateWithFlags(&x, hipEventDisableTiming));
-    c.ev_sync.push_back(x);
-  }
-  *e = c.ev_sync[i];
-  return HPLX_SUCCESS;
-}
-
-// Columns a look-ahead step needs right of the next panel before it
-// overlaps the rest of the update with that panel: fewer go serially.
-constexpr int kMinOverlapCols = 256;
+using hplx::sync_event;
+using hplx::kMinOverlapCols;
 
 // The panel loop over the first `ncols` columns of mat->A (n+1 with b for
 // hplx_pdgesv, n for hplx_pdgetrf).  Returns 0, an error code, or the
@@ -607,14 +596,8 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
       if((r = L.prepare(k, 0, la_cols, U, &fast)) ||
          (r = L.gemm(k, 0, la_cols, U, fast)))
         return r;
-      // the CUs the next panel's leaves need: one per 512 * RPT rows, taken
-      // evenly from the 8 XCDs and, within each, from all 4 shader engines
-      // (a workgroup waits for a CU of the engine it was handed to, so an
-      // engine without a free CU would hold up the panel's kernels)
-      const int rows1 = N - j - jb;
-      const int G1    = (rows1 + hplx::kLeafThreads * hplx::kLeafRPT - 1) /
-                     (hplx::kLeafThreads * hplx::kLeafRPT);
-      const int rR    = (G1 + 31) / 32 * 4;
+      // the CUs the next panel's leaves need
+      const int rR = hplx::vacate_set_size(N - j - jb);
       const bool overlap = la && jb1 > 0 && rest >= kMinOverlapCols &&
                            rR <= c.cus_per_xcd && rR < 28;
       if(overlap) {

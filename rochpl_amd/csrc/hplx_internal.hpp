@@ -19,6 +19,18 @@ namespace hplx {
 constexpr int kLeafMax     = 32;    // widest leaf the register-resident kernel takes
 constexpr int kLeafThreads = 512;   // threads per leaf workgroup
 constexpr int kLeafRPT     = 3;     // rows per thread in the leaf kernel
+
+// Look-ahead: columns a step needs right of the next panel before it overlaps
+// the rest of the update with that panel (fewer go serially), and the CUs
+// per XCD a panel of `rows` rows is given -- one per leaf workgroup (512 x
+// RPT rows each), taken evenly from the 8 XCDs and, within each, from all 4
+// shader engines (a workgroup waits for a CU of the engine it was handed to,
+// so an engine without a free CU would hold up the panel's kernels).
+constexpr int kMinOverlapCols = 256;
+inline int vacate_set_size(int rows) {
+  const int g = (rows + kLeafThreads * kLeafRPT - 1) / (kLeafThreads * kLeafRPT);
+  return (g + 31) / 32 * 4;
+}
 constexpr int kMaxLeafGrid = 256;   // workgroups a cooperative leaf may use
 constexpr int kMaxGrid     = kMaxLeafGrid;  // candidate slots per parity
 constexpr int kMaxNB       = 1024;  // widest panel (pivot composition, Linv)

@@ -40,7 +40,10 @@
 // MFMAs run; LDS is single-buffered (two barriers per stage).  Workgroup ids
 // are remapped XCD-aware as in the FP64 GEMM.
 #include <type_traits>
+#include <utility>
+#include <vector>
 
+#include "claim.hpp"
 #include "hplx_internal.hpp"
 #include "mxp.hpp"
 
@@ -294,13 +297,11 @@ __device__ inline void st_c(S* p, float v) {
   *p = static_cast<S>(v);
 }
 
-// XCD-aware tile id; false when the workgroup has no tile.
-__device__ inline bool map_tile(int tiles_m, int tiles_n, int& tm, int& tn) {
-  const int T    = tiles_m * tiles_n;
-  const int per  = gridDim.x / kXcds;
-  const int bid  = blockIdx.x;
-  const int tile = (bid % kXcds) * per + bid / kXcds;
-  if(tile >= T) return false;
+// Tile `tile` of the launch's order: groups of kGroupM tile rows walked
+// column by column.  Every schedule below hands XCD x the tiles
+// [x per, (x + 1) per) of this order (per = ceil(T / 8)).
+__device__ inline void tile_coords(int tile, int tiles_m, int tiles_n, int& tm,
+                                   int& tn) {
   const int width  = kGroupM * tiles_n;
   const int group  = tile / width;
   const int firstm = group * kGroupM;
@@ -308,7 +309,60 @@ __device__ inline bool map_tile(int tiles_m, int tiles_n, int& tm, int& tn) {
   const int within = tile - group * width;
   tm               = firstm + within % gsz;
   tn               = within / gsz;
-  return true;
+}
+
+// Launch modes of the update kernels: one tile per workgroup (XCD-aware),
+// the look-ahead's claimed mode (a persistent grid beside the panel
+// factorisation: workgroups on the vacated CUs exit, the others claim tiles
+// from their XCD's counter until the stop flag is set) and its resume mode
+// (one tile per workgroup from where each XCD's claims ended).
+enum : int { kModePlain = 0, kModeClaimed = 1, kModeResume = 2 };
+
+template <int MODE, typename F>
+__device__ __forceinline__ void run_tiles(int tiles_m, int tiles_n,
+                                          const UpdSched& sch,
+                                          const int* __restrict__ claimed,
+                                          F&& tile_fn) {
+  const int T = tiles_m * tiles_n;
+  int       tm, tn;
+  if constexpr(MODE == kModePlain) {
+    const int per  = gridDim.x / kXcds, bid = blockIdx.x;
+    const int tile = (bid % kXcds) * per + bid / kXcds;
+    if(tile >= T) return;
+    tile_coords(tile, tiles_m, tiles_n, tm, tn);
+    tile_fn(tm, tn);
+  } else if constexpr(MODE == kModeResume) {
+    const int x = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+    const int done = min(claimed[32 * x], sch.per);
+    const int tile = x * sch.per + done + i;
+    if(done + i >= sch.per || tile >= T) return;
+    tile_coords(tile, tiles_m, tiles_n, tm, tn);
+    tile_fn(tm, tn);
+  } else {
+    __shared__ int      s_tile;
+    __shared__ unsigned s_dead;
+    // all waves of a workgroup share one CU, so this exit is uniform
+    if(on_vacated_cu(sch)) return;
+    if(threadIdx.x == 0) {
+      unsigned dead = 0u;
+      s_tile        = claim_next_tile(sch, T, dead);
+      s_dead        = dead;
+    }
+    __syncthreads();
+    for(;;) {
+      const int tile = __builtin_amdgcn_readfirstlane(s_tile);
+      if(tile < 0) break;
+      tile_coords(tile, tiles_m, tiles_n, tm, tn);
+      tile_fn(tm, tn);
+      __syncthreads();  // every wave is done with the tile's LDS and s_tile
+      if(threadIdx.x == 0) {
+        unsigned dead = s_dead;
+        s_tile        = claim_next_tile(sch, T, dead);
+        s_dead        = dead;
+      }
+      __syncthreads();
+    }
+  }
 }
 
 // Epilogue C -= acc for one wave's 64 x 64 sub-tile.
@@ -378,28 +432,33 @@ __device__ inline void epilogue_vec(f32x4 (&acc)[4][NI], S* __restrict__ C,
 __device__ constexpr int kPairA[6] = {0, 0, 1, 0, 1, 2};
 __device__ constexpr int kPairB[6] = {0, 1, 0, 2, 1, 0};
 
+constexpr int kBf16RS = 48;  // LDS row stride in bf16 (96 bytes; 3 % faster than 80)
+
+// One LDS arena per workgroup: operand stages during the K loop, epilogue
+// scratch after.
+template <int TERMS, int WM, int WN>
+constexpr int bf16_smem_bytes() {
+  constexpr int ops = TERMS * (64 * WM + 64 * WN) * kBf16RS * 2;
+  return ops > epi_bytes(WM * WN) ? ops : epi_bytes(WM * WN);
+}
+
+// Tile (tm, tn) of C -= A B on BF16 MFMA.
 template <int TERMS, int WM, int WN, typename S>
-__global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
-    int M, int N, int nks, const uint16_t* __restrict__ Ap, int m_pad,
-    size_t aplane, const uint16_t* __restrict__ Bp, int n_pad, size_t bplane,
-    S* __restrict__ C, int ldc, int tiles_m, int tiles_n, int vec_ok) {
+__device__ __forceinline__ void bf16_tile(
+    int tm, int tn, int M, int N, int nks, const uint16_t* __restrict__ Ap,
+    int m_pad, size_t aplane, const uint16_t* __restrict__ Bp, int n_pad,
+    size_t bplane, S* __restrict__ C, int ldc, int vec_ok, uint8_t* smem) {
   constexpr int NT     = 64 * WM * WN;
   constexpr int BM     = 64 * WM;
   constexpr int BN     = 64 * WN;
-  constexpr int RS     = 48;  // LDS row stride in bf16 (96 bytes; 3 % faster than 80)
+  constexpr int RS     = kBf16RS;
   constexpr int NPROD  = TERMS * (TERMS + 1) / 2;
   constexpr int ACH    = BM * 4 / NT;  // 16-byte chunks per thread, A, per term
   constexpr int BCH    = BN * 4 / NT;
-  // one LDS arena: operand stages during the K loop, epilogue scratch after
-  constexpr int kOps   = TERMS * (BM + BN) * RS * 2;
-  __shared__ __attribute__((aligned(16)))
-      uint8_t smem[kOps > epi_bytes(WM * WN) ? kOps : epi_bytes(WM * WN)];
   uint16_t(*sA)[BM * RS] = reinterpret_cast<uint16_t(*)[BM * RS]>(smem);
   uint16_t(*sB)[BN * RS] =
       reinterpret_cast<uint16_t(*)[BN * RS]>(smem + TERMS * BM * RS * 2);
 
-  int tm, tn;
-  if(!map_tile(tiles_m, tiles_n, tm, tn)) return;
   const int m0 = tm * BM, n0 = tn * BN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave % WM) * 64, wn0 = (wave / WM) * 64;
@@ -490,32 +549,47 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
   }
 }
 
-// WM x WN waves, each 64 rows x 16*NI columns (4 x NI MFMA blocks); the
-// workgroup tile is 64*WM x 16*NI*WN.  MINW: waves per SIMD the register
-// allocation must allow.
-template <int WM, int WN, int NI, int KS, int MINW, typename S>
-__global__ __launch_bounds__(64 * WM * WN, MINW) void gemm_fp8_kernel(
-    int M, int N, int nks, const uint8_t* __restrict__ Ap, int m_pad,
-    const int* __restrict__ ea, const uint8_t* __restrict__ Bp, int n_pad,
-    const int* __restrict__ eb, int ntn128, S* __restrict__ C, int ldc,
-    int tiles_m, int tiles_n, int vec_ok) {
+template <int TERMS, int WM, int WN, int MODE, typename S>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(
+    int M, int N, int nks, const uint16_t* __restrict__ Ap, int m_pad,
+    size_t aplane, const uint16_t* __restrict__ Bp, int n_pad, size_t bplane,
+    S* __restrict__ C, int ldc, int tiles_m, int tiles_n, int vec_ok,
+    UpdSched sch, const int* __restrict__ claimed) {
+  __shared__ __attribute__((aligned(16)))
+      uint8_t smem[bf16_smem_bytes<TERMS, WM, WN>()];
+  run_tiles<MODE>(tiles_m, tiles_n, sch, claimed, [&](int tm, int tn) {
+    bf16_tile<TERMS, WM, WN, S>(tm, tn, M, N, nks, Ap, m_pad, aplane, Bp,
+                                n_pad, bplane, C, ldc, vec_ok, smem);
+  });
+}
+
+// LDS row of the FP8 stages: KS stages of 128 bytes, padded by 16 (the 16
+// rows of a fragment read start 4 banks apart).
+template <int WM, int WN, int NI, int KS>
+constexpr int fp8_smem_bytes() {
+  constexpr int ops = (64 * WM + 16 * NI * WN) * (KS * 128 + 16);
+  return ops > epi_bytes(WM * WN) ? ops : epi_bytes(WM * WN);
+}
+
+// Tile (tm, tn) of C -= A B on FP8 MFMA: WM x WN waves, each 64 rows x
+// 16*NI columns (4 x NI MFMA blocks); the workgroup tile is 64*WM x
+// 16*NI*WN.
+template <int WM, int WN, int NI, int KS, typename S>
+__device__ __forceinline__ void fp8_tile(
+    int tm, int tn, int M, int N, int nks, const uint8_t* __restrict__ Ap,
+    int m_pad, const int* __restrict__ ea, const uint8_t* __restrict__ Bp,
+    int n_pad, const int* __restrict__ eb, int ntn128, S* __restrict__ C,
+    int ldc, int vec_ok, uint8_t* smem) {
   constexpr int NT  = 64 * WM * WN;
   constexpr int BM  = 64 * WM;
   constexpr int WNC = 16 * NI;  // columns per wave
   constexpr int BN  = WNC * WN;
-  // LDS row: KS stages of 128 bytes, padded by 16 (68 dwords for KS = 2: the
-  // 16 rows of a fragment read start 4 banks apart)
   constexpr int RS  = KS * 128 + 16;
   constexpr int ACH = BM * 8 / NT;  // 16-byte chunks per thread per stage
   constexpr int BCH = BN * 8 / NT;
-  constexpr int kOps = (BM + BN) * RS;
-  __shared__ __attribute__((aligned(16)))
-      uint8_t smem[kOps > epi_bytes(WM * WN) ? kOps : epi_bytes(WM * WN)];
   uint8_t* sA = smem;
   uint8_t* sB = smem + BM * RS;
 
-  int tm, tn;
-  if(!map_tile(tiles_m, tiles_n, tm, tn)) return;
   const int m0 = tm * BM, n0 = tn * BN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm0 = (wave % WM) * 64, wn0 = (wave / WM) * WNC;
@@ -621,16 +695,27 @@ __global__ __launch_bounds__(64 * WM * WN, MINW) void gemm_fp8_kernel(
   }
 }
 
+// MINW: waves per SIMD the register allocation must allow.
+template <int WM, int WN, int NI, int KS, int MINW, int MODE, typename S>
+__global__ __launch_bounds__(64 * WM * WN, MINW) void gemm_fp8_kernel(
+    int M, int N, int nks, const uint8_t* __restrict__ Ap, int m_pad,
+    const int* __restrict__ ea, const uint8_t* __restrict__ Bp, int n_pad,
+    const int* __restrict__ eb, int ntn128, S* __restrict__ C, int ldc,
+    int tiles_m, int tiles_n, int vec_ok, UpdSched sch,
+    const int* __restrict__ claimed) {
+  __shared__ __attribute__((aligned(16)))
+      uint8_t smem[fp8_smem_bytes<WM, WN, NI, KS>()];
+  run_tiles<MODE>(tiles_m, tiles_n, sch, claimed, [&](int tm, int tn) {
+    fp8_tile<WM, WN, NI, KS, S>(tm, tn, M, N, nks, Ap, m_pad, ea, Bp, n_pad,
+                                eb, ntn128, C, ldc, vec_ok, smem);
+  });
+}
+
 constexpr int kWM = 2, kWN = 2;  // 128 x 128 workgroup tiles (BF16)
 // FP8: 256 x 128 workgroup tiles of 4 waves, each 64 x 128 (two workgroups
 // per CU); 128-deep FP8 stages, one per LDS fill
 constexpr int kFp8WM = 4, kFp8WN = 1, kFp8NI = 8, kFp8MinW = 2;
 constexpr int kFp8Stages = 1;
-
-inline long tile_grid(int tiles_m, int tiles_n) {
-  const long T = static_cast<long>(tiles_m) * tiles_n;
-  return ((T + kXcds - 1) / kXcds) * kXcds;
-}
 
 // The vector epilogue needs C 16-byte aligned with ldc a multiple of 4
 // (every 4-row group of a column starts on a 4-element boundary).
@@ -639,18 +724,94 @@ int vec_epilogue_ok(const S* C, int ldc) {
   return reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % 4 == 0;
 }
 
+// Grid and LDS reservation of a claimed launch of kernel fn (threads per
+// workgroup): one workgroup per slot of the device (occupancy x CUs), each
+// asking for its share of the CU's LDS, so that no workgroup of another
+// kernel that uses LDS (every kernel of the panel factorisation does) is
+// placed beside it: those run on the vacated CUs only.
+struct ClaimedShape {
+  int grid = 0, pad = 0;
+};
+hipError_t claimed_shape(const void* fn, int threads, ClaimedShape* out) {
+  static std::vector<std::pair<const void*, ClaimedShape>> cache;
+  for(const auto& c : cache)
+    if(c.first == fn) {
+      *out = c.second;
+      return hipSuccess;
+    }
+  int        dev = 0, cus = 0, per_cu = 0, occ = 0;
+  hipError_t e   = hipGetDevice(&dev);
+  if(e == hipSuccess)
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if(e == hipSuccess)
+    e = hipDeviceGetAttribute(
+        &per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+  if(e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, 0);
+  hipFuncAttributes fa;
+  if(e == hipSuccess) e = hipFuncGetAttributes(&fa, fn);
+  if(e != hipSuccess) return e;
+  if(occ < 1 || cus < 1) return hipErrorInvalidValue;
+  ClaimedShape cs;
+  cs.grid = occ * cus;
+  cs.pad  = per_cu / occ - static_cast<int>(fa.sharedSizeBytes);
+  if(cs.pad < 0) cs.pad = 0;
+  if(cs.pad > 0)
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            cs.pad);
+  if(e != hipSuccess) return e;
+  cache.emplace_back(fn, cs);
+  *out = cs;
+  return hipSuccess;
+}
+
+// Tiles per XCD share of a T-tile launch (the claim counters' range).
+inline hipError_t share_of(long T, int* per) {
+  const long p = (T + kXcds - 1) / kXcds;
+  if(p * kXcds > 0x7fffffffL || p >= (1L << 24)) return hipErrorInvalidValue;
+  *per = static_cast<int>(p);
+  return hipSuccess;
+}
+
 template <int TERMS, int WM, int WN, typename S>
 hipError_t gemm_bf16_launch(hipStream_t s, int m, int n, int K,
-                            const MxpPacked& P, S* C, int ldc) {
-  constexpr int BM = 64 * WM, BN = 64 * WN;
+                            const MxpPacked& P, S* C, int ldc,
+                            const MxpLaunch& L) {
+  constexpr int BM = 64 * WM, BN = 64 * WN, NT = 64 * WM * WN;
   const int  tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
-  const long grid    = tile_grid(tiles_m, tiles_n);
-  if(grid > 0x7fffffffL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((gemm_bf16_kernel<TERMS, WM, WN, S>),
-                     dim3(static_cast<unsigned>(grid)), dim3(64 * WM * WN), 0, s,
-                     m, n, (K + 31) / 32, static_cast<const uint16_t*>(P.a),
-                     P.m_pad, P.aplane, static_cast<const uint16_t*>(P.b), P.n_pad,
-                     P.bplane, C, ldc, tiles_m, tiles_n, vec_epilogue_ok(C, ldc));
+  int        per     = 0;
+  hipError_t e       = share_of(static_cast<long>(tiles_m) * tiles_n, &per);
+  if(e != hipSuccess) return e;
+  const UpdSched sch{L.claim, per, L.vacate, L.stop};
+  const int      nks = (K + 31) / 32, vec = vec_epilogue_ok(C, ldc);
+  const auto*    A   = static_cast<const uint16_t*>(P.a);
+  const auto*    B   = static_cast<const uint16_t*>(P.b);
+  const dim3     plain(static_cast<unsigned>(per) * kXcds);
+  if(L.mode == kModePlain) {
+    hipLaunchKernelGGL((gemm_bf16_kernel<TERMS, WM, WN, kModePlain, S>), plain,
+                       dim3(NT), 0, s, m, n, nks, A, P.m_pad, P.aplane, B,
+                       P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n, vec, sch,
+                       L.claim);
+  } else if(L.mode == kModeResume) {
+    if(!L.claim) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_bf16_kernel<TERMS, WM, WN, kModeResume, S>), plain,
+                       dim3(NT), 0, s, m, n, nks, A, P.m_pad, P.aplane, B,
+                       P.n_pad, P.bplane, C, ldc, tiles_m, tiles_n, vec, sch,
+                       L.claim);
+  } else if(L.mode == kModeClaimed) {
+    if(!L.claim || !L.stop) return hipErrorInvalidValue;
+    ClaimedShape cs;
+    e = claimed_shape(reinterpret_cast<const void*>(
+                          gemm_bf16_kernel<TERMS, WM, WN, kModeClaimed, S>),
+                      NT, &cs);
+    if(e != hipSuccess) return e;
+    hipLaunchKernelGGL((gemm_bf16_kernel<TERMS, WM, WN, kModeClaimed, S>),
+                       dim3(cs.grid), dim3(NT), cs.pad, s, m, n, nks, A,
+                       P.m_pad, P.aplane, B, P.n_pad, P.bplane, C, ldc, tiles_m,
+                       tiles_n, vec, sch, L.claim);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -660,34 +821,63 @@ hipError_t gemm_bf16_launch(hipStream_t s, int m, int n, int K,
 // tile is the faster, 6.5 against 6.9 ms).  Rows are padded to 256.
 template <typename S>
 hipError_t gemm_bf16_t(hipStream_t s, int terms, int m, int n, int K,
-                       const MxpPacked& P, S* C, int ldc) {
+                       const MxpPacked& P, S* C, int ldc, const MxpLaunch& L) {
   if(m <= 0 || n <= 0 || K <= 0) return hipSuccess;
   switch(terms) {
-    case 1: return gemm_bf16_launch<1, 4, 2>(s, m, n, K, P, C, ldc);
-    case 2: return gemm_bf16_launch<2, kWM, kWN>(s, m, n, K, P, C, ldc);
-    case 3: return gemm_bf16_launch<3, kWM, kWN>(s, m, n, K, P, C, ldc);
+    case 1: return gemm_bf16_launch<1, 4, 2>(s, m, n, K, P, C, ldc, L);
+    case 2: return gemm_bf16_launch<2, kWM, kWN>(s, m, n, K, P, C, ldc, L);
+    case 3: return gemm_bf16_launch<3, kWM, kWN>(s, m, n, K, P, C, ldc, L);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <typename S>
 hipError_t gemm_fp8_t(hipStream_t s, int m, int n, int K, const MxpPacked& P,
-                      S* C, int ldc) {
+                      S* C, int ldc, const MxpLaunch& L) {
   if(m <= 0 || n <= 0 || K <= 0) return hipSuccess;
   constexpr int BM = 64 * kFp8WM, BN = 16 * kFp8NI * kFp8WN;
+  constexpr int NT = 64 * kFp8WM * kFp8WN;
   static_assert(kMxpRowPad % BM == 0 && kMxpRowPad % BN == 0,
                 "packed operands are padded to whole tiles");
   const int  tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
-  const long grid    = tile_grid(tiles_m, tiles_n);
-  if(grid > 0x7fffffffL) return hipErrorInvalidValue;
-  const int nks = (K + 127) / 128;
-  const dim3 g(static_cast<unsigned>(grid)), b(64 * kFp8WM * kFp8WN);
-  hipLaunchKernelGGL((gemm_fp8_kernel<kFp8WM, kFp8WN, kFp8NI, kFp8Stages, kFp8MinW, S>),
-                     g, b, 0, s, m, n, nks,
-                     static_cast<const uint8_t*>(P.a), P.m_pad, P.ea,
-                     static_cast<const uint8_t*>(P.b), P.n_pad, P.eb,
-                     P.n_pad / 128, C, ldc, tiles_m, tiles_n,
-                     vec_epilogue_ok(C, ldc));
+  int        per     = 0;
+  hipError_t e       = share_of(static_cast<long>(tiles_m) * tiles_n, &per);
+  if(e != hipSuccess) return e;
+  const UpdSched sch{L.claim, per, L.vacate, L.stop};
+  const int      nks = (K + 127) / 128, vec = vec_epilogue_ok(C, ldc);
+  const auto*    A   = static_cast<const uint8_t*>(P.a);
+  const auto*    B   = static_cast<const uint8_t*>(P.b);
+  const dim3     plain(static_cast<unsigned>(per) * kXcds);
+  if(L.mode == kModePlain) {
+    hipLaunchKernelGGL((gemm_fp8_kernel<kFp8WM, kFp8WN, kFp8NI, kFp8Stages,
+                                        kFp8MinW, kModePlain, S>),
+                       plain, dim3(NT), 0, s, m, n, nks, A, P.m_pad, P.ea, B,
+                       P.n_pad, P.eb, P.n_pad / 128, C, ldc, tiles_m, tiles_n,
+                       vec, sch, L.claim);
+  } else if(L.mode == kModeResume) {
+    if(!L.claim) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_fp8_kernel<kFp8WM, kFp8WN, kFp8NI, kFp8Stages,
+                                        kFp8MinW, kModeResume, S>),
+                       plain, dim3(NT), 0, s, m, n, nks, A, P.m_pad, P.ea, B,
+                       P.n_pad, P.eb, P.n_pad / 128, C, ldc, tiles_m, tiles_n,
+                       vec, sch, L.claim);
+  } else if(L.mode == kModeClaimed) {
+    if(!L.claim || !L.stop) return hipErrorInvalidValue;
+    ClaimedShape cs;
+    e = claimed_shape(
+        reinterpret_cast<const void*>(
+            gemm_fp8_kernel<kFp8WM, kFp8WN, kFp8NI, kFp8Stages, kFp8MinW,
+                            kModeClaimed, S>),
+        NT, &cs);
+    if(e != hipSuccess) return e;
+    hipLaunchKernelGGL((gemm_fp8_kernel<kFp8WM, kFp8WN, kFp8NI, kFp8Stages,
+                                        kFp8MinW, kModeClaimed, S>),
+                       dim3(cs.grid), dim3(NT), cs.pad, s, m, n, nks, A,
+                       P.m_pad, P.ea, B, P.n_pad, P.eb, P.n_pad / 128, C, ldc,
+                       tiles_m, tiles_n, vec, sch, L.claim);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -751,15 +941,28 @@ hipError_t launch_mxp_pack(hipStream_t s, int update, int terms, int m, int K,
 }
 
 hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
-                           int K, const MxpPacked& P, float* C, int ldc) {
-  return update == HPLX_UPDATE_FP8 ? gemm_fp8_t(s, m, n, K, P, C, ldc)
-                                   : gemm_bf16_t(s, terms, m, n, K, P, C, ldc);
+                           int K, const MxpPacked& P, float* C, int ldc,
+                           const MxpLaunch& L) {
+  return update == HPLX_UPDATE_FP8 ? gemm_fp8_t(s, m, n, K, P, C, ldc, L)
+                                   : gemm_bf16_t(s, terms, m, n, K, P, C, ldc, L);
 }
 
 hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
-                           int K, const MxpPacked& P, __half* C, int ldc) {
-  return update == HPLX_UPDATE_FP8 ? gemm_fp8_t(s, m, n, K, P, C, ldc)
-                                   : gemm_bf16_t(s, terms, m, n, K, P, C, ldc);
+                           int K, const MxpPacked& P, __half* C, int ldc,
+                           const MxpLaunch& L) {
+  return update == HPLX_UPDATE_FP8 ? gemm_fp8_t(s, m, n, K, P, C, ldc, L)
+                                   : gemm_bf16_t(s, terms, m, n, K, P, C, ldc, L);
+}
+
+MxpPacked mxp_cols(const MxpPacked& P, int update, int c0) {
+  MxpPacked Q = P;
+  if(update == HPLX_UPDATE_FP8) {
+    Q.b  = static_cast<uint8_t*>(P.b) + static_cast<size_t>(c0) * 128;
+    Q.eb = P.eb + c0 / 128;
+  } else {
+    Q.b = static_cast<uint16_t*>(P.b) + static_cast<size_t>(c0) * 32;
+  }
+  return Q;
 }
 
 }  // namespace hplx

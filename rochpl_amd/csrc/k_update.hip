@@ -39,6 +39,7 @@
 // The stage loads come from lds_stream.hpp; the kernel waits explicitly, no
 // compiler-issued vector memory access is in flight inside the K loop, and
 // all stage loads are drained before the epilogue.
+#include "claim.hpp"
 #include "hplx_internal.hpp"
 #include "lds_stream.hpp"
 
@@ -52,46 +53,8 @@ constexpr int kUXcds = 8, kUGroupM = 8;
 
 typedef double ud4 __attribute__((ext_vector_type(4)));
 
-// Tiles of the claimed mode: counters claim[32 x] for XCD x, tile ranges
-// [x per, (x + 1) per) of the order below, handed out kClaimChunk at a time.
-constexpr int kClaimChunk = 1;
-struct UpdSched {
-  int*                      claim;   // per-XCD counters
-  int                       per;
-  const unsigned long long* vacate;  // 8 words: __smid() values whose CUs
-                                     // this launch leaves (null: none)
-  const int*                stop;    // claims stop once *stop != 0
-};
-
-__device__ inline bool on_vacated_cu(const UpdSched& s) {
-  if(!s.vacate) return false;
-  const unsigned id = __smid() & 511u;
-  return (s.vacate[id >> 6] >> (id & 63)) & 1ull;
-}
-
-// First claim on the first XCD share not yet found exhausted, starting with
-// this workgroup's XCD: returns (XCD << 24 | counter value), or -1.
-__device__ inline int claim_issue(int* claim, int x, unsigned dead) {
-  for(int d = 0; d < kUXcds; ++d) {
-    const int y = (x + d) & (kUXcds - 1);
-    if(!((dead >> y) & 1u))
-      return (y << 24) | atomicAdd(&claim[32 * y], kClaimChunk);
-  }
-  return -1;
-}
-
-// The tile a claim stands for, or further claims on the other shares until
-// one succeeds; -1 once every share is exhausted.
-__device__ inline int claim_resolve(int* claim, int x, int v, int per, int T,
-                                    unsigned& dead) {
-  while(v >= 0) {
-    const int y = v >> 24, k = v & 0xFFFFFF;
-    if(k < per && y * per + k < T) return y * per + k;
-    dead |= 1u << y;
-    v = claim_issue(claim, x, dead);
-  }
-  return -1;
-}
+// Tiles of the claimed mode (claim.hpp): counters claim[32 x] for XCD x,
+// tile ranges [x per, (x + 1) per) of the order below.
 
 #define HPLX_USLOT(n)                                             \
   __shared__ __attribute__((aligned(16))) double sA##n[UBK][UROW]; \
@@ -291,13 +254,7 @@ __global__ __launch_bounds__(512, 2) void update_claimed_kernel(
   const int T = tiles_m * tiles_n;
   // thread 0: claim the next chunk into sc, or none once *stop is set
   auto claim_next = [&](unsigned dead) {
-    int t0 = -1;
-    if(!__hip_atomic_load(sch.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-      const int x = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) &
-                    (kUXcds - 1);  // XCC_ID[3:0]
-      t0 = claim_resolve(sch.claim, x, claim_issue(sch.claim, x, dead),
-                         sch.per, T, dead);
-    }
+    const int t0 = claim_next_tile(sch, T, dead);
     sc.tile = t0;
     // a chunk ends at its XCD share's end and at the last tile
     sc.end  = t0 < 0 ? t0 : min(min(t0 + kClaimChunk, T),

@@ -34,11 +34,26 @@ size_t mxp_scale_count(int rows, int K);
 hipError_t launch_mxp_pack(hipStream_t s, int update, int terms, int m, int K,
                            const float* L, int ldl, int n, const float* U,
                            int ldu, MxpPacked& P);
+// How an update launch is scheduled: plain (one tile per workgroup), or the
+// look-ahead's claimed / resume pair (claim: per-XCD counters claim[32 x];
+// vacate: the CUs a claimed launch leaves to the panel factorisation; stop:
+// the flag that ends its claims; the resume launch continues from claim).
+struct MxpLaunch {
+  int                       mode   = 0;  // 0 plain, 1 claimed, 2 resume
+  int*                      claim  = nullptr;
+  const unsigned long long* vacate = nullptr;
+  const int*                stop   = nullptr;
+};
 // C (m x n, storage type) -= L * U from the packed operands.
 hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
-                           int K, const MxpPacked& P, float* C, int ldc);
+                           int K, const MxpPacked& P, float* C, int ldc,
+                           const MxpLaunch& L = MxpLaunch());
 hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
-                           int K, const MxpPacked& P, __half* C, int ldc);
+                           int K, const MxpPacked& P, __half* C, int ldc,
+                           const MxpLaunch& L = MxpLaunch());
+// The packed U12 from column c0 on (c0 a multiple of 128): the same packing
+// seen as starting at column c0, for an update of a column range.
+MxpPacked mxp_cols(const MxpPacked& P, int update, int c0);
 
 // Storage <-> FP32 copies of an m x n block (panel staging of FP16 storage,
 // U rows back into storage).

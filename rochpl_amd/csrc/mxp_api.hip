@@ -120,6 +120,119 @@ int build_inverses(Ctx& c, const hplx_mxp_pmat* mat) {
              : build_inverses_t<S, float>(c, mat);
 }
 
+// Steps of the mixed-precision panel loop (shared by the serial and the
+// look-ahead schedules).
+template <typename S>
+struct MxpLoop {
+  static constexpr bool kF16 = sizeof(S) == 2;
+  Ctx&                 c;
+  const hplx_palg*     palg;
+  const hplx_mxp_opts* o;
+  hplx_mxp_pmat*       mat;
+  MxpWs*               ws;
+  S*                   A;
+  int                  N, nb, lda;
+
+  int width(int k) const { return (N - k * nb) < nb ? (N - k * nb) : nb; }
+  // the FP32 panel of step k (FP16 storage: its staged copy)
+  float* panel_ptr(int k) const {
+    const int j = k * nb;
+    if constexpr(kF16)
+      return ws->P32;
+    else
+      return A + j + static_cast<size_t>(j) * lda;
+  }
+  int panel_ld() const {
+    if constexpr(kF16)
+      return ws->ldp;
+    else
+      return lda;
+  }
+
+  // Panel k factored in FP32 on stream s (FP16 storage: staged in P32 and
+  // written back).
+  int panel(int k, hipStream_t s) {
+    const int j = k * nb, jb = width(k), m = N - j;
+    S*        Ap = A + j + static_cast<size_t>(j) * lda;
+    if constexpr(kF16) HIPCHK(c, hplx::launch_copy_block(s, m, jb, Ap, lda, ws->P32, ws->ldp));
+    c.panel_stream_ = s;
+    const hipError_t e = hplx::panel_factor(c, hplx::panel_algo(*palg), panel_ptr(k),
+                                            panel_ld(), m, jb, mat->ipiv + j, j);
+    c.panel_stream_ = nullptr;
+    HIPCHK(c, e);
+    if constexpr(kF16) HIPCHK(c, hplx::launch_copy_block(s, m, jb, ws->P32, ws->ldp, Ap, lda));
+    return HPLX_SUCCESS;
+  }
+
+  // Step k up to its update, on stream s: composed pivots, L11^{-1}, the
+  // interchanges of the nt trailing columns (new top rows -> W), U12 in FP32
+  // (stored back in S), and the operands packed for the low-precision GEMM.
+  int prepare(int k, hipStream_t s, hplx::MxpPacked* pk, const float** U,
+              int* ldu) {
+    const int j = k * nb, jb = width(k), m = N - j, nt = N - j - jb;
+    S*        A12 = A + j + static_cast<size_t>(j + jb) * lda;
+    float*    P = panel_ptr(k);
+    const int ldP = panel_ld();
+    HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
+    HIPCHK(c, hplx::launch_linv(s, jb, P, ldP, ws->linv, nb));
+    HIPCHK(c, hplx::launch_swap_gather(s, jb, nt, A12, lda, c.permU, ws->W, nb));
+    if constexpr(kF16) {
+      HIPCHK(c, hplx::launch_sgemm(s, jb, nt, jb, 1.0, ws->linv, nb, ws->W, nb,
+                                   0.0, ws->U, nb, true));
+      HIPCHK(c, hplx::launch_copy_block(s, jb, nt, ws->U, nb, A12, lda));
+      *U   = ws->U;
+      *ldu = nb;
+    } else {
+      HIPCHK(c, hplx::launch_sgemm(s, jb, nt, jb, 1.0, ws->linv, nb, ws->W, nb,
+                                   0.0, A12, lda, true));
+      *U   = A12;
+      *ldu = lda;
+    }
+    pk->a  = ws->pa;
+    pk->b  = ws->pb;
+    pk->ea = ws->ea;
+    pk->eb = ws->eb;
+    if(o->update != HPLX_UPDATE_FP32 && m - jb > 0)
+      HIPCHK(c, hplx::launch_mxp_pack(s, o->update, o->terms, m - jb, jb, P + jb,
+                                      ldP, nt, *U, *ldu, *pk));
+    return HPLX_SUCCESS;
+  }
+
+  // C22 columns [c0, c0 + n) of step k -= L21 U12, on stream s, timed by an
+  // event pair of the given kind (1 whole launch, 2 claimed, 3 resume)
+  // carrying `flops`.
+  int gemm(int k, int c0, int n, hipStream_t s, const hplx::MxpPacked& pk,
+           const float* U, int ldu, const hplx::MxpLaunch& L, int kind,
+           double flops) {
+    const int j = k * nb, jb = width(k), m = N - j;
+    if(m - jb <= 0 || n <= 0) return HPLX_SUCCESS;
+    S*        C = A + j + jb + static_cast<size_t>(j + jb + c0) * lda;
+    const int pi = c.ev_pairs++;
+    c.ev_kind[pi]  = kind;
+    c.ev_flops[pi] = flops;
+    HIPCHK(c, hipEventRecord(c.ev[2 * pi], s));
+    if(o->update == HPLX_UPDATE_FP32) {
+      if constexpr(!kF16)
+        HIPCHK(c, hplx::launch_sgemm(s, m - jb, n, jb, -1.0, panel_ptr(k) + jb,
+                                     panel_ld(), U + static_cast<size_t>(c0) * ldu,
+                                     ldu, 1.0, C, lda, false));
+    } else {
+      const hplx::MxpPacked q = c0 ? hplx::mxp_cols(pk, o->update, c0) : pk;
+      HIPCHK(c, hplx::launch_mxp_gemm(s, o->update, o->terms, m - jb, n, jb, q,
+                                      C, lda, L));
+    }
+    HIPCHK(c, hipEventRecord(c.ev[2 * pi + 1], s));
+    return HPLX_SUCCESS;
+  }
+};
+
+// The panel loop (src/pgesv/HPL_pdgesv.cpp:168-245 at P=Q=1) with depth-1
+// look-ahead as in the FP64 loop: per step, the update of panel k+1's
+// columns first, then panel k+1 factored on the panel stream while the rest
+// of update k runs beside it (a claimed launch on the CUs the panel leaves,
+// resumed on every CU once the panel is done).  Each C tile is computed by
+// one launch with the same operands and K order in both schedules, so the
+// factors are the serial schedule's bit for bit.
 template <typename S>
 int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
                 hplx_mxp_pmat* mat) {
@@ -127,96 +240,108 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
   if(N > hplx::max_leaf_rows(c))
     return fail(c, HPLX_ERR_UNSUPPORTED, "N=%d exceeds the panel kernel's %d rows",
                 N, hplx::max_leaf_rows(c));
-  MxpWs*      ws = static_cast<MxpWs*>(mat->ws);
-  S*          A  = static_cast<S*>(mat->A);
-  hipStream_t s  = c.stream;
-  constexpr bool kF16 = sizeof(S) == 2;
+  MxpWs*     ws = static_cast<MxpWs*>(mat->ws);
+  MxpLoop<S> L{c, palg, o, mat, ws, static_cast<S*>(mat->A), N, nb, lda};
 
   const int npanels = (N + nb - 1) / nb;
-  while(static_cast<int>(c.ev.size()) < 2 * npanels) {
+  while(static_cast<int>(c.ev.size()) < 6 * npanels) {
     hipEvent_t e;
     HIPCHK(c, hipEventCreate(&e));
     c.ev.push_back(e);
   }
   c.ev_pairs   = 0;
   c.gemm_flops = 0.0;
-  c.ev_kind.assign(npanels, 1);
-  c.ev_flops.assign(npanels, 0.0);
+  c.ev_kind.assign(3 * npanels, 1);
+  c.ev_flops.assign(3 * npanels, 0.0);
   c.splits.clear();
   c.claim_host.clear();
   hplx::Progress prog(c, npanels, nb, N);
   HIPCHK(c, prog.init());
 
-  ws->swp_ready = false;
-  HIPCHK(c, hplx::reset_panel_state(c));
-  for(int j = 0; j < N; j += nb) {
-    const int jb = (N - j) < nb ? (N - j) : nb;
-    const int m  = N - j;
-    S*        Ap = A + j + static_cast<size_t>(j) * lda;
-    float*    P;
-    int       ldP;
-    if constexpr(kF16) {
-      HIPCHK(c, hplx::launch_copy_block(s, m, jb, Ap, lda, ws->P32, ws->ldp));
-      P   = ws->P32;
-      ldP = ws->ldp;
-    } else {
-      P   = Ap;
-      ldP = lda;
-    }
-    HIPCHK(c, hplx::panel_factor(c, hplx::panel_algo(*palg), P, ldP, m, jb,
-                                 mat->ipiv + j, j));
-    if constexpr(kF16)
-      HIPCHK(c, hplx::launch_copy_block(s, m, jb, P, ldP, Ap, lda));
-    const int nt = N - j - jb;
-    if(nt > 0) {
-      S* A12 = A + j + static_cast<size_t>(j + jb) * lda;
-      HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, c.permU));
-      HIPCHK(c, hplx::launch_linv(s, jb, P, ldP, ws->linv, nb));
-      HIPCHK(c, hplx::launch_swap_gather(s, jb, nt, A12, lda, c.permU, ws->W, nb));
-      const float* U;
-      int          ldu;
-      if constexpr(kF16) {
-        HIPCHK(c, hplx::launch_sgemm(s, jb, nt, jb, 1.0, ws->linv, nb, ws->W, nb,
-                                     0.0, ws->U, nb, true));
-        HIPCHK(c, hplx::launch_copy_block(s, jb, nt, ws->U, nb, A12, lda));
-        U   = ws->U;
-        ldu = nb;
-      } else {
-        HIPCHK(c, hplx::launch_sgemm(s, jb, nt, jb, 1.0, ws->linv, nb, ws->W, nb,
-                                     0.0, A12, lda, true));
-        U   = A12;
-        ldu = lda;
-      }
-      if(m - jb > 0) {
-        S*        C22 = A12 + jb;
-        hipEvent_t e0 = c.ev[2 * c.ev_pairs], e1 = c.ev[2 * c.ev_pairs + 1];
-        if(o->update == HPLX_UPDATE_FP32) {
-          if constexpr(!kF16) {
-            HIPCHK(c, hipEventRecord(e0, s));
-            HIPCHK(c, hplx::launch_sgemm(s, m - jb, nt, jb, -1.0, P + jb, ldP, U,
-                                         ldu, 1.0, C22, lda, false));
-            HIPCHK(c, hipEventRecord(e1, s));
-          }
-        } else {
-          hplx::MxpPacked pk;
-          pk.a  = ws->pa;
-          pk.b  = ws->pb;
-          pk.ea = ws->ea;
-          pk.eb = ws->eb;
-          HIPCHK(c, hplx::launch_mxp_pack(s, o->update, o->terms, m - jb, jb,
-                                          P + jb, ldP, nt, U, ldu, pk));
-          HIPCHK(c, hipEventRecord(e0, s));
-          HIPCHK(c, hplx::launch_mxp_gemm(s, o->update, o->terms, m - jb, nt,
-                                          jb, pk, C22, lda));
-          HIPCHK(c, hipEventRecord(e1, s));
-        }
-        c.ev_flops[c.ev_pairs] = 2.0 * (m - jb) * static_cast<double>(nt) * jb;
-        c.gemm_flops += c.ev_flops[c.ev_pairs];
-        c.ev_pairs++;
-      }
-    }
-    HIPCHK(c, prog.panel_queued(j / nb, s));
+  // the look-ahead needs the claimed GEMM (BF16 / FP8) and column splits on
+  // whole 128-column tiles
+  const bool la = c.lookahead && c.cus_per_xcd > 0 &&
+                  o->update != HPLX_UPDATE_FP32 && nb % 128 == 0;
+  hipStream_t U  = la ? c.ustream : c.stream;
+  hipStream_t P  = c.pstream;
+  size_t      nev = 0;
+  hipEvent_t  ev;
+  if(la) {  // U and P start after the caller's stream
+    if(const int r = hplx::sync_event(c, nev++, &ev)) return r;
+    HIPCHK(c, hipEventRecord(ev, c.stream));
+    HIPCHK(c, hipStreamWaitEvent(U, ev, 0));
+    HIPCHK(c, hipStreamWaitEvent(P, ev, 0));
   }
+  ws->swp_ready = false;
+  c.panel_stream_ = U;
+  const hipError_t er = hplx::reset_panel_state(c);
+  c.panel_stream_ = nullptr;
+  HIPCHK(c, er);
+  int r;
+  if((r = L.panel(0, U))) return r;
+  for(int k = 0; k < npanels; ++k) {
+    const int j = k * nb, jb = L.width(k), m = N - j;
+    const int nt  = N - j - jb;
+    const int jb1 = k + 1 < npanels ? L.width(k + 1) : 0;
+    if(nt > 0) {
+      hplx::MxpPacked pk;
+      const float*    Uf  = nullptr;
+      int             ldu = 0;
+      if((r = L.prepare(k, U, &pk, &Uf, &ldu))) return r;
+      const double fl = 2.0 * (m - jb) * static_cast<double>(jb);
+      c.gemm_flops += fl * nt;
+      const int  rest = nt - jb1;
+      const int  rR   = hplx::vacate_set_size(N - j - jb);
+      const bool overlap = la && jb1 > 0 && rest >= hplx::kMinOverlapCols &&
+                           rR <= c.cus_per_xcd && rR < 28 && m - jb > 0;
+      if(overlap) {
+        // panel k+1's columns, then that panel beside the rest of update k
+        if((r = L.gemm(k, 0, jb1, U, pk, Uf, ldu, hplx::MxpLaunch(), 1, fl * jb1)))
+          return r;
+        int* claim = c.claim + (k & 1) * hplx::kClaimInts;
+        int* stop  = c.stop + (k & 1) * 32;
+        HIPCHK(c, hplx::launch_zero_ints(U, claim, hplx::kClaimInts));
+        HIPCHK(c, hplx::launch_zero_ints(U, stop, 1));
+        hipEvent_t ev_prep, ev_pan;
+        if((r = hplx::sync_event(c, nev++, &ev_prep)) ||
+           (r = hplx::sync_event(c, nev++, &ev_pan)))
+          return r;
+        HIPCHK(c, hipEventRecord(ev_prep, U));
+        HIPCHK(c, hipStreamWaitEvent(P, ev_prep, 0));
+        if((r = L.panel(k + 1, P))) return r;
+        HIPCHK(c, hplx::launch_set_flag(P, stop));
+        HIPCHK(c, hipEventRecord(ev_pan, P));
+        hplx::MxpLaunch lc;
+        lc.mode   = 1;
+        lc.claim  = claim;
+        lc.vacate = c.vacate + 8 * (rR - 1);
+        lc.stop   = stop;
+        if((r = L.gemm(k, jb1, rest, U, pk, Uf, ldu, lc, 2, 0.0))) return r;
+        HIPCHK(c, hipStreamWaitEvent(U, ev_pan, 0));
+        hplx::MxpLaunch lr;
+        lr.mode  = 2;
+        lr.claim = claim;
+        if((r = L.gemm(k, jb1, rest, U, pk, Uf, ldu, lr, 3, fl * rest))) return r;
+      } else {
+        if((r = L.gemm(k, 0, nt, U, pk, Uf, ldu, hplx::MxpLaunch(), 1, fl * nt)))
+          return r;
+        if(jb1 > 0 && (r = L.panel(k + 1, U))) return r;
+      }
+    } else if(jb1 > 0) {
+      if((r = L.panel(k + 1, U))) return r;
+    }
+    HIPCHK(c, prog.panel_queued(k, U));
+  }
+  if(la) {  // the caller's stream continues after U and P
+    hipEvent_t eu, ep;
+    if((r = hplx::sync_event(c, nev++, &eu)) || (r = hplx::sync_event(c, nev++, &ep)))
+      return r;
+    HIPCHK(c, hipEventRecord(eu, U));
+    HIPCHK(c, hipEventRecord(ep, P));
+    HIPCHK(c, hipStreamWaitEvent(c.stream, eu, 0));
+    HIPCHK(c, hipStreamWaitEvent(c.stream, ep, 0));
+  }
+  hipStream_t s = c.stream;
   int flags[2] = {0, 0};
   HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
                            s));
@@ -225,8 +350,8 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
     return fail(c, HPLX_ERR_HIP,
                 "panel kernel wait timed out (workgroups not co-resident)");
   mat->info = flags[0];
-  if(const int r = build_inverses<S>(c, mat)) return r;
-  if(const int r = build_swaps(c, mat)) return r;
+  if(const int r2 = build_inverses<S>(c, mat)) return r2;
+  if(const int r2 = build_swaps(c, mat)) return r2;
   return flags[0];
 }
 

@@ -339,6 +339,46 @@ def test_mxp_pdgesv_converges(ctx, storage, update, terms, n, nb, shift,
         assert np.array_equal(ipiv, r64["ipiv"])
 
 
+@pytest.mark.parametrize("storage,update,terms,n,nb,shift", [
+    (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1, 8192, 1024, 2048.0),
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_BF16, 3, 6000, 512, 0.0),
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_BF16, 1, 5000, 256, 1250.0),
+    (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_BF16, 2, 5000, 256, 1250.0),
+])
+def test_mxp_lookahead_is_bit_identical_to_serial(ctx, storage, update, terms,
+                                                  n, nb, shift):
+    """Depth-1 look-ahead of the mixed-precision loop (panel k+1 factored on
+    the panel stream beside the claimed / resumed rest of update k) against
+    every step in order on one stream: the same pivots, the same stored
+    factors, the same x and the same refinement history, bit for bit.  6000
+    and 5000 leave a ragged last panel."""
+    say("mxp look-ahead vs serial storage=%d update=%d terms=%d n=%d nb=%d"
+        % (storage, update, terms, n, nb))
+    res = {}
+    mat = ctx.mxp_alloc(n, nb, storage)
+    opts = rp.mxp_opts_default(update, terms)
+    try:
+        for la in (False, True):
+            ctx.lookahead = la
+            ctx.mxp_matgen(mat, rp.HPL_ISEED, shift)
+            rc, r = ctx.mxp_pdgesv(rp.palg_default(), opts, mat)
+            assert rc == 0 and r["converged"] == 1
+            res[la] = (ctx.download_vec(mat.ipiv, n, np.int32),
+                       ctx.download_vec(mat.X, n), download_factors(ctx, mat),
+                       r)
+    finally:
+        ctx.lookahead = True
+        ctx.mxp_free(mat)
+    (p0, x0, f0, r0), (p1, x1, f1, r1) = res[False], res[True]
+    assert np.array_equal(p0, p1)
+    assert np.array_equal(f0.view(np.uint64), f1.view(np.uint64))
+    assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64))
+    assert r0["its"] == r1["its"] and r0["history"] == r1["history"]
+    print("[mxp lookahead] identical; its %d resid %.4f; factor %.1f ms serial, "
+          "%.1f ms look-ahead" % (r1["its"], r1["resid"], r0["factor_ms"],
+                                  r1["factor_ms"]), flush=True)
+
+
 def test_mxp_rejects_bad_options(ctx):
     mat = ctx.mxp_alloc(256, 64, rp.HPLX_STORE_FP16)
     try:
