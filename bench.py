@@ -384,6 +384,9 @@ def run_mixed(ctx, rp, form, args):
                                  "%r" % (form, res))
             runs.append((t, res))
         t, res = runs[-1]
+        # the last run's update launches by kind (1 whole update, 2 claimed
+        # beside the panel, 3 resumed): (ms, launches, flops)
+        kinds = {k: ctx.update_timing(k) for k in (1, 2, 3)}
         ipiv = ctx.download_vec(mat.ipiv, n, np.int32)
         ident = int(np.sum(ipiv != np.arange(n)))
         x = ctx.download_vec(mat.X, n, np.float64) if form == "bf16" else None
@@ -408,11 +411,26 @@ def run_mixed(ctx, rp, form, args):
                 "fp64_nb": args.nb}
     sb = 4 if storage == "fp32" else 2
     ops, hbm = update_traffic(n, nb, update, terms, sb)
-    sec = res["update_ms"] * 1e-3
     peak = BF16_PEAK_TFLOPS if update == "bf16" else FP8_PEAK_TFLOPS
-    mfma = ops / sec / 1e12
-    gbs = hbm / sec / 1e9
+    prods = terms * (terms + 1) // 2 if update == "bf16" else 1
+    total_fl = sum(v[2] for v in kinds.values()) or 1.0
+    # the roofline of the kernel kind with the most time (the resumed part
+    # of the split updates with the look-ahead): its MFMA operations and its
+    # share of the algorithmic bytes over its own launches
+    dom = max(kinds, key=lambda k: kinds[k][0])
+    d_ms, d_cnt, d_fl = kinds[dom]
+    sec = d_ms * 1e-3
+    mfma = d_fl * prods / sec / 1e12 if sec > 0 else 0.0
+    gbs = hbm * d_fl / total_fl / sec / 1e9 if sec > 0 else 0.0
     fr_m, fr_h = mfma / peak, gbs / HBM_PEAK_GBS
+    names = {1: "update of one launch (the next panel's columns, or a whole "
+                "update)", 2: "claimed part beside the panel",
+             3: "resumed part, every CU"}
+    by_kernel = {str(k): {"kernel": names[k], "launches": v[1],
+                          "ms": round(v[0], 1),
+                          "mfma_tflops": round(v[2] * prods / v[0] / 1e9, 2)
+                          if v[0] > 0 else None}
+                 for k, v in kinds.items() if v[1] > 0}
     return {
         "value": round(hpl_flops(n) / t / 1e12, 3), "ms": round(t * 1e3, 1),
         "config": {"workload": "hpl_mxp_%s_p1q1" % form, "N": n, "NB": nb,
@@ -428,11 +446,16 @@ def run_mixed(ctx, rp, form, args):
         "x_vs_fp64": xcmp,
         "update_roofline": {
             "bound": "mfma" if fr_m >= fr_h else "hbm",
+            "kernel": names[dom],
             "mfma_achieved": round(mfma, 2), "mfma_peak": peak,
             "mfma_frac": round(fr_m, 4), "hbm_achieved_gbs": round(gbs, 1),
             "hbm_peak_gbs": HBM_PEAK_GBS, "hbm_frac": round(fr_h, 4),
-            "launches": res["update_launches"],
-            "ms": round(res["update_ms"], 1)},
+            "launches": d_cnt, "ms": round(d_ms, 1),
+            "all_update_launches": {"launches": res["update_launches"],
+                                    "ms": round(res["update_ms"], 1),
+                                    "mfma_tflops": round(ops / res["update_ms"]
+                                                         / 1e9, 2)},
+            "by_kernel": by_kernel},
     }
 
 
@@ -882,7 +905,8 @@ def main():
                 "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
                 "frac": ur["mfma_frac"] if bound == "mfma" else ur["hbm_frac"],
                 "traffic": None,
-                "kernel": "low-precision trailing-update GEMM"}})
+                "kernel": "low-precision trailing-update GEMM: "
+                          + ur.get("kernel", "")}})
     for form, m in mixed.items():
         if fp64 is not None and "value" in m:
             m["vs_fp64"] = round(m["value"] / fp64["value"], 3)

@@ -917,7 +917,8 @@ int hplx_update_timing(hplx_ctx* ctx, int kind, double* ms, int* launches,
     const double total = 2.0 * sp.m * static_cast<double>(sp.n) * sp.k;
     const double took  = 8 * i < c.claim_host.size()
                              ? hplx::claimed_flops(sp.m, sp.n, sp.k,
-                                                   &c.claim_host[8 * i])
+                                                   &c.claim_host[8 * i], sp.bm,
+                                                   sp.bn)
                              : 0.0;
     fl[sp.pair_claimed] = took;
     fl[sp.pair_resume]  = total - took;

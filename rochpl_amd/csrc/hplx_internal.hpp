@@ -95,6 +95,7 @@ struct Ctx {
   // each XCD share had claimed (8 per split, copied back at the end)
   struct Split {
     int m, n, k, pair_claimed, pair_resume;
+    int bm = 128, bn = 128;  // the update kernel's tile
   };
   std::vector<Split> splits;
   std::vector<int>   claim_host;
@@ -245,12 +246,15 @@ hipError_t launch_zero_ints(hipStream_t s, int* p, int n);
 // *p = 1 with an agent-scope store (read by running kernels on any XCD).
 hipError_t launch_set_flag(hipStream_t s, int* p);
 // out[x] = min(claim[32 x], per) for the 8 XCD shares of a claimed update of
-// m x n tiles (queued after the claimed launch).
+// an m x n C in bm x bn tiles (queued after the claimed launch).
 hipError_t launch_claim_log(hipStream_t s, const int* claim, int m, int n,
-                            int* out);
+                            int* out, int bm = 128, int bn = 128);
 // Algorithmic flops of the tiles that the claimed launch of an m x n x k
-// update took, given the 8 counts launch_claim_log recorded.
-double claimed_flops(int m, int n, int k, const int* done);
+// update in bm x bn tiles took (the tile order of every update kernel:
+// groups of 8 tile rows walked column by column, XCD x taking the share
+// [x per, (x + 1) per)), given the 8 counts launch_claim_log recorded.
+double claimed_flops(int m, int n, int k, const int* done, int bm = 128,
+                     int bn = 128);
 // out[b] = __smid() of the CU that ran workgroup b (grid workgroups that
 // each stay ~10 us, so that they spread over the device).
 hipError_t launch_cu_census(hipStream_t s, int* out, int grid);

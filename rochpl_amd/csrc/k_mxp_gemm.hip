@@ -954,6 +954,19 @@ hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
                                    : gemm_bf16_t(s, terms, m, n, K, P, C, ldc, L);
 }
 
+void mxp_tile(int update, int terms, int* bm, int* bn) {
+  if(update == HPLX_UPDATE_FP8) {
+    *bm = 64 * kFp8WM;
+    *bn = 16 * kFp8NI * kFp8WN;
+  } else if(terms == 1) {
+    *bm = 64 * 4;
+    *bn = 64 * 2;
+  } else {
+    *bm = 64 * kWM;
+    *bn = 64 * kWN;
+  }
+}
+
 MxpPacked mxp_cols(const MxpPacked& P, int update, int c0) {
   MxpPacked Q = P;
   if(update == HPLX_UPDATE_FP8) {

@@ -440,29 +440,28 @@ hipError_t launch_update_resume(hipStream_t s, int m, int n, int k,
 }
 
 hipError_t launch_claim_log(hipStream_t s, const int* claim, int m, int n,
-                            int* out) {
-  const long T   = static_cast<long>((m + 127) / 128) * ((n + 127) / 128);
+                            int* out, int bm, int bn) {
+  const long T   = static_cast<long>((m + bm - 1) / bm) * ((n + bn - 1) / bn);
   const int  per = static_cast<int>((T + kUXcds - 1) / kUXcds);
   hipLaunchKernelGGL(claim_log_kernel, dim3(1), dim3(64), 0, s, claim, per,
                      out);
   return hipGetLastError();
 }
 
-double claimed_flops(int m, int n, int k, const int* done) {
-  const int  tiles_m = (m + 127) / 128, tiles_n = (n + 127) / 128;
+double claimed_flops(int m, int n, int k, const int* done, int bm, int bn) {
+  const int  tiles_m = (m + bm - 1) / bm, tiles_n = (n + bn - 1) / bn;
   const long T       = static_cast<long>(tiles_m) * tiles_n;
   const long per     = (T + kUXcds - 1) / kUXcds;
   const long width   = static_cast<long>(kUGroupM) * tiles_n;
   double     f       = 0.0;
   for(int x = 0; x < kUXcds; ++x)
     for(long t = x * per; t < x * per + done[x] && t < T; ++t) {
-      // the tile order of update_tile
       const long group = t / width, firstm = group * kUGroupM;
       const long gsz   = tiles_m - firstm < kUGroupM ? tiles_m - firstm : kUGroupM;
       const long within = t - group * width;
       const long tm = firstm + within % gsz, tn = within / gsz;
-      const long h = m - tm * 128 < 128 ? m - tm * 128 : 128;
-      const long w = n - tn * 128 < 128 ? n - tn * 128 : 128;
+      const long h = m - tm * bm < bm ? m - tm * bm : bm;
+      const long w = n - tn * bn < bn ? n - tn * bn : bn;
       f += 2.0 * static_cast<double>(h * w) * k;
     }
   return f;

@@ -54,6 +54,8 @@ hipError_t launch_mxp_gemm(hipStream_t s, int update, int terms, int m, int n,
 // The packed U12 from column c0 on (c0 a multiple of 128): the same packing
 // seen as starting at column c0, for an update of a column range.
 MxpPacked mxp_cols(const MxpPacked& P, int update, int c0);
+// The C tile of the update kernel a form runs (BF16 / FP8).
+void mxp_tile(int update, int terms, int* bm, int* bn);
 
 // Storage <-> FP32 copies of an m x n block (panel staging of FP16 storage,
 // U rows back into storage).

@@ -316,12 +316,24 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
         lc.claim  = claim;
         lc.vacate = c.vacate + 8 * (rR - 1);
         lc.stop   = stop;
+        const int pc = c.ev_pairs;
         if((r = L.gemm(k, jb1, rest, U, pk, Uf, ldu, lc, 2, 0.0))) return r;
+        int bm = 0, bn = 0;
+        hplx::mxp_tile(o->update, o->terms, &bm, &bn);
+        const int ns = static_cast<int>(c.splits.size());
+        if(ns < hplx::kMaxClaimLog)
+          HIPCHK(c, hplx::launch_claim_log(U, claim, m - jb, rest,
+                                           c.claim_log + 8 * ns, bm, bn));
         HIPCHK(c, hipStreamWaitEvent(U, ev_pan, 0));
         hplx::MxpLaunch lr;
         lr.mode  = 2;
         lr.claim = claim;
+        const int pr = c.ev_pairs;
         if((r = L.gemm(k, jb1, rest, U, pk, Uf, ldu, lr, 3, fl * rest))) return r;
+        Ctx::Split sp{m - jb, rest, jb, pc, pr};
+        sp.bm = bm;
+        sp.bn = bn;
+        c.splits.push_back(sp);
       } else {
         if((r = L.gemm(k, 0, nt, U, pk, Uf, ldu, hplx::MxpLaunch(), 1, fl * nt)))
           return r;
@@ -345,6 +357,11 @@ int factor_loop(Ctx& c, const hplx_palg* palg, const hplx_mxp_opts* o,
   int flags[2] = {0, 0};
   HIPCHK(c, hipMemcpyAsync(flags, c.flags, sizeof(flags), hipMemcpyDeviceToHost,
                            s));
+  const size_t nlog = std::min<size_t>(c.splits.size(), hplx::kMaxClaimLog);
+  c.claim_host.assign(8 * nlog, 0);
+  if(nlog)
+    HIPCHK(c, hipMemcpyAsync(c.claim_host.data(), c.claim_log,
+                             sizeof(int) * 8 * nlog, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   if(flags[1])
     return fail(c, HPLX_ERR_HIP,
