@@ -379,6 +379,44 @@ def test_mxp_lookahead_is_bit_identical_to_serial(ctx, storage, update, terms,
                                   r1["factor_ms"]), flush=True)
 
 
+@pytest.mark.parametrize("pf,rf", [(rp.HPLX_CROUT, rp.HPLX_LEFT_LOOKING),
+                                   (rp.HPLX_LEFT_LOOKING, rp.HPLX_CROUT)])
+@pytest.mark.parametrize("storage,update,terms,n,nb,shift", [
+    (rp.HPLX_STORE_FP32, rp.HPLX_UPDATE_BF16, 3, 4096, 512, 0.0),
+    (rp.HPLX_STORE_FP16, rp.HPLX_UPDATE_FP8, 1, 4096, 1024, 1024.0),
+])
+def test_mxp_panel_variants(ctx, pf, rf, storage, update, terms, n, nb, shift):
+    """The FP32 panel in the left-looking / Crout orders (the FP64 path's
+    variant loop): the refinement converges to the HPL criterion, and the
+    pivots agree with the right-looking run's except where FP32 rounding
+    decides a near tie (none on the diagonally dominant matrix)."""
+    say("mxp variants pfact=%d rfact=%d storage=%d update=%d n=%d"
+        % (pf, rf, storage, update, n))
+    mat = ctx.mxp_alloc(n, nb, storage)
+    opts = rp.mxp_opts_default(update, terms)
+    piv = {}
+    try:
+        for variant in ((rp.HPLX_RIGHT_LOOKING, rp.HPLX_RIGHT_LOOKING), (pf, rf)):
+            palg = rp.palg_default()
+            palg.pfact, palg.rfact = variant
+            ctx.mxp_matgen(mat, rp.HPL_ISEED, shift)
+            rc, res = ctx.mxp_pdgesv(palg, opts, mat)
+            assert rc == 0 and res["converged"] == 1
+            x = ctx.download_vec(mat.X, n)
+            assert o.residual(n, x, diag_shift=shift)["resid"] < 16.0
+            piv[variant] = ctx.download_vec(mat.ipiv, n, np.int32)
+    finally:
+        ctx.mxp_free(mat)
+    p0, p1 = piv.values()
+    agree = float(np.mean(p0 == p1))
+    print("[mxp variants] pivot agreement with right-looking %.4f" % agree,
+          flush=True)
+    if shift >= n / 4:
+        assert np.array_equal(p1, np.arange(n))
+    else:
+        assert agree >= 0.99
+
+
 def test_mxp_rejects_bad_options(ctx):
     mat = ctx.mxp_alloc(256, 64, rp.HPLX_STORE_FP16)
     try:
