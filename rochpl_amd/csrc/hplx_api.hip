@@ -2,9 +2,10 @@
 // HPL_pdgesv panel loop and the residual check.
 //
 // The panel loop restates src/pgesv/HPL_pdgesv.cpp:168-245 at P=Q=1 with the
-// host out of the loop: every step is queued on one stream and no host
-// synchronisation happens until the back-solve.  Per panel (width jb, at
-// global row/column j):
+// host out of the loop: every step is queued on the device (an update
+// stream and, with the depth-1 look-ahead, a panel stream linked by events)
+// and no host synchronisation happens until the back-solve.  Per panel
+// (width jb, at global row/column j):
 //   1. panel factorisation in place (HPL_pdfact)             -> ipiv[j:j+jb)
 //   2. pivot composition on the device (HPL_pdpanel_swapids) -> permU
 //   3. explicit inverse of the unit lower L11                -> Linv
@@ -12,8 +13,9 @@
 //      (HPL_pdlaswp_end / HPL_dlaswp00N)
 //   5. U12 = Linv * W written in place over A12      (dtrsm of HPL_pdupdateTT)
 //   6. A22 -= L21 * U12                              (dgemm of HPL_pdupdateTT)
-// The reference's lookahead and split update only reorder these steps for
-// overlap; each matrix element sees the same sequence of operations.  b is
+// The look-ahead (here as in the reference) and the reference's split update
+// only reorder these steps for overlap; each matrix element sees the same
+// sequence of operations.  b is
 // column n of the augmented matrix and is carried through 4-6, so after the
 // loop the back-solve U x = b' finishes the solve (HPL_pdtrsv).
 #include <algorithm>

@@ -6,8 +6,8 @@
 // src/pgesv/HPL_pdupdateTT.cpp:95-106), for the explicit inverse of L11, for
 // the diagonal blocks of the back-solve (rocblas_dtrsv at
 // src/pgesv/HPL_pdtrsv_device.cpp:170-178,279-287), for the transposed
-// solves of the adjoint, and (diag_inv_kernel) for the FP64 inverses of the
-// diagonal blocks of the mixed-precision factors.
+// solves of the adjoint, and (diag_inv_kernel) for the FP32 or FP64
+// inverses of the diagonal blocks of the mixed-precision factors.
 //
 // x lives in LDS.  The block is processed in 64-row chunks along the sweep
 // direction: wave 0 solves the chunk's 64 x 64 triangle with the chunk's rows
