@@ -209,7 +209,7 @@ def run_fp64(ctx, rp, args):
     log("fp64: allocating N=%d NB=%d (%.1f GiB)" % (
         n, nb, rp.hpl_ld(n) * (n + 1) * 8 / 2 ** 30))
     mat = ctx.pmat_alloc(n, nb)
-    times, checks, stats, kstats = [], [], [], []
+    times, checks, stats, kstats, swaps = [], [], [], [], []
     # Warm-up solves only where they are short: above N = 32768 one solve
     # takes tens of seconds and the code objects are loaded by the first
     # panel, so the requested warm-up steps are not run there.
@@ -236,6 +236,7 @@ def run_fp64(ctx, rp, args):
             raise SystemExit("pdgesv: exact zero pivot at column %d" % info)
         st = ctx.update_stats()
         kinds = {k: ctx.update_timing(k) for k in (1, 2, 3)}
+        swp = ctx.swap_timing()
         chk = ctx.pdtest_check(mat)
         last = time.monotonic() - s0
         log("fp64 %s step %d: %.3f s  %.2f TFLOP/s  resid %.4f  trailing GEMM "
@@ -251,6 +252,7 @@ def run_fp64(ctx, rp, args):
             checks.append(chk)
             stats.append(st)
             kstats.append(kinds)
+            swaps.append(swp)
         else:
             warm_done += 1
 
@@ -301,6 +303,19 @@ def run_fp64(ctx, rp, args):
     dom = max(per_kind, key=lambda k: per_kind[k]["ms"])
     d = per_kind[dom]
     achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
+    # the row interchanges of the trailing columns (hplx_swap_timing): HBM
+    # bound, 4 x 8 B per interchanged element
+    sw_ms = sum(w[0] for w in swaps)
+    sw_cnt = sum(w[1] for w in swaps)
+    sw_bytes = sum(w[2] for w in swaps)
+    sw_gbs = sw_bytes / (sw_ms * 1e-3) / 1e9 if sw_ms > 0 else None
+    swap = {"kernel": "swap_gather_kernel (row interchanges of the trailing "
+                      "columns)",
+            "launches": sw_cnt, "ms": round(sw_ms, 2),
+            "avg_launch_ms": round(sw_ms / sw_cnt, 4) if sw_cnt else None,
+            "algorithmic_gbs": round(sw_gbs, 1) if sw_gbs else None,
+            "bytes_per_element": 32,
+            "hbm_frac": round(sw_gbs / HBM_PEAK_GBS, 4) if sw_gbs else None}
     return {
         "value": hpl_flops(n) / t_mean / 1e12, "t_mean": t_mean,
         "config": {"workload": "hpl_pdgesv_fp64_p1q1", "N": n, "NB": nb,
@@ -324,6 +339,7 @@ def run_fp64(ctx, rp, args):
                      "by_kernel": {str(k): {kk: vv for kk, vv in v.items()
                                             if kk != "flops"}
                                    for k, v in per_kind.items()}},
+        "swap": swap,
         "residual": round(max(c["resid"] for c in checks), 6),
         "residual_r0_form": round(max(c["resid_r0_form"] for c in checks), 6),
         "tflops_ref_formula": round(ref_flops(n) / t_mean / 1e12, 3),
@@ -882,8 +898,8 @@ def main():
             "data": "synthetic: HPL 64-bit LCG matgen, seed 100, uniform "
                     "[-0.5,0.5], b = column N",
             "config": head["config"], "roofline": head["roofline"]})
-        for k in ("residual", "residual_r0_form", "tflops_ref_formula",
-                  "pivot_prefix_check"):
+        for k in ("swap", "residual", "residual_r0_form",
+                  "tflops_ref_formula", "pivot_prefix_check"):
             out[k] = head[k]
     else:
         form = args.mode

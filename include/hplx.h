@@ -284,6 +284,15 @@ int hplx_update_stats(hplx_ctx* ctx, double* gemm_ms, int* launches,
 int hplx_update_timing(hplx_ctx* ctx, int kind, double* ms, int* launches,
                        double* flops);
 
+/* Timing of the row interchanges of the trailing columns (the gather /
+ * scatter kernel of HPL_pdlaswp_end / HPL_dlaswp00N at P=1,
+ * src/pauxil/HPL_dlaswp00N_device.cpp:22-112) in the last hplx_pdgesv /
+ * hplx_pdgetrf: summed launch time (ms, HIP events), launches, and their
+ * algorithmic bytes (4 x 8 B per interchanged element: the pivot row read,
+ * the U buffer written, the displaced top row read and written).  No
+ * counterpart in the reference, which times none. */
+int hplx_swap_timing(hplx_ctx* ctx, double* ms, int* launches, double* bytes);
+
 /* ======================================================================
  * Mixed precision (no counterpart in the reference, which is FP64 only).
  *

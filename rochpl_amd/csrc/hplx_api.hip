@@ -482,7 +482,17 @@ struct Fp64Loop {
     const int j = k * nb, jb = width(k), m = N - j;
     double*   P   = A + j + static_cast<size_t>(j) * lda;
     double*   A12 = A + j + static_cast<size_t>(j + jb + c0) * lda;
+    // the interchanges, timed by their own event pair
+    while(static_cast<int>(c.ev_swap.size()) < 2 * (c.swap_pairs + 1)) {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreate(&e));
+      c.ev_swap.push_back(e);
+    }
+    HIPCHK(c, hipEventRecord(c.ev_swap[2 * c.swap_pairs], s));
     HIPCHK(c, hplx::launch_swap_gather(s, jb, n, A12, lda, permU(k), ws->W, nb));
+    HIPCHK(c, hipEventRecord(c.ev_swap[2 * c.swap_pairs + 1], s));
+    ++c.swap_pairs;
+    c.swap_bytes += 4.0 * 8.0 * jb * static_cast<double>(n);
     HIPCHK(c, hplx::launch_dgemm(s, jb, n, jb, 1.0, Linv(k), nb, ws->W, nb,
                                  0.0, A12, lda, true));
     *fast = false;
@@ -563,6 +573,8 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
   }
   c.ev_pairs   = 0;
   c.gemm_flops = 0.0;
+  c.swap_pairs = 0;
+  c.swap_bytes = 0.0;
   c.ev_kind.assign(3 * npanels, 0);
   c.ev_flops.assign(3 * npanels, 0.0);
   c.splits.clear();
@@ -938,6 +950,24 @@ int hplx_update_timing(hplx_ctx* ctx, int kind, double* ms, int* launches,
   if(ms) *ms = tot;
   if(launches) *launches = cnt;
   if(flops) *flops = f;
+  return HPLX_SUCCESS;
+}
+
+int hplx_swap_timing(hplx_ctx* ctx, double* ms, int* launches,
+                     double* bytes) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  double tot = 0.0;
+  for(int i = 0; i < c.swap_pairs; ++i) {
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c.ev_swap[2 * i], c.ev_swap[2 * i + 1]));
+    tot += t;
+  }
+  if(ms) *ms = tot;
+  if(launches) *launches = c.swap_pairs;
+  if(bytes) *bytes = c.swap_bytes;
   return HPLX_SUCCESS;
 }
 

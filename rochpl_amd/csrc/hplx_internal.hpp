@@ -88,6 +88,11 @@ struct Ctx {
   // kinds 2 and 3 filled from the claim log when the stats are read)
   std::vector<hipEvent_t> ev;
   int                     ev_pairs = 0;
+  // the FP64 loop's row interchanges (swap_gather launches): one event pair
+  // per launch and their algorithmic bytes (4 x 8 B per interchanged element)
+  std::vector<hipEvent_t> ev_swap;
+  int                     swap_pairs = 0;
+  double                  swap_bytes = 0.0;
   double                  gemm_flops = 0.0;
   std::vector<int>        ev_kind;
   std::vector<double>     ev_flops;
