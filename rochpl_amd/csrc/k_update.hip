@@ -230,20 +230,20 @@ __global__ __launch_bounds__(512, 2) void update_resume_kernel(
 // The claimed mode: a persistent grid whose workgroups take kClaimChunk
 // consecutive tiles per counter update (thread 0, after the epilogue of the
 // chunk's last tile) until the tiles run out or *stop becomes non-zero.  The
-// claim state lives in LDS.  The tile loop needs more than the 128 VGPRs of
-// update_kernel, so this kernel runs one workgroup per CU; it only covers
-// the part of an update that overlaps the panel factorisation, after which
-// update_resume_kernel finishes the update at full occupancy.
+// claim state lives in LDS.  Like update_kernel it is held to 128 VGPRs (four
+// waves per SIMD, two workgroups per CU); the tile loop then spills a few
+// registers once per tile, outside the K loop.
 struct UpdClaim {
   int      tile, end;  // current chunk [tile, end), tile < 0: none left
   unsigned dead;
 };
 
 // The launch also reserves the rest of the CU's LDS (dynamic shared memory it
-// does not use), so that no other kernel's workgroup that uses LDS (every
-// kernel of the panel factorisation does) is placed beside it: those run on
-// the vacated CUs only, instead of competing with the update for a SIMD.
-__global__ __launch_bounds__(512, 2) void update_claimed_kernel(
+// does not use; each of the two workgroups on a CU takes half), so that no
+// other kernel's workgroup that uses LDS (every kernel of the panel
+// factorisation does) is placed beside them: those run on the vacated CUs
+// only, instead of competing with the update for a SIMD.
+__global__ __launch_bounds__(512, 4) void update_claimed_kernel(
     int M, int N, int K, const double* __restrict__ A, int lda,
     const double* __restrict__ Bt, int ldb, double* __restrict__ C, int ldc,
     int tiles_m, int tiles_n, int a_rows, int b_cols, UpdSched sch) {
@@ -396,7 +396,7 @@ hipError_t launch_update_claimed(hipStream_t s, int m, int n, int k,
   sch.per    = static_cast<int>((g.T + kUXcds - 1) / kUXcds);
   sch.vacate = vacate;
   sch.stop   = stop;
-  // LDS left on a CU beside one workgroup of this kernel, reserved by it
+  // LDS left on a CU beside two workgroups of this kernel, reserved by them
   static int pad = -1;
   if(pad < 0) {
     int dev = 0, per_cu = 0;
@@ -409,7 +409,7 @@ hipError_t launch_update_claimed(hipStream_t s, int m, int n, int k,
       e = hipFuncGetAttributes(&fa,
                                reinterpret_cast<const void*>(update_claimed_kernel));
     if(e != hipSuccess) return e;
-    const int rest = per_cu - static_cast<int>(fa.sharedSizeBytes);
+    const int rest = per_cu / 2 - static_cast<int>(fa.sharedSizeBytes);
     if(rest > 0)
       e = hipFuncSetAttribute(reinterpret_cast<const void*>(update_claimed_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, rest);
