@@ -237,6 +237,7 @@ def run_fp64(ctx, rp, args):
         st = ctx.update_stats()
         kinds = {k: ctx.update_timing(k) for k in (1, 2, 3)}
         swp = ctx.swap_timing()
+        pk_runs = ctx.panel_kernel_runs()
         chk = ctx.pdtest_check(mat)
         last = time.monotonic() - s0
         log("fp64 %s step %d: %.3f s  %.2f TFLOP/s  resid %.4f  trailing GEMM "
@@ -321,7 +322,9 @@ def run_fp64(ctx, rp, args):
         "config": {"workload": "hpl_pdgesv_fp64_p1q1", "N": n, "NB": nb,
                    "P": 1, "Q": 1, "pfact": "right", "rfact": "right",
                    "nbmin": 32, "ndiv": 2, "depth": 1, "swap": "long",
-                   "frac": args.frac},
+                   "frac": args.frac,
+                   "panel": "persistent kernel (%d panels)" % pk_runs
+                            if args.panel_kernel else "launch sequence"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
@@ -769,6 +772,10 @@ def main():
     ap.add_argument("--check-pivots", type=int, default=0,
                     help="compare the first K pivots with LAPACK dgetrf on the "
                          "N x K column prefix")
+    ap.add_argument("--panel-kernel", action="store_true",
+                    help="factor each FP64 panel with the persistent panel "
+                         "kernel (HPLX_OPT_PANEL_KERNEL) instead of the "
+                         "launch sequence")
     ap.add_argument("--no-x-check", dest="x_check", action="store_false",
                     help="skip the FP64 solve that the bf16 form's x is "
                          "compared with")
@@ -817,6 +824,7 @@ def main():
     import rochpl_amd as rp
 
     ctx = rp.Context(0)
+    ctx.panel_kernel = args.panel_kernel
     out = {"metric": METRIC}
     # legs after the first run only while the wall-clock budget leaves them
     # their expected time; a skipped leg is named in the line

@@ -35,12 +35,13 @@ HPLX_SWAP00, HPLX_SWAP01, HPLX_SW_MIX, HPLX_NO_SWP = 451, 452, 453, 499
 HPLX_STORE_FP32, HPLX_STORE_FP16 = 32, 16
 HPLX_UPDATE_FP32, HPLX_UPDATE_BF16, HPLX_UPDATE_FP8 = 0, 1, 2
 HPLX_OPT_LOOKAHEAD = 1
+HPLX_OPT_PANEL_KERNEL = 2
 
 # Every symbol include/hplx.h declares (checked by tests/test_abi.py).
 EXPORTS = [
     "hplx_palg_default", "hplx_palg_check", "hplx_ctx_create", "hplx_ctx_destroy",
     "hplx_last_error", "hplx_set_stream", "hplx_synchronize", "hplx_num_cus",
-    "hplx_set_option", "hplx_get_option",
+    "hplx_set_option", "hplx_get_option", "hplx_panel_kernel_runs",
     "hplx_malloc", "hplx_free", "hplx_memcpy_h2d", "hplx_memcpy_d2h",
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
     "hplx_pdtest_check", "hplx_pdfact", "hplx_pdfact_palg", "hplx_swap_timing",
@@ -149,6 +150,7 @@ def lib():
         "hplx_num_cus": (i32, [vp]),
         "hplx_set_option": (i32, [vp, i32, i32]),
         "hplx_get_option": (i32, [vp, i32, P(i32)]),
+        "hplx_panel_kernel_runs": (i32, [vp]),
         "hplx_malloc": (i32, [vp, sz, P(vp)]),
         "hplx_free": (i32, [vp, vp]),
         "hplx_memcpy_h2d": (i32, [vp, vp, vp, sz]),
@@ -314,6 +316,26 @@ class Context:
     def lookahead(self, on: bool) -> None:
         self._chk(lib().hplx_set_option(self.h, HPLX_OPT_LOOKAHEAD,
                                         1 if on else 0))
+
+    @property
+    def panel_kernel(self) -> bool:
+        """HPLX_OPT_PANEL_KERNEL: each FP64 panel as one persistent kernel."""
+        v = ctypes.c_int()
+        self._chk(lib().hplx_get_option(self.h, HPLX_OPT_PANEL_KERNEL,
+                                        ctypes.byref(v)))
+        return bool(v.value)
+
+    @panel_kernel.setter
+    def panel_kernel(self, on: bool) -> None:
+        self._chk(lib().hplx_set_option(self.h, HPLX_OPT_PANEL_KERNEL,
+                                        1 if on else 0))
+
+    def panel_kernel_runs(self) -> int:
+        """Panels of the last factorisation that ran as the persistent
+        panel kernel (hplx_panel_kernel_runs)."""
+        n = lib().hplx_panel_kernel_runs(self.h)
+        self._chk(min(n, 0))
+        return n
 
     def alloc(self, nbytes: int) -> DeviceArray:
         return DeviceArray(self, nbytes)

@@ -226,6 +226,10 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
       return HPLX_ERR_HIP;
     }
     c.leaf_resident = per_cu * c.num_cus;
+    // the persistent panel kernel is optional: no residency, no use
+    int pk = 0;
+    if(hplx::panel_kernel_occupancy(&pk) != hipSuccess) pk = 0;
+    c.pk_resident = pk * c.num_cus;
   }
   if(const char* pe = std::getenv("HPLX_PROGRESS")) c.progress_every = std::atoi(pe);
   // small workspace
@@ -244,7 +248,9 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   size_t off_vacate = off_stop + 256;
   size_t off_census = align_up(off_vacate + hplx::kVacateSets * 8 * 8, 256);
   size_t off_clog   = align_up(off_census + kCensusGrid * sizeof(int), 256);
-  size_t total      = off_clog + 8 * hplx::kMaxClaimLog * sizeof(int);
+  size_t off_pops   = align_up(off_clog + 8 * hplx::kMaxClaimLog * sizeof(int), 256);
+  size_t off_pbar   = align_up(off_pops + 2 * hplx::kMaxPanelOps * sizeof(hplx::PanelOp), 256);
+  size_t total      = off_pbar + 256;
   auto undo = [&](int code) {
     if(c.small_base) (void)hipFree(c.small_base);
     if(c.ustream) (void)hipStreamDestroy(c.ustream);
@@ -270,10 +276,14 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
   c.vacate  = reinterpret_cast<unsigned long long*>(b + off_vacate);
   int* census = reinterpret_cast<int*>(b + off_census);
   c.claim_log = reinterpret_cast<int*>(b + off_clog);
+  c.pops      = reinterpret_cast<hplx::PanelOp*>(b + off_pops);
+  c.pbar      = reinterpret_cast<unsigned*>(b + off_pbar);
   if(hipMemset(c.small_base, 0, total) != hipSuccess ||
      hipDeviceSynchronize() != hipSuccess)
     return undo(HPLX_ERR_HIP);
   if(const char* la = std::getenv("HPLX_LOOKAHEAD")) c.lookahead = std::atoi(la) != 0;
+  if(const char* pk = std::getenv("HPLX_PANEL_KERNEL"))
+    c.panel_kernel = std::atoi(pk) != 0;
   if(cu_sets(c, census) != HPLX_SUCCESS) return undo(HPLX_ERR_HIP);
   *out = ctx;
   return HPLX_SUCCESS;
@@ -322,19 +332,25 @@ int hplx_num_cus(hplx_ctx* ctx) { return ctx ? ctx->c.num_cus : 0; }
 int hplx_set_option(hplx_ctx* ctx, int option, int value) {
   int r = use(ctx);
   if(r) return r;
-  if(option != HPLX_OPT_LOOKAHEAD || (value != 0 && value != 1))
+  if((option != HPLX_OPT_LOOKAHEAD && option != HPLX_OPT_PANEL_KERNEL) ||
+     (value != 0 && value != 1))
     return fail(ctx->c, HPLX_ERR_INVALID, "set_option: option %d value %d",
                 option, value);
-  ctx->c.lookahead = value;
+  (option == HPLX_OPT_LOOKAHEAD ? ctx->c.lookahead : ctx->c.panel_kernel) = value;
   return HPLX_SUCCESS;
 }
 
 int hplx_get_option(hplx_ctx* ctx, int option, int* value) {
   if(!ctx || !value) return HPLX_ERR_INVALID;
-  if(option != HPLX_OPT_LOOKAHEAD)
+  if(option != HPLX_OPT_LOOKAHEAD && option != HPLX_OPT_PANEL_KERNEL)
     return fail(ctx->c, HPLX_ERR_INVALID, "get_option: option %d", option);
-  *value = ctx->c.lookahead;
+  *value = option == HPLX_OPT_LOOKAHEAD ? ctx->c.lookahead : ctx->c.panel_kernel;
   return HPLX_SUCCESS;
+}
+
+int hplx_panel_kernel_runs(hplx_ctx* ctx) {
+  if(!ctx) return HPLX_ERR_INVALID;
+  return ctx->c.panel_kernel_runs;
 }
 
 int hplx_malloc(hplx_ctx* ctx, size_t bytes, void** dptr) {
@@ -579,6 +595,12 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
   c.ev_flops.assign(3 * npanels, 0.0);
   c.splits.clear();
   c.claim_host.clear();
+  c.panel_kernel_runs = 0;
+  if(c.panel_kernel) {  // the recorded steps of both panel widths, up front
+    int pr = hplx::panel_kernel_prepare(c, L.algo, N < nb ? N : nb);
+    if(!pr && N > nb && N % nb) pr = hplx::panel_kernel_prepare(c, L.algo, N % nb);
+    if(pr) return fail(c, pr, "persistent panel kernel: recording failed");
+  }
   hplx::Progress prog(c, npanels, nb, N);
   HIPCHK(c, prog.init());
 
@@ -627,7 +649,10 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
         HIPCHK(c, hipEventRecord(ev_prep, U));
         // panel k+1 on P, on the CUs the update leaves free
         HIPCHK(c, hipStreamWaitEvent(P, ev_prep, 0));
-        if((r = L.panel(k + 1, P))) return r;
+        c.panel_cus = 8 * rR;
+        r           = L.panel(k + 1, P);
+        c.panel_cus = 0;
+        if(r) return r;
         HIPCHK(c, hplx::launch_set_flag(P, stop));
         HIPCHK(c, hipEventRecord(ev_pan, P));
         const int m = N - j;
@@ -801,6 +826,9 @@ static int pdfact_algo(hplx_ctx* ctx, const hplx::PanelAlgo& algo, double* P,
     return fail(c, HPLX_ERR_INVALID, "pdfact: m=%d jb=%d lda=%d", m, jb, lda);
   if(jb > hplx::kMaxNB || m > hplx::max_leaf_rows(c))
     return fail(c, HPLX_ERR_UNSUPPORTED, "pdfact: m=%d jb=%d too large", m, jb);
+  c.panel_kernel_runs = 0;
+  if(c.panel_kernel && (r = hplx::panel_kernel_prepare(c, algo, jb)))
+    return fail(c, r, "persistent panel kernel: recording failed");
   HIPCHK(c, hplx::reset_panel_state(c));
   HIPCHK(c, hplx::panel_factor(c, algo, P, lda, m, jb, ipiv, row_offset));
   int flags[2] = {0, 0};

@@ -36,6 +36,23 @@ constexpr int kMaxGrid     = kMaxLeafGrid;  // candidate slots per parity
 constexpr int kMaxNB       = 1024;  // widest panel (pivot composition, Linv)
 constexpr int kSlotStride  = 72;    // doubles per pivot candidate slot
 
+// One step of a panel factorisation as the persistent panel kernel runs it
+// (k_panel_kernel.hip): the steps k_panel.hip's recursion would launch, in
+// its order, recorded once per (panel width, PanelAlgo) in panel-relative
+// coordinates.  kind: kPanelLeaf (columns [ac, ac + n) of rows [ac, m)),
+// kPanelTrsm (X (k x n at br, bc) = L^{-1} X, L unit lower at ar, ac),
+// kPanelGemm (C (rows x n at cr, cc) -= A (rows x k at ar, ac) B (k x n at
+// br, bc)), rows = m_fix if m_fix >= 0, else m - m_sub.
+enum : int { kPanelLeaf = 0, kPanelTrsm = 1, kPanelGemm = 2 };
+struct PanelOp {
+  int kind;
+  int m_fix, m_sub;
+  int n, k;
+  int ar, ac, br, bc, cr, cc;
+  int pad;
+};
+constexpr int kMaxPanelOps = 512;  // steps of one recorded panel
+
 // Per-context device state.  The small workspace (candidate slots, barrier
 // counter, flags, L11 inverse, composed pivots) lives with the context; the
 // n-sized workspace lives with the matrix (MatWs).
@@ -67,6 +84,25 @@ struct Ctx {
   // products use as scratch (null: stream / Linv).
   hipStream_t panel_stream_ = nullptr;
   double*     panel_scratch_ = nullptr;
+
+  // The persistent panel kernel (hplx_set_option HPLX_OPT_PANEL_KERNEL):
+  // panel_kernel 1 runs each FP64 panel as one launch.  panel_rec, while
+  // set, makes the recursion record its steps instead of launching them.
+  // Two recorded step lists live on the device (pops, kMaxPanelOps each),
+  // keyed by panel width and PanelAlgo; pbar is the kernel's grid barrier
+  // counter.
+  int                   panel_kernel = 0;
+  std::vector<PanelOp>* panel_rec    = nullptr;
+  PanelOp*              pops         = nullptr;
+  unsigned*             pbar         = nullptr;
+  int                   pop_key[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+  int                   pop_count[2]  = {0, 0};   // 0: slot empty
+  int                   pop_leaves[2] = {0, 0};
+  int                   pop_next      = 0;
+  std::vector<PanelOp>  pop_host[2];
+  int                   pk_resident   = 0;        // its workgroups resident at once
+  int                   panel_cus     = 0;        // CUs the next panel has (0: all)
+  int                   panel_kernel_runs = 0;    // panels of the last solve
 
   // Look-ahead (hplx_set_option HPLX_OPT_LOOKAHEAD): internal streams of the
   // update and the panel, and the CU sets a panel may be given: vacate set r
@@ -133,8 +169,10 @@ inline int max_leaf_rows(const Ctx& c) {
   return g * kLeafThreads * kLeafRPT;
 }
 
-// Workgroups of the leaf kernel one CU holds (occupancy query).
+// Workgroups of the leaf kernel (and of the persistent panel kernel) one CU
+// holds (occupancy query).
 hipError_t leaf_occupancy(int* blocks_per_cu);
+
 
 // Zero the flags (queued on c.stream).
 hipError_t reset_panel_state(Ctx& c);
@@ -173,6 +211,22 @@ hipError_t panel_factor(Ctx& c, const PanelAlgo& a, double* P, int lda,
 // The same recursion in FP32 (FP32 leaves, solves and MFMA GEMMs).
 hipError_t panel_factor(Ctx& c, const PanelAlgo& a, float* P, int lda,
                         int m, int jb, int* ipiv, int row_offset);
+
+// Record the steps of an FP64 panel of width jb under algorithm a and keep
+// them in one of the context's two device slots (synchronous; call it
+// before queuing the panels that use it, never while a panel kernel that
+// reads the slot being replaced may still run).  Returns HPLX_SUCCESS or a
+// negative HPLX_ERR_*; a panel that cannot run as one kernel (too many
+// steps) is left to the launch sequence and is not an error.
+int panel_kernel_prepare(Ctx& c, const PanelAlgo& a, int jb);
+// Workgroups of the persistent panel kernel one CU holds.
+hipError_t panel_kernel_occupancy(int* blocks_per_cu);
+// The panel from recorded slot `slot` as one launch on the panel stream;
+// *ran = false (nothing queued) when this panel does not qualify (grid, or
+// a product the streamed tile does not take), and the caller then queues
+// the launch sequence.
+hipError_t launch_panel_kernel(Ctx& c, int slot, double* P, int lda, int m,
+                               int jb, int* ipiv, int row_offset, bool* ran);
 
 hipError_t launch_pipid(hipStream_t s, int jb, const int* ipiv, int ipiv_base,
                         int* permU);
@@ -268,7 +322,7 @@ hipError_t launch_transpose(hipStream_t s, int rows, int cols, const double* A,
                             int lda, double* B, int ldb);
 // Leading dimension of the transposed U12 of n columns (even, with room for
 // the clamped pair reads).
-inline int ut_ld(int n) { return (n + 3) & ~1; }
+__host__ __device__ inline int ut_ld(int n) { return (n + 3) & ~1; }
 
 // C (m x n, ldc) -= A^T B for A (K x m, lda) and B (K x n, ldb) (the
 // transposed solve's U12^T z and L21^T z steps).  When the 64 x 64 tiles of

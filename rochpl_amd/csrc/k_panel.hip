@@ -255,6 +255,7 @@ hipError_t launch_leaf(Ctx& c, V* P, int lda, int m, int JB, int c0, int w,
   la.seq_base    = c.leaf_launches << 6;
   la.err         = c.flags + 1;
   la.rows_per_wg = CH;
+  la.grid        = G;
   // A plain launch: every workgroup is resident at once because the grid is
   // capped by the occupancy of this kernel (one workgroup per CU) and the
   // solve queues nothing else concurrently; the kernel's bounded waits turn
@@ -317,38 +318,59 @@ hipError_t chunk_loop(Ctx& c, int variant, V* P, int lda, int m, int c0,
   auto at = [&](int r, int col) {
     return P + r + static_cast<size_t>(col) * lda;
   };
-  hipStream_t s  = panel_stream(c);
-  int         jj = 0, jb = nb < n ? nb : n;
+  hipStream_t s = panel_stream(c);
+  // X (k x nx at xr, xc) = L^{-1} X for the unit lower L at (tr, tc)
+  auto trsm = [&](int k, int nx, int tr, int tc, int xr, int xc) {
+    if(c.panel_rec) {
+      PanelOp o{};
+      o.kind = kPanelTrsm;
+      o.k = k, o.n = nx, o.ar = tr, o.ac = tc, o.br = xr, o.bc = xc;
+      c.panel_rec->push_back(o);
+      return hipSuccess;
+    }
+    return launch_trsm_colsweep(s, k, nx, at(tr, tc), lda, at(xr, xc), lda);
+  };
+  // C -= A B with rows = m_fix (>= 0) or m - m_sub, n columns, depth k
+  auto gemm = [&](int m_fix, int m_sub, int nc, int k, int ar, int ac, int br,
+                  int bc, int cr, int cc) {
+    if(c.panel_rec) {
+      PanelOp o{};
+      o.kind = kPanelGemm;
+      o.m_fix = m_fix, o.m_sub = m_sub, o.n = nc, o.k = k;
+      o.ar = ar, o.ac = ac, o.br = br, o.bc = bc, o.cr = cr, o.cc = cc;
+      c.panel_rec->push_back(o);
+      return hipSuccess;
+    }
+    return panel_gemm(c, m_fix >= 0 ? m_fix : m - m_sub, nc, k, at(ar, ac),
+                      lda, at(br, bc), lda, at(cr, cc), lda);
+  };
+  int jj = 0, jb = nb < n ? nb : n;
   while(jj < n) {
     const int  top  = c0 + jj;
     const int  nrem = n - jj - jb;
     hipError_t e;
     if(jj > 0 && variant == HPLX_LEFT_LOOKING) {
-      e = launch_trsm_colsweep(s, jj, jb, at(c0, c0), lda, at(c0, top), lda);
+      e = trsm(jj, jb, c0, c0, c0, top);
       if(e != hipSuccess) return e;
     }
     if(jj > 0 && variant != HPLX_RIGHT_LOOKING) {
-      e = panel_gemm(c, m - top, jb, jj, at(top, c0), lda, at(c0, top), lda,
-                     at(top, top), lda);
+      e = gemm(-1, top, jb, jj, top, c0, c0, top, top, top);
       if(e != hipSuccess) return e;
     }
     e = factor(top, jb);
     if(e != hipSuccess) return e;
     if(nrem > 0 && variant == HPLX_RIGHT_LOOKING) {
-      e = launch_trsm_colsweep(s, jb, nrem, at(top, top), lda,
-                               at(top, top + jb), lda);
+      e = trsm(jb, nrem, top, top, top, top + jb);
       if(e != hipSuccess) return e;
-      e = panel_gemm(c, m - top - jb, nrem, jb, at(top + jb, top), lda,
-                     at(top, top + jb), lda, at(top + jb, top + jb), lda);
+      e = gemm(-1, top + jb, nrem, jb, top + jb, top, top, top + jb, top + jb,
+               top + jb);
       if(e != hipSuccess) return e;
     } else if(nrem > 0 && variant == HPLX_CROUT) {
       if(jj > 0) {
-        e = panel_gemm(c, jb, nrem, jj, at(top, c0), lda, at(c0, top + jb),
-                       lda, at(top, top + jb), lda);
+        e = gemm(jb, 0, nrem, jj, top, c0, c0, top + jb, top, top + jb);
         if(e != hipSuccess) return e;
       }
-      e = launch_trsm_colsweep(s, jb, nrem, at(top, top), lda,
-                               at(top, top + jb), lda);
+      e = trsm(jb, nrem, top, top, top, top + jb);
       if(e != hipSuccess) return e;
     }
     jj += jb;
@@ -370,6 +392,13 @@ hipError_t leaf_or_chunks(Ctx& c, int pfact, V* P, int lda, int m, int JB,
                           int c0, int n, int* ipiv, int row_offset,
                           int info_base) {
   auto leaf = [&](int top, int w) {
+    if(c.panel_rec) {
+      PanelOp o{};
+      o.kind = kPanelLeaf;
+      o.ac = top, o.n = w;
+      c.panel_rec->push_back(o);
+      return hipSuccess;
+    }
     return launch_leaf(c, P, lda, m, JB, top, w, ipiv, row_offset, info_base);
   };
   if(n <= kLeafMax) return leaf(c0, n);
@@ -391,18 +420,83 @@ hipError_t rpanel(Ctx& c, const PanelAlgo& a, V* P, int lda, int m, int JB,
   });
 }
 
+PanelAlgo normalized(PanelAlgo a) {
+  if(a.nbmin < 1) a.nbmin = 1;
+  if(a.nbdiv < 2) a.nbdiv = 2;
+  return a;
+}
+
+// The device slot holding the recorded steps of (jb, a), or -1.
+int recorded_slot(const Ctx& c, const PanelAlgo& a, int jb) {
+  const int key[5] = {jb, a.nbmin, a.nbdiv, a.pfact, a.rfact};
+  for(int s = 0; s < 2; ++s) {
+    if(!c.pop_count[s]) continue;
+    bool same = true;
+    for(int i = 0; i < 5; ++i) same = same && c.pop_key[s][i] == key[i];
+    if(same) return s;
+  }
+  return -1;
+}
+
 template <typename V>
 hipError_t panel_factor_t(Ctx& c, PanelAlgo a, V* P, int lda, int m, int jb,
                           int* ipiv, int row_offset) {
-  if(a.nbmin < 1) a.nbmin = 1;
-  if(a.nbdiv < 2) a.nbdiv = 2;
+  a = normalized(a);
   if(!valid_fact(a.pfact) || !valid_fact(a.rfact)) return hipErrorInvalidValue;
   if(jb > kMaxNB || jb > m) return hipErrorInvalidValue;
   if(m > max_leaf_rows(c)) return hipErrorInvalidValue;
+  if constexpr(sizeof(V) == 8) {
+    if(c.panel_kernel && !c.panel_rec) {
+      const int slot = recorded_slot(c, a, jb);
+      if(slot >= 0) {
+        bool       ran = false;
+        hipError_t e   = launch_panel_kernel(c, slot, P, lda, m, jb, ipiv,
+                                             row_offset, &ran);
+        if(e != hipSuccess) return e;
+        if(ran) {
+          ++c.panel_kernel_runs;
+          return hipSuccess;
+        }
+      }
+    }
+  }
   return rpanel(c, a, P, lda, m, jb, 0, jb, ipiv, row_offset, row_offset);
 }
 
 }  // namespace
+
+int panel_kernel_prepare(Ctx& c, const PanelAlgo& a0, int jb) {
+  PanelAlgo a = normalized(a0);
+  if(!valid_fact(a.pfact) || !valid_fact(a.rfact) || jb < 1 || jb > kMaxNB)
+    return HPLX_ERR_INVALID;
+  // a list in use stays: the next recording replaces the other slot
+  const int have = recorded_slot(c, a, jb);
+  if(have >= 0) {
+    c.pop_next = have ^ 1;
+    return HPLX_SUCCESS;
+  }
+  std::vector<PanelOp> ops;
+  c.panel_rec    = &ops;
+  hipError_t e   = rpanel(c, a, static_cast<double*>(nullptr), 0, jb, jb, 0,
+                          jb, nullptr, 0, 0);
+  c.panel_rec    = nullptr;
+  if(e != hipSuccess) return HPLX_ERR_HIP;
+  if(ops.size() > static_cast<size_t>(kMaxPanelOps)) return HPLX_SUCCESS;
+  const int slot = c.pop_next;
+  c.pop_next ^= 1;
+  c.pop_count[slot] = 0;
+  if(hipMemcpy(c.pops + static_cast<size_t>(slot) * kMaxPanelOps, ops.data(),
+               ops.size() * sizeof(PanelOp), hipMemcpyHostToDevice) != hipSuccess)
+    return HPLX_ERR_HIP;
+  int nleaf = 0;
+  for(const PanelOp& o : ops) nleaf += o.kind == kPanelLeaf;
+  const int key[5] = {jb, a.nbmin, a.nbdiv, a.pfact, a.rfact};
+  for(int i = 0; i < 5; ++i) c.pop_key[slot][i] = key[i];
+  c.pop_leaves[slot] = nleaf;
+  c.pop_count[slot]  = static_cast<int>(ops.size());
+  c.pop_host[slot]   = std::move(ops);
+  return HPLX_SUCCESS;
+}
 
 hipError_t leaf_occupancy(int* blocks_per_cu) {
   // the smaller of the FP64 and FP32 leaves' occupancies

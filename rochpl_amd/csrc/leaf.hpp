@@ -78,6 +78,7 @@ struct LeafArgs {
   uint32_t seq_base;   // unique per launch; column j publishes seq_base+j+1
   int*     err;
   int      rows_per_wg;  // kLeafThreads * RPT
+  int      grid;         // workgroups taking part: 0 .. grid-1 (<= gridDim.x)
 };
 
 // Shared-memory scratch of the leaf kernel (the exchange works in double;
@@ -216,7 +217,7 @@ __device__ __forceinline__ int exchange(const LeafArgs* ap, int jj,
   const int       t    = threadIdx.x;
   const int       lane = t & 63;
   const int       b    = blockIdx.x;
-  const int       G    = gridDim.x;
+  const int       G    = a.grid;
   const unsigned long long seq =
       static_cast<unsigned long long>(a.seq_base) + jj + 1;
   const double* slots = a.slots + static_cast<size_t>(jj & 1) * (kMaxGrid + 1) * kSlotStride;
@@ -299,7 +300,7 @@ __device__ __forceinline__ void leaf_step(const LeafArgs& a,
   // ---- steps 2-3
   int           p;
   const double* u;
-  if(gridDim.x == 1) {
+  if(a.grid == 1) {
 #pragma unroll
     for(int q = 0; q < RPT; ++q)
       if(rows[q] == br) {
@@ -387,20 +388,19 @@ struct LeafSteps<T, RPT, kLeafMax> {
                                              bool&) {}
 };
 
+// One leaf on workgroups 0 .. a.grid-1: the body of leaf_kernel, also run
+// by the persistent panel kernel (k_panel_kernel.hip) once per leaf.
+// Workgroups whose rows all lie at or below a.m take part in every exchange
+// with an empty candidate.
 template <typename T, typename S, int RPT>
-__global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
+__device__ __forceinline__ void leaf_run(const LeafArgs& a, LeafShared& sh) {
   constexpr int CH   = kLeafThreads * RPT;  // rows per workgroup
   const int     t    = threadIdx.x;
   const int     lane = t & 63;
   const int     wave = t >> 6;
   const int     b    = blockIdx.x;
-  const int     G    = gridDim.x;
+  const int     G    = a.grid;
   S*            P    = static_cast<S*>(a.P);
-
-  __shared__ LeafShared sh;
-
-  if(__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-    return;
 
   int rows[RPT];
   T   v[RPT][kLeafMax];
@@ -498,6 +498,14 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
       }
     }
   }
+}
+
+template <typename T, typename S, int RPT>
+__global__ __launch_bounds__(kLeafThreads) void leaf_kernel(LeafArgs a) {
+  __shared__ LeafShared sh;
+  if(__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    return;
+  leaf_run<T, S, RPT>(a, sh);
 }
 
 }  // namespace leaf

@@ -1,0 +1,172 @@
+"""The persistent panel kernel (HPLX_OPT_PANEL_KERNEL = 1): each FP64 panel
+factorisation as one launch that walks the recursion of HPL_pdrpan*T /
+HPL_pdpan*T (src/pfact/HPL_pdrpanrlT.cpp:19-161) with grid barriers between
+its steps.  Checked against the oracle's restatement of the same variant
+(pivots bit-exact, factors to rounding), against the launch sequence it
+replaces, and inside the whole solve with and without the look-ahead."""
+import numpy as np
+import pytest
+
+import rochpl_amd as rp
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+_FACT = {"L": rp.HPLX_LEFT_LOOKING, "C": rp.HPLX_CROUT,
+         "R": rp.HPLX_RIGHT_LOOKING}
+
+
+def say(msg):
+    print("[gpu-case] " + msg, flush=True)
+
+
+@pytest.fixture
+def pk(ctx):
+    ctx.panel_kernel = True
+    yield ctx
+    ctx.panel_kernel = False
+
+
+def _even_ld(m):
+    # the streamed product tile wants an even leading dimension
+    return m + 10 + (m & 1)
+
+
+def _pdfact(ctx, P0, m, jb, a):
+    lda = _even_ld(m)
+    buf = np.zeros((lda, jb), dtype=np.float64, order="F")
+    buf[:m] = P0
+    dP = ctx.alloc(buf.nbytes)
+    dP.upload(buf)
+    dpiv = ctx.alloc(4 * jb)
+    info = ctx.pdfact_palg(a, dP.ptr, lda, m, jb, dpiv.ptr, 0)
+    return (info, dP.download((lda, jb), np.float64)[:m],
+            dpiv.download(jb, np.int32))
+
+
+# 3001: odd panel height (the products' last row pair); 70000: one leaf over
+# 46 workgroups; 900 x 128 with nbmin 64: leaves blocked in 32-column
+# chunks; 5 x 3: a single workgroup
+@pytest.mark.parametrize("m,jb,nbmin,ndiv", [
+    (3000, 256, 32, 2), (3001, 256, 32, 2), (20000, 64, 32, 2),
+    (1500, 96, 16, 3), (900, 128, 64, 2), (5, 3, 32, 2), (70000, 32, 32, 2)])
+def test_panel_kernel_pdfact_matches_oracle(pk, m, jb, nbmin, ndiv):
+    say("panel kernel pdfact m=%d jb=%d nbmin=%d ndiv=%d" % (m, jb, nbmin, ndiv))
+    P0 = np.asfortranarray(o.matgen_tile(m + 11, 0, 0, m, jb, seed=m + jb))
+    a = rp.palg_default()
+    a.nbmin, a.nbdiv = nbmin, ndiv
+    info, got, gpiv = _pdfact(pk, P0, m, jb, a)
+    assert pk.panel_kernel_runs() == 1
+    P = P0.copy(order="F")
+    wpiv, winfo = o.pdfact(P, nbmin=nbmin, ndiv=ndiv)
+    assert info == winfo == 0
+    assert np.array_equal(gpiv, wpiv)
+    assert np.max(np.abs(got - P)) < 1e-11
+
+
+@pytest.mark.parametrize("pf", ["L", "C", "R"])
+@pytest.mark.parametrize("rf", ["L", "C", "R"])
+@pytest.mark.parametrize("m,jb,nbmin,ndiv", [
+    (3000, 256, 16, 2), (2000, 192, 48, 3)])
+def test_panel_kernel_variants_match_oracle(pk, pf, rf, m, jb, nbmin, ndiv):
+    say("panel kernel variant pfact=%s rfact=%s m=%d jb=%d nbmin=%d ndiv=%d"
+        % (pf, rf, m, jb, nbmin, ndiv))
+    P0 = np.asfortranarray(o.matgen_tile(m + 11, 0, 0, m, jb, seed=m + jb))
+    a = rp.palg_default()
+    a.pfact, a.rfact, a.nbmin, a.nbdiv = _FACT[pf], _FACT[rf], nbmin, ndiv
+    info, got, gpiv = _pdfact(pk, P0, m, jb, a)
+    assert pk.panel_kernel_runs() == 1
+    P = P0.copy(order="F")
+    wpiv, winfo = o.pdfact(P, nbmin=nbmin, ndiv=ndiv, pfact=_FACT[pf],
+                           rfact=_FACT[rf])
+    assert info == winfo == 0
+    assert np.array_equal(gpiv, wpiv)
+    assert np.max(np.abs(got - P)) < 1e-11
+
+
+@pytest.mark.parametrize("m,jb", [(12000, 512), (3001, 896)])
+def test_panel_kernel_against_launch_sequence(ctx, m, jb):
+    """The same panel factored both ways: equal pivots; the factors agree to
+    rounding (products under 128 columns take the generic GEMM in the launch
+    sequence and the streamed tile in the kernel)."""
+    say("panel kernel vs launches m=%d jb=%d" % (m, jb))
+    P0 = np.asfortranarray(o.matgen_tile(m + 11, 0, 0, m, jb, seed=7 * m + jb))
+    out = {}
+    try:
+        for on in (False, True):
+            ctx.panel_kernel = on
+            out[on] = _pdfact(ctx, P0, m, jb, rp.palg_default())
+            assert ctx.panel_kernel_runs() == (1 if on else 0)
+    finally:
+        ctx.panel_kernel = False
+    (i0, f0, p0), (i1, f1, p1) = out[False], out[True]
+    assert i0 == i1 == 0
+    assert np.array_equal(p0, p1)
+    d = np.max(np.abs(f0 - f1))
+    print("[panel-kernel] m=%d jb=%d max |diff| %.3e" % (m, jb, d), flush=True)
+    assert d < 1e-11
+
+
+@pytest.mark.parametrize("n,nb", [(4096, 256), (2000, 896), (1000, 64)])
+def test_panel_kernel_pdgesv_matches_oracle(pk, n, nb):
+    say("panel kernel pdgesv n=%d nb=%d" % (n, nb))
+    mat = pk.pmat_alloc(n, nb)
+    try:
+        pk.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+        assert pk.pdgesv(rp.palg_default(), mat) == 0
+        runs = pk.panel_kernel_runs()
+        x = pk.download_vec(mat.X, n)
+        ipiv = pk.download_vec(mat.ipiv, n, np.int32)
+        chk = pk.pdtest_check(mat)
+    finally:
+        pk.pmat_free(mat)
+    assert runs == (n + nb - 1) // nb
+    ref = o.pdgesv(n, nb)
+    assert np.array_equal(ipiv, ref["ipiv"])
+    assert np.max(np.abs(x - ref["x"])) < 1e-8 * np.abs(ref["x"]).max()
+    assert chk["resid"] < 16.0
+
+
+@pytest.mark.parametrize("n,nb", [(16384, 512), (9001, 256)])
+def test_panel_kernel_lookahead_is_bit_identical_to_serial(pk, n, nb):
+    """The persistent panel beside the claimed update (look-ahead) against
+    every step in order: identical pivots, factors and x; the pivots also
+    equal those of the launch sequence.  9001: a ragged last panel of 41
+    columns."""
+    say("panel kernel look-ahead vs serial n=%d nb=%d" % (n, nb))
+    res = {}
+    mat = pk.pmat_alloc(n, nb)
+    try:
+        for on, la in ((True, False), (True, True), (False, True)):
+            pk.panel_kernel = on
+            pk.lookahead = la
+            pk.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+            assert pk.pdgesv(rp.palg_default(), mat) == 0
+            res[(on, la)] = (pk.download_vec(mat.ipiv, n, np.int32),
+                             pk.download_vec(mat.X, n),
+                             pk.download_matrix(mat, rows=n, cols=n + 1),
+                             pk.pdtest_check(mat)["resid"],
+                             pk.panel_kernel_runs())
+    finally:
+        pk.lookahead = True
+        pk.pmat_free(mat)
+    (p0, x0, f0, r0, k0) = res[(True, False)]
+    (p1, x1, f1, r1, k1) = res[(True, True)]
+    (p2, _, _, r2, k2) = res[(False, True)]
+    assert k0 == k1 and k0 >= n // nb and k2 == 0
+    assert np.array_equal(p0, p1) and np.array_equal(p0, p2)
+    assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64))
+    assert np.array_equal(np.asarray(f0).view(np.uint64),
+                          np.asarray(f1).view(np.uint64))
+    assert r0 < 16.0 and r1 < 16.0 and r2 < 16.0
+    print("[panel-kernel] n=%d nb=%d look-ahead identical; resid %.4f"
+          % (n, nb, r1), flush=True)
+
+
+def test_panel_kernel_option_roundtrip(ctx):
+    assert ctx.panel_kernel is False
+    ctx.panel_kernel = True
+    assert ctx.panel_kernel is True
+    ctx.panel_kernel = False
+    assert rp.lib().hplx_set_option(ctx.h, rp.HPLX_OPT_PANEL_KERNEL, 2) == \
+        rp.HPLX_ERR_INVALID
