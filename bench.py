@@ -772,10 +772,11 @@ def main():
     ap.add_argument("--check-pivots", type=int, default=0,
                     help="compare the first K pivots with LAPACK dgetrf on the "
                          "N x K column prefix")
-    ap.add_argument("--panel-kernel", action="store_true",
-                    help="factor each FP64 panel with the persistent panel "
-                         "kernel (HPLX_OPT_PANEL_KERNEL) instead of the "
-                         "launch sequence")
+    ap.add_argument("--panel-launches", dest="panel_kernel",
+                    action="store_false",
+                    help="factor each FP64 panel as a sequence of launches "
+                         "instead of the persistent panel kernel "
+                         "(HPLX_OPT_PANEL_KERNEL = 0)")
     ap.add_argument("--no-x-check", dest="x_check", action="store_false",
                     help="skip the FP64 solve that the bf16 form's x is "
                          "compared with")

@@ -86,12 +86,12 @@ struct Ctx {
   double*     panel_scratch_ = nullptr;
 
   // The persistent panel kernel (hplx_set_option HPLX_OPT_PANEL_KERNEL):
-  // panel_kernel 1 runs each FP64 panel as one launch.  panel_rec, while
+  // panel_kernel 1 (default) runs each FP64 panel as one launch.  panel_rec, while
   // set, makes the recursion record its steps instead of launching them.
   // Two recorded step lists live on the device (pops, kMaxPanelOps each),
   // keyed by panel width and PanelAlgo; pbar is the kernel's grid barrier
   // counter.
-  int                   panel_kernel = 0;
+  int                   panel_kernel = 1;
   std::vector<PanelOp>* panel_rec    = nullptr;
   PanelOp*              pops         = nullptr;
   unsigned*             pbar         = nullptr;
