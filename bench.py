@@ -323,8 +323,9 @@ def run_fp64(ctx, rp, args):
                    "P": 1, "Q": 1, "pfact": "right", "rfact": "right",
                    "nbmin": 32, "ndiv": 2, "depth": 1, "swap": "long",
                    "frac": args.frac,
-                   "panel": "persistent kernel (%d panels)" % pk_runs
-                            if args.panel_kernel else "launch sequence"},
+                   "panel": "persistent kernel on %d of %d panels, the "
+                            "launch sequence on the rest (%s)"
+                            % (pk_runs, (n + nb - 1) // nb, args.panel)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
@@ -772,11 +773,12 @@ def main():
     ap.add_argument("--check-pivots", type=int, default=0,
                     help="compare the first K pivots with LAPACK dgetrf on the "
                          "N x K column prefix")
-    ap.add_argument("--panel-launches", dest="panel_kernel",
-                    action="store_false",
-                    help="factor each FP64 panel as a sequence of launches "
-                         "instead of the persistent panel kernel "
-                         "(HPLX_OPT_PANEL_KERNEL = 0)")
+    ap.add_argument("--panel", choices=["auto", "kernel", "launches"],
+                    default="auto",
+                    help="FP64 panels: the persistent panel kernel for "
+                         "panels of >= 65536 rows (auto, the library "
+                         "default), for every panel (kernel), or none "
+                         "(launches): HPLX_OPT_PANEL_KERNEL 2, 1, 0")
     ap.add_argument("--no-x-check", dest="x_check", action="store_false",
                     help="skip the FP64 solve that the bf16 form's x is "
                          "compared with")
@@ -825,7 +827,7 @@ def main():
     import rochpl_amd as rp
 
     ctx = rp.Context(0)
-    ctx.panel_kernel = args.panel_kernel
+    ctx.panel_kernel = {"auto": 2, "kernel": 1, "launches": 0}[args.panel]
     out = {"metric": METRIC}
     # legs after the first run only while the wall-clock budget leaves them
     # their expected time; a skipped leg is named in the line

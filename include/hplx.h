@@ -131,15 +131,16 @@ int         hplx_num_cus(hplx_ctx* ctx);
  * an unknown option or value; hplx_get_option writes the current value. */
 #define HPLX_OPT_LOOKAHEAD 1
 /* HPLX_OPT_PANEL_KERNEL: how an FP64 panel factorisation (HPL_pdfact with
- * the recursive HPL_pdrpan*T / HPL_pdpan*T of src/pfact/) runs.  1
- * (default): as one persistent kernel per panel that walks the recursion
- * with a grid barrier between its steps (a panel it cannot take, e.g. one
- * with more than 512 recursion steps, a product depth that is not a
- * multiple of 8 or an odd leading dimension, runs as the sequence).  0: as
- * a sequence of kernels, one per leaf, triangular solve and product of the
- * recursion.  Pivots and factors are the same bit for bit either way.  The
- * environment variable HPLX_PANEL_KERNEL=0|1 sets the default of new
- * contexts. */
+ * the recursive HPL_pdrpan*T / HPL_pdpan*T of src/pfact/) runs.  1: every
+ * panel as one persistent kernel that walks the recursion with a grid
+ * barrier between its steps (a panel it cannot take, e.g. one with more
+ * than 512 recursion steps, a product depth that is not a multiple of 8 or
+ * an odd leading dimension, runs as the sequence).  0: every panel as a
+ * sequence of kernels, one per leaf, triangular solve and product of the
+ * recursion.  2 (default): the persistent kernel for panels of at least
+ * 65536 rows, the sequence below (where it is faster).  Pivots and factors
+ * are the same bit for bit in all three.  The environment variable
+ * HPLX_PANEL_KERNEL=0|1|2 sets the default of new contexts. */
 #define HPLX_OPT_PANEL_KERNEL 2
 int         hplx_set_option(hplx_ctx* ctx, int option, int value);
 int         hplx_get_option(hplx_ctx* ctx, int option, int* value);

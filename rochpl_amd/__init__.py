@@ -318,17 +318,18 @@ class Context:
                                         1 if on else 0))
 
     @property
-    def panel_kernel(self) -> bool:
-        """HPLX_OPT_PANEL_KERNEL: each FP64 panel as one persistent kernel."""
+    def panel_kernel(self) -> int:
+        """HPLX_OPT_PANEL_KERNEL: 1 every FP64 panel as one persistent
+        kernel, 0 none (launch sequence), 2 panels of >= 65536 rows."""
         v = ctypes.c_int()
         self._chk(lib().hplx_get_option(self.h, HPLX_OPT_PANEL_KERNEL,
                                         ctypes.byref(v)))
-        return bool(v.value)
+        return v.value
 
     @panel_kernel.setter
-    def panel_kernel(self, on: bool) -> None:
+    def panel_kernel(self, mode) -> None:
         self._chk(lib().hplx_set_option(self.h, HPLX_OPT_PANEL_KERNEL,
-                                        1 if on else 0))
+                                        int(mode)))
 
     def panel_kernel_runs(self) -> int:
         """Panels of the last factorisation that ran as the persistent

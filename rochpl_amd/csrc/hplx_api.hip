@@ -282,8 +282,10 @@ int hplx_ctx_create(int device, hplx_ctx** out) {
      hipDeviceSynchronize() != hipSuccess)
     return undo(HPLX_ERR_HIP);
   if(const char* la = std::getenv("HPLX_LOOKAHEAD")) c.lookahead = std::atoi(la) != 0;
-  if(const char* pk = std::getenv("HPLX_PANEL_KERNEL"))
-    c.panel_kernel = std::atoi(pk) != 0;
+  if(const char* pk = std::getenv("HPLX_PANEL_KERNEL")) {
+    const int v = std::atoi(pk);
+    c.panel_kernel = v >= 0 && v <= 2 ? v : c.panel_kernel;
+  }
   if(cu_sets(c, census) != HPLX_SUCCESS) return undo(HPLX_ERR_HIP);
   *out = ctx;
   return HPLX_SUCCESS;
@@ -333,7 +335,7 @@ int hplx_set_option(hplx_ctx* ctx, int option, int value) {
   int r = use(ctx);
   if(r) return r;
   if((option != HPLX_OPT_LOOKAHEAD && option != HPLX_OPT_PANEL_KERNEL) ||
-     (value != 0 && value != 1))
+     value < 0 || value > (option == HPLX_OPT_PANEL_KERNEL ? 2 : 1))
     return fail(ctx->c, HPLX_ERR_INVALID, "set_option: option %d value %d",
                 option, value);
   (option == HPLX_OPT_LOOKAHEAD ? ctx->c.lookahead : ctx->c.panel_kernel) = value;

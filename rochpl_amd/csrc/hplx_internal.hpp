@@ -52,6 +52,11 @@ struct PanelOp {
   int pad;
 };
 constexpr int kMaxPanelOps = 512;  // steps of one recorded panel
+// Panel height from which HPLX_OPT_PANEL_KERNEL = 2 takes the persistent
+// kernel: below it the launch sequence is faster (measured: N = 16384 and
+// 32768 solves 21 % and 11 % slower with the kernel on every panel, 65536
+// even, 98304 and 193280 0.5 % faster).
+constexpr int kPanelKernelMinRows = 65536;
 
 // Per-context device state.  The small workspace (candidate slots, barrier
 // counter, flags, L11 inverse, composed pivots) lives with the context; the
@@ -86,12 +91,13 @@ struct Ctx {
   double*     panel_scratch_ = nullptr;
 
   // The persistent panel kernel (hplx_set_option HPLX_OPT_PANEL_KERNEL):
-  // panel_kernel 1 (default) runs each FP64 panel as one launch.  panel_rec, while
+  // panel_kernel 1 runs each FP64 panel as one launch, 2 (default) those of
+  // at least kPanelKernelMinRows rows, 0 none.  panel_rec, while
   // set, makes the recursion record its steps instead of launching them.
   // Two recorded step lists live on the device (pops, kMaxPanelOps each),
   // keyed by panel width and PanelAlgo; pbar is the kernel's grid barrier
   // counter.
-  int                   panel_kernel = 1;
+  int                   panel_kernel = 2;
   std::vector<PanelOp>* panel_rec    = nullptr;
   PanelOp*              pops         = nullptr;
   unsigned*             pbar         = nullptr;

@@ -446,7 +446,8 @@ hipError_t panel_factor_t(Ctx& c, PanelAlgo a, V* P, int lda, int m, int jb,
   if(jb > kMaxNB || jb > m) return hipErrorInvalidValue;
   if(m > max_leaf_rows(c)) return hipErrorInvalidValue;
   if constexpr(sizeof(V) == 8) {
-    if(c.panel_kernel && !c.panel_rec) {
+    if(!c.panel_rec && (c.panel_kernel == 1 ||
+                        (c.panel_kernel == 2 && m >= kPanelKernelMinRows))) {
       const int slot = recorded_slot(c, a, jb);
       if(slot >= 0) {
         bool       ran = false;

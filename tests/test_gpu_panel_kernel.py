@@ -22,8 +22,9 @@ def say(msg):
 
 @pytest.fixture
 def pk(ctx):
-    ctx.panel_kernel = True
+    ctx.panel_kernel = 1  # every panel
     yield ctx
+    ctx.panel_kernel = 2  # the default
 
 
 def _even_ld(m):
@@ -97,7 +98,7 @@ def test_panel_kernel_against_launch_sequence(ctx, m, jb):
             out[on] = _pdfact(ctx, P0, m, jb, rp.palg_default())
             assert ctx.panel_kernel_runs() == (1 if on else 0)
     finally:
-        ctx.panel_kernel = True
+        ctx.panel_kernel = 2
     (i0, f0, p0), (i1, f1, p1) = out[False], out[True]
     assert i0 == i1 == 0
     assert np.array_equal(p0, p1)
@@ -148,7 +149,6 @@ def test_panel_kernel_lookahead_is_bit_identical_to_serial(pk, n, nb):
                              pk.panel_kernel_runs())
     finally:
         pk.lookahead = True
-        pk.panel_kernel = True
         pk.pmat_free(mat)
     (p0, x0, f0, r0, k0) = res[(True, False)]
     (p1, x1, f1, r1, k1) = res[(True, True)]
@@ -164,9 +164,23 @@ def test_panel_kernel_lookahead_is_bit_identical_to_serial(pk, n, nb):
 
 
 def test_panel_kernel_option_roundtrip(ctx):
-    assert ctx.panel_kernel is True  # the default
-    ctx.panel_kernel = False
-    assert ctx.panel_kernel is False
-    ctx.panel_kernel = True
-    assert rp.lib().hplx_set_option(ctx.h, rp.HPLX_OPT_PANEL_KERNEL, 2) == \
-        rp.HPLX_ERR_INVALID
+    assert ctx.panel_kernel == 2  # the default: panels of >= 65536 rows
+    for v in (0, 1, 2):
+        ctx.panel_kernel = v
+        assert ctx.panel_kernel == v
+    for bad in (-1, 3):
+        assert rp.lib().hplx_set_option(ctx.h, rp.HPLX_OPT_PANEL_KERNEL,
+                                        bad) == rp.HPLX_ERR_INVALID
+    assert ctx.panel_kernel == 2
+
+
+def test_panel_kernel_default_takes_tall_panels_only(ctx):
+    """The default (2): a panel of 65536 rows or more runs as the kernel,
+    a shorter one as the launch sequence; both give the same bits."""
+    out = {}
+    for m in (65535, 65536):
+        say("panel kernel default m=%d" % m)
+        P0 = np.asfortranarray(o.matgen_tile(m + 11, 0, 0, m, 64, seed=m))
+        out[m] = _pdfact(ctx, P0, m, 64, rp.palg_default())
+        assert out[m][0] == 0
+        assert ctx.panel_kernel_runs() == (1 if m >= 65536 else 0)
