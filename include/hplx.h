@@ -309,6 +309,13 @@ int hplx_update_timing(hplx_ctx* ctx, int kind, double* ms, int* launches,
  * counterpart in the reference, which times none. */
 int hplx_swap_timing(hplx_ctx* ctx, double* ms, int* launches, double* bytes);
 
+/* Timing of the panel factorisations (HPL_pdfact, src/pfact/HPL_pdfact.cpp)
+ * in the last hplx_pdgesv / hplx_pdgetrf: summed time (ms, HIP events
+ * around each panel on the stream it runs on, beside the update with the
+ * look-ahead), panels, and their algorithmic bytes (each m x jb panel read
+ * and written once, 16 B per element).  No counterpart in the reference. */
+int hplx_panel_timing(hplx_ctx* ctx, double* ms, int* panels, double* bytes);
+
 /* ======================================================================
  * Mixed precision (no counterpart in the reference, which is FP64 only).
  *

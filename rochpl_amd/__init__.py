@@ -45,6 +45,7 @@ EXPORTS = [
     "hplx_malloc", "hplx_free", "hplx_memcpy_h2d", "hplx_memcpy_d2h",
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
     "hplx_pdtest_check", "hplx_pdfact", "hplx_pdfact_palg", "hplx_swap_timing",
+    "hplx_panel_timing",
     "hplx_pipid", "hplx_dlaswp00N",
     "hplx_pdupdate", "hplx_dgemm", "hplx_pdtrsv", "hplx_update_stats",
     "hplx_update_timing",
@@ -163,6 +164,7 @@ def lib():
         "hplx_pdfact": (i32, [vp, i32, i32, vp, i32, i32, i32, vp, i32]),
         "hplx_pdfact_palg": (i32, [vp, vp, vp, i32, i32, i32, vp, i32]),
         "hplx_swap_timing": (i32, [vp, P(f64), P(i32), P(f64)]),
+        "hplx_panel_timing": (i32, [vp, P(f64), P(i32), P(f64)]),
         "hplx_pipid": (i32, [vp, i32, vp, vp]),
         "hplx_dlaswp00N": (i32, [vp, i32, i32, vp, i32, vp]),
         "hplx_pdupdate": (i32, [vp, vp, i32, i32, i32, i32, vp, vp]),
@@ -439,6 +441,14 @@ class Context:
         ms, cnt, b = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
         self._chk(lib().hplx_swap_timing(self.h, ctypes.byref(ms),
                                          ctypes.byref(cnt), ctypes.byref(b)))
+        return ms.value, cnt.value, b.value
+
+    def panel_timing(self):
+        """(ms, panels, algorithmic bytes) of the last FP64 solve's panel
+        factorisations (hplx_panel_timing)."""
+        ms, cnt, b = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        self._chk(lib().hplx_panel_timing(self.h, ctypes.byref(ms),
+                                          ctypes.byref(cnt), ctypes.byref(b)))
         return ms.value, cnt.value, b.value
 
     def update_timing(self, kind: int):

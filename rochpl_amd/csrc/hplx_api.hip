@@ -301,6 +301,8 @@ int hplx_ctx_destroy(hplx_ctx* ctx) {
   for(hipEvent_t e : c.ev) (void)hipEventDestroy(e);
   for(hipEvent_t e : c.ev_prog) (void)hipEventDestroy(e);
   for(hipEvent_t e : c.ev_sync) (void)hipEventDestroy(e);
+  for(hipEvent_t e : c.ev_swap) (void)hipEventDestroy(e);
+  for(hipEvent_t e : c.ev_panel) (void)hipEventDestroy(e);
   if(c.small_base) (void)hipFree(c.small_base);
   if(c.ustream) (void)hipStreamDestroy(c.ustream);
   if(c.pstream) (void)hipStreamDestroy(c.pstream);
@@ -482,11 +484,21 @@ struct Fp64Loop {
     double*   P = A + j + static_cast<size_t>(j) * lda;
     c.panel_stream_  = s;
     c.panel_scratch_ = Linv(k);
+    // the factorisation, timed by its own event pair
+    while(static_cast<int>(c.ev_panel.size()) < 2 * (c.panel_pairs + 1)) {
+      hipEvent_t ev;
+      HIPCHK(c, hipEventCreate(&ev));
+      c.ev_panel.push_back(ev);
+    }
+    HIPCHK(c, hipEventRecord(c.ev_panel[2 * c.panel_pairs], s));
     const hipError_t e = hplx::panel_factor(c, algo, P, lda, N - j, jb,
                                             mat->ipiv + j, j);
     c.panel_stream_  = nullptr;
     c.panel_scratch_ = nullptr;
     HIPCHK(c, e);
+    HIPCHK(c, hipEventRecord(c.ev_panel[2 * c.panel_pairs + 1], s));
+    ++c.panel_pairs;
+    c.panel_bytes += 16.0 * (N - j) * static_cast<double>(jb);
     HIPCHK(c, hplx::launch_pipid(s, jb, mat->ipiv + j, j, permU(k)));
     HIPCHK(c, hplx::launch_linv(s, jb, P, lda, Linv(k), nb));
     return HPLX_SUCCESS;
@@ -593,6 +605,8 @@ static int factor_loop(Ctx& c, const hplx_palg* palg, hplx_pmat* mat,
   c.gemm_flops = 0.0;
   c.swap_pairs = 0;
   c.swap_bytes = 0.0;
+  c.panel_pairs = 0;
+  c.panel_bytes = 0.0;
   c.ev_kind.assign(3 * npanels, 0);
   c.ev_flops.assign(3 * npanels, 0.0);
   c.splits.clear();
@@ -998,6 +1012,24 @@ int hplx_swap_timing(hplx_ctx* ctx, double* ms, int* launches,
   if(ms) *ms = tot;
   if(launches) *launches = c.swap_pairs;
   if(bytes) *bytes = c.swap_bytes;
+  return HPLX_SUCCESS;
+}
+
+int hplx_panel_timing(hplx_ctx* ctx, double* ms, int* panels,
+                      double* bytes) {
+  int r = use(ctx);
+  if(r) return r;
+  Ctx& c = ctx->c;
+  HIPCHK(c, hipStreamSynchronize(c.stream));
+  double tot = 0.0;
+  for(int i = 0; i < c.panel_pairs; ++i) {
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c.ev_panel[2 * i], c.ev_panel[2 * i + 1]));
+    tot += t;
+  }
+  if(ms) *ms = tot;
+  if(panels) *panels = c.panel_pairs;
+  if(bytes) *bytes = c.panel_bytes;
   return HPLX_SUCCESS;
 }
 

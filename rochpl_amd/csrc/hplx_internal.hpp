@@ -54,8 +54,8 @@ struct PanelOp {
 constexpr int kMaxPanelOps = 512;  // steps of one recorded panel
 // Panel height from which HPLX_OPT_PANEL_KERNEL = 2 takes the persistent
 // kernel: below it the launch sequence is faster (measured: N = 16384 and
-// 32768 solves 21 % and 11 % slower with the kernel on every panel, 65536
-// even, 98304 and 193280 0.5 % faster).
+// 32768 solves 21 % and 11 % slower with the kernel on every panel; 65536,
+// 98304 and 193280 within run-to-run spread).
 constexpr int kPanelKernelMinRows = 65536;
 
 // Per-context device state.  The small workspace (candidate slots, barrier
@@ -135,6 +135,12 @@ struct Ctx {
   std::vector<hipEvent_t> ev_swap;
   int                     swap_pairs = 0;
   double                  swap_bytes = 0.0;
+  // the FP64 loop's panel factorisations: one event pair per panel (on the
+  // stream it runs on) and their algorithmic bytes (the m x jb panel read
+  // and written once: 16 B per element)
+  std::vector<hipEvent_t> ev_panel;
+  int                     panel_pairs = 0;
+  double                  panel_bytes = 0.0;
   double                  gemm_flops = 0.0;
   std::vector<int>        ev_kind;
   std::vector<double>     ev_flops;
