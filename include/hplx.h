@@ -148,6 +148,19 @@ int         hplx_get_option(hplx_ctx* ctx, int option, int* value);
  * ran as the persistent panel kernel (HPLX_OPT_PANEL_KERNEL); no
  * counterpart in the reference. */
 int         hplx_panel_kernel_runs(hplx_ctx* ctx);
+/* The steps the persistent panel kernel walks for a panel of width jb under
+ * palg's NBMIN, NDIV, PFACT and RFACT: the recursion of HPL_pdrpan*T /
+ * HPL_pdpan*T (src/pfact/HPL_pdrpanrlT.cpp:93-160) recorded on the host, no
+ * device call.  11 ints per step: kind (0 leaf, 1 triangular solve, 2
+ * product), m_fix, m_sub, n, k, ar, ac, br, bc, cr, cc (panel-relative
+ * coordinates; a leaf factors columns [ac, ac + n); a solve sets the k x n
+ * block at (br, bc) to L^-1 times it, L the unit lower triangle at (ar, ac);
+ * a product subtracts A (rows x k at ar, ac) times B (k x n at br, bc) from
+ * C (rows x n at cr, cc), rows = m_fix if m_fix >= 0, else m - m_sub).
+ * Writes at most max_steps steps; returns the step count or a negative
+ * HPLX_ERR_*.  No counterpart in the reference. */
+int         hplx_panel_steps(const hplx_palg* palg, int jb, int* steps,
+                             int max_steps);
 
 /* ---- plain device memory helpers (for callers without a runtime of their
  * own, e.g. ctypes tests) ------------------------------------------------- */

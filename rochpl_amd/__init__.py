@@ -42,6 +42,7 @@ EXPORTS = [
     "hplx_palg_default", "hplx_palg_check", "hplx_ctx_create", "hplx_ctx_destroy",
     "hplx_last_error", "hplx_set_stream", "hplx_synchronize", "hplx_num_cus",
     "hplx_set_option", "hplx_get_option", "hplx_panel_kernel_runs",
+    "hplx_panel_steps",
     "hplx_malloc", "hplx_free", "hplx_memcpy_h2d", "hplx_memcpy_d2h",
     "hplx_pmat_alloc", "hplx_pmat_free", "hplx_pdrandmat", "hplx_pdgesv",
     "hplx_pdtest_check", "hplx_pdfact", "hplx_pdfact_palg", "hplx_swap_timing",
@@ -152,6 +153,7 @@ def lib():
         "hplx_set_option": (i32, [vp, i32, i32]),
         "hplx_get_option": (i32, [vp, i32, P(i32)]),
         "hplx_panel_kernel_runs": (i32, [vp]),
+        "hplx_panel_steps": (i32, [P(PAlg), i32, vp, i32]),
         "hplx_malloc": (i32, [vp, sz, P(vp)]),
         "hplx_free": (i32, [vp, vp]),
         "hplx_memcpy_h2d": (i32, [vp, vp, vp, sz]),
@@ -203,6 +205,25 @@ def palg_default() -> PAlg:
     p = PAlg()
     lib().hplx_palg_default(ctypes.byref(p))
     return p
+
+
+PANEL_LEAF, PANEL_TRSM, PANEL_GEMM = 0, 1, 2
+
+
+def panel_steps(palg: PAlg, jb: int):
+    """The steps the persistent panel kernel walks for a panel of width jb
+    (hplx_panel_steps): an (n, 11) int array of (kind, m_fix, m_sub, n, k,
+    ar, ac, br, bc, cr, cc).  Host only, no device needed."""
+    import numpy as np
+    n = lib().hplx_panel_steps(ctypes.byref(palg), jb, None, 0)
+    if n < 0:
+        raise HplxError(n, "hplx_panel_steps: invalid arguments")
+    out = np.zeros((max(n, 1), 11), dtype=np.int32)
+    got = lib().hplx_panel_steps(ctypes.byref(palg), jb,
+                                 out.ctypes.data_as(ctypes.c_void_p), n)
+    if got != n:
+        raise HplxError(min(got, -1), "hplx_panel_steps: changed size")
+    return out[:n]
 
 
 def mxp_opts_default(update: int | None = None,

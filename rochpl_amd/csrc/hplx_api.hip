@@ -352,6 +352,12 @@ int hplx_get_option(hplx_ctx* ctx, int option, int* value) {
   return HPLX_SUCCESS;
 }
 
+int hplx_panel_steps(const hplx_palg* palg, int jb, int* steps,
+                     int max_steps) {
+  if(!palg) return HPLX_ERR_INVALID;
+  return hplx::panel_steps(hplx::panel_algo(*palg), jb, steps, max_steps);
+}
+
 int hplx_panel_kernel_runs(hplx_ctx* ctx) {
   if(!ctx) return HPLX_ERR_INVALID;
   return ctx->c.panel_kernel_runs;

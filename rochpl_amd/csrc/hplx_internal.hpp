@@ -231,6 +231,12 @@ hipError_t panel_factor(Ctx& c, const PanelAlgo& a, float* P, int lda,
 // negative HPLX_ERR_*; a panel that cannot run as one kernel (too many
 // steps) is left to the launch sequence and is not an error.
 int panel_kernel_prepare(Ctx& c, const PanelAlgo& a, int jb);
+// The recorded steps of a panel of width jb under a (host only, no context):
+// kPanelStepInts ints per step (kind, m_fix, m_sub, n, k, ar, ac, br, bc,
+// cr, cc) into out, at most max_steps of them; returns the step count or a
+// negative HPLX_ERR_*.
+constexpr int kPanelStepInts = 11;
+int panel_steps(const PanelAlgo& a, int jb, int* out, int max_steps);
 // Workgroups of the persistent panel kernel one CU holds.
 hipError_t panel_kernel_occupancy(int* blocks_per_cu);
 // The panel from recorded slot `slot` as one launch on the panel stream;

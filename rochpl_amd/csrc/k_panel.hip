@@ -466,6 +466,17 @@ hipError_t panel_factor_t(Ctx& c, PanelAlgo a, V* P, int lda, int m, int jb,
 
 }  // namespace
 
+// The steps of a panel of width jb under a (host only: in recording mode the
+// recursion queues nothing).
+int record_panel_steps(Ctx& c, const PanelAlgo& a, int jb,
+                       std::vector<PanelOp>* ops) {
+  c.panel_rec  = ops;
+  hipError_t e = rpanel(c, a, static_cast<double*>(nullptr), 0, jb, jb, 0, jb,
+                        nullptr, 0, 0);
+  c.panel_rec  = nullptr;
+  return e == hipSuccess ? HPLX_SUCCESS : HPLX_ERR_HIP;
+}
+
 int panel_kernel_prepare(Ctx& c, const PanelAlgo& a0, int jb) {
   PanelAlgo a = normalized(a0);
   if(!valid_fact(a.pfact) || !valid_fact(a.rfact) || jb < 1 || jb > kMaxNB)
@@ -477,11 +488,7 @@ int panel_kernel_prepare(Ctx& c, const PanelAlgo& a0, int jb) {
     return HPLX_SUCCESS;
   }
   std::vector<PanelOp> ops;
-  c.panel_rec    = &ops;
-  hipError_t e   = rpanel(c, a, static_cast<double*>(nullptr), 0, jb, jb, 0,
-                          jb, nullptr, 0, 0);
-  c.panel_rec    = nullptr;
-  if(e != hipSuccess) return HPLX_ERR_HIP;
+  if(int r = record_panel_steps(c, a, jb, &ops)) return r;
   if(ops.size() > static_cast<size_t>(kMaxPanelOps)) return HPLX_SUCCESS;
   const int slot = c.pop_next;
   c.pop_next ^= 1;
@@ -497,6 +504,24 @@ int panel_kernel_prepare(Ctx& c, const PanelAlgo& a0, int jb) {
   c.pop_count[slot]  = static_cast<int>(ops.size());
   c.pop_host[slot]   = std::move(ops);
   return HPLX_SUCCESS;
+}
+
+int panel_steps(const PanelAlgo& a0, int jb, int* out, int max_steps) {
+  const PanelAlgo a = normalized(a0);
+  if(!valid_fact(a.pfact) || !valid_fact(a.rfact) || jb < 1 || jb > kMaxNB ||
+     max_steps < 0 || (max_steps > 0 && !out))
+    return HPLX_ERR_INVALID;
+  Ctx                  c;  // recording only: no device state is touched
+  std::vector<PanelOp> ops;
+  if(int r = record_panel_steps(c, a, jb, &ops)) return r;
+  const int n = static_cast<int>(ops.size());
+  for(int i = 0; i < n && i < max_steps; ++i) {
+    const PanelOp& o = ops[i];
+    const int      v[kPanelStepInts] = {o.kind, o.m_fix, o.m_sub, o.n, o.k, o.ar,
+                                        o.ac, o.br, o.bc, o.cr, o.cc};
+    for(int f = 0; f < kPanelStepInts; ++f) out[i * kPanelStepInts + f] = v[f];
+  }
+  return n;
 }
 
 hipError_t leaf_occupancy(int* blocks_per_cu) {
