@@ -102,8 +102,7 @@ __global__ __launch_bounds__(kLeafThreads) void panel_kernel(PkArgs a) {
   unsigned  nbar   = 0;
   uint32_t  leaf_i = 0;
   for(int i = 0; i < a.nops; ++i) {
-    // the step's fields as wave-uniform scalars (SGPRs), so the leaf keeps
-    // every VGPR it has in its own kernel
+    // the step's fields as wave-uniform scalars (SGPRs), not VGPRs
     PanelOp o;
     {
       const int* src = reinterpret_cast<const int*>(a.ops + i);
