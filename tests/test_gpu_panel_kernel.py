@@ -163,6 +163,40 @@ def test_panel_kernel_lookahead_is_bit_identical_to_serial(pk, n, nb):
           % (n, nb, r1), flush=True)
 
 
+def test_panel_kernel_default_mixes_forms_bit_identically(ctx):
+    """The default (2) at N = 66560, NB = 512: the first panels (66560,
+    66048 and 65536 rows) run as the kernel, the rest as the launch sequence.  Against
+    the launch sequence throughout, and against the serial schedule: the same
+    pivots and the same x, bit for bit; the residual passes."""
+    n, nb = 66560, 512
+    say("panel kernel default, mixed forms n=%d nb=%d" % (n, nb))
+    res = {}
+    mat = ctx.pmat_alloc(n, nb)
+    try:
+        for mode, la in ((2, True), (0, True), (2, False)):
+            ctx.panel_kernel = mode
+            ctx.lookahead = la
+            ctx.pdrandmat(n, n + 1, nb, mat.A, mat.ld, rp.HPL_ISEED, 0.0)
+            assert ctx.pdgesv(rp.palg_default(), mat) == 0
+            res[(mode, la)] = (ctx.download_vec(mat.ipiv, n, np.int32),
+                               ctx.download_vec(mat.X, n),
+                               ctx.pdtest_check(mat)["resid"],
+                               ctx.panel_kernel_runs())
+    finally:
+        ctx.panel_kernel = 2
+        ctx.lookahead = True
+        ctx.pmat_free(mat)
+    (p0, x0, r0, k0) = res[(2, True)]
+    (p1, x1, r1, k1) = res[(0, True)]
+    (p2, x2, r2, k2) = res[(2, False)]
+    tall = sum(1 for k in range((n + nb - 1) // nb) if n - k * nb >= 65536)
+    assert k0 == k2 == tall == 3 and k1 == 0
+    assert np.array_equal(p0, p1) and np.array_equal(p0, p2)
+    assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64))
+    assert np.array_equal(x0.view(np.uint64), x2.view(np.uint64))
+    assert max(r0, r1, r2) < 16.0
+
+
 def test_panel_kernel_option_roundtrip(ctx):
     assert ctx.panel_kernel == 2  # the default: panels of >= 65536 rows
     for v in (0, 1, 2):
