@@ -197,6 +197,41 @@ def test_panel_kernel_default_mixes_forms_bit_identically(ctx):
     assert max(r0, r1, r2) < 16.0
 
 
+@pytest.mark.parametrize("case", ["many_steps", "odd_ld", "odd_width"])
+def test_panel_kernel_falls_back_to_the_launch_sequence(pk, case):
+    """Panels the kernel does not take run as the launch sequence, with the
+    oracle's pivots: more than 512 recursion steps (NBMIN 1 on 600
+    columns), an odd leading dimension (the streamed product's pairs), a
+    product depth that is not a multiple of 8 (41 columns at NBMIN 20: a
+    product of depth 20)."""
+    m, jb, nbmin = {"many_steps": (2000, 600, 1), "odd_ld": (3000, 256, 32),
+                    "odd_width": (3000, 41, 20)}[case]
+    say("panel kernel fallback %s m=%d jb=%d nbmin=%d" % (case, m, jb, nbmin))
+    a = rp.palg_default()
+    a.nbmin = nbmin
+    steps = rp.panel_steps(a, jb)
+    if case == "many_steps":
+        assert len(steps) > 512
+    if case == "odd_width":
+        assert any(st[0] == rp.PANEL_GEMM and st[4] % 8 for st in steps)
+    P0 = np.asfortranarray(o.matgen_tile(m + 11, 0, 0, m, jb, seed=m + jb))
+    lda = m + 11 if case == "odd_ld" else _even_ld(m)
+    assert (lda % 2 == 1) == (case == "odd_ld")
+    buf = np.zeros((lda, jb), dtype=np.float64, order="F")
+    buf[:m] = P0
+    dP = pk.alloc(buf.nbytes)
+    dP.upload(buf)
+    dpiv = pk.alloc(4 * jb)
+    info = pk.pdfact_palg(a, dP.ptr, lda, m, jb, dpiv.ptr, 0)
+    assert pk.panel_kernel_runs() == 0
+    got = dP.download((lda, jb), np.float64)[:m]
+    P = P0.copy(order="F")
+    wpiv, winfo = o.pdfact(P, nbmin=nbmin, ndiv=2)
+    assert info == winfo == 0
+    assert np.array_equal(dpiv.download(jb, np.int32), wpiv)
+    assert np.max(np.abs(got - P)) < 1e-11
+
+
 def test_panel_kernel_option_roundtrip(ctx):
     assert ctx.panel_kernel == 2  # the default: panels of >= 65536 rows
     for v in (0, 1, 2):
